@@ -1,0 +1,496 @@
+"""CPU oracle for the DMRG effective-Hamiltonian hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import this module, and only as the checker (or as the timed CPU baseline).  The
+product path (``mpsdyn_amd``) never calls it and fails loudly when its HIP library is
+missing.
+
+It restates, in numpy, the algorithm of the reference library (mcbapo/mpsdyn) for the
+path that the device library implements.  Tensors are numpy arrays indexed exactly like
+the reference's ``mwArray`` (first index first); every reshape is column-major
+(``order='F'``), as ``mwArray`` stores its data (wrapper/mwArray.cpp:452-466), and
+``permute`` takes 1-based orders like ``Indices`` (wrapper/mwArray.cpp:646-724).
+
+Conventions (reference layout, column-major):
+  site tensor   A[s, a_l, a_r]            (core/headers/Site.h:49)
+  MPO tensor    W[s_up, w_l, s_down, w_r] (core/headers/Operator.h:37-43)
+  environments  L[b, alpha, a], R[b, beta, a]  (bra, MPO, ket)
+
+Pinning (how each function is checked; tests/test_oracle.py):
+  heff_product      -- against heff_full applied to unit vectors, against the explicit
+                       projection P^H H P of a dense Hamiltonian built from Pauli
+                       Kronecker products (independent of the MPO path), Hermiticity,
+                       linearity, and 7-fold explicit loops on tiny shapes.
+  contract_left/right -- env of the full chain against <psi|H|psi> from the dense
+                       Hamiltonian; explicit loops on tiny shapes.
+  heff_full         -- against heff_product on unit vectors and the dense projection.
+  *_mpo builders    -- MPO contracted to a dense matrix against the Pauli-sum Hamiltonian.
+  find_ground_state -- reference energies recorded in tests/golden (config 1 value from
+                       the compiled reference), exact diagonalisation and the open-chain
+                       free-fermion closed form.
+  permute/reshape   -- the known answers of the reference's own unit test
+                       (src/tests/testmwArray_new.cpp:92-170, 176-246).
+  random_mps (seeded)-- parity unpinned: the reference draws libc random(); energies do not
+                       depend on the start state, which is what the tests compare.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+__all__ = [
+    "reshape_f", "permute", "heff_product", "heff_full", "contract_left", "contract_right",
+    "ising_mpo", "heisenberg_mpo", "schwinger_mpo", "random_mps", "gauge_right_all",
+    "gauge_left_all", "find_ground_state", "complex_gaussian", "splitmix64",
+    "matvec_flops", "to_colmajor", "from_colmajor",
+]
+
+
+# --------------------------------------------------------------------------- layout helpers
+
+def reshape_f(a, dims):
+    """Column-major reshape, as mwArray::reshape (wrapper/mwArray.cpp:556-583)."""
+    return np.reshape(a, dims, order="F")
+
+
+def permute(a, order):
+    """mwArray::permute with a 1-based order (wrapper/mwArray.cpp:646-724):
+    new index i is old index order[i]-1."""
+    return np.transpose(a, [o - 1 for o in order])
+
+
+def to_colmajor(a):
+    """Flat complex128 buffer in mwArray storage order (interleaved re/im, column-major)."""
+    return np.ascontiguousarray(np.reshape(np.asarray(a, dtype=np.complex128), -1, order="F"))
+
+
+def from_colmajor(buf, dims):
+    return reshape_f(np.asarray(buf, dtype=np.complex128), tuple(dims))
+
+
+# --------------------------------------------------------------------------- RNG (project's)
+
+_MASK = (1 << 64) - 1
+
+
+def splitmix64(seed, n):
+    """n successive SplitMix64 outputs (uint64) for a seed.  Pure integer arithmetic, so
+    the C++ generator in the device library reproduces it bit for bit."""
+    out = np.empty(n, dtype=np.uint64)
+    state = np.uint64(seed & _MASK)
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = state + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        out[:] = z ^ (z >> np.uint64(31))
+    return out
+
+
+def complex_gaussian(seed, shape):
+    """Complex tensor whose re and im parts are N(0,1): SplitMix64 uniforms (53-bit) fed
+    to Box--Muller.  Values are laid out column-major over ``shape`` (the i-th draw is the
+    i-th element of the mwArray buffer)."""
+    n = int(np.prod(shape))
+    u = splitmix64(seed, 2 * n)
+    u1 = ((u[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+    u2 = (u[1::2] >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    r = np.sqrt(-2.0 * np.log(u1))
+    re = r * np.cos(2.0 * np.pi * u2)
+    im = r * np.sin(2.0 * np.pi * u2)
+    return reshape_f(re + 1j * im, shape)
+
+
+# --------------------------------------------------------------------------- the hot path
+
+def heff_product(L, W, R, x, offset=0.0):
+    """y = Heff x (+ offset x).  Restates TensorMultiplier::TensorMultiplier
+    (src/wrapper/multipliers/TensorMultiplier.cpp:8-21) and ::product (:61-93), and the
+    offset of TensorMultiplierOffset::product (TensorMultiplierOffset.cpp:14-17).
+    ``x`` is the flat column-major vector of the site tensor [dd, betak, betakr]; the
+    result is flat over [du, betab, betabr]."""
+    betab, alpha, betak = L.shape
+    du, al1, dd, al2 = W.shape
+    betabr, alphar, betakr = R.shape
+    AL = reshape_f(L, (betab * alpha, betak))
+    Aop = reshape_f(permute(W, (1, 4, 2, 3)), (du * al2, al1 * dd))
+    AR = permute(reshape_f(R, (betabr, alphar * betakr)), (2, 1))
+    x = np.asarray(x, dtype=np.complex128).reshape(-1, order="F")
+    aux = reshape_f(x, (dd, betak, betakr))
+    aux = reshape_f(permute(aux, (2, 1, 3)), (betak, dd * betakr))
+    aux = AL @ aux                                    # stage 1: L . x
+    aux = reshape_f(aux, (betab, alpha * dd, betakr))
+    aux = reshape_f(permute(aux, (2, 1, 3)), (alpha * dd, betab * betakr))
+    aux = Aop @ aux                                   # stage 2: W
+    aux = reshape_f(aux, (du, al2, betab, betakr))
+    aux = reshape_f(permute(aux, (1, 3, 2, 4)), (du * betab, al2 * betakr))
+    aux = aux @ AR                                    # stage 3: . R
+    y = reshape_f(aux, (-1,))
+    if offset != 0.0:
+        y = y + offset * x
+    return y
+
+
+def heff_full(L, W, R, offset=0.0):
+    """Dense Heff with rows (du, betab, betabr) and columns (dd, betak, betakr).
+    Restates TensorMultiplier::getFullTensor (TensorMultiplier.cpp:102-117) and the offset
+    of TensorMultiplierOffset::getFullTensor (TensorMultiplierOffset.cpp:21-25)."""
+    betab, alpha, betak = L.shape
+    du, al1, dd, al2 = W.shape
+    betabr, alphar, betakr = R.shape
+    AL = reshape_f(L, (betab * alpha, betak))
+    Aop = reshape_f(permute(W, (1, 4, 2, 3)), (du * al2, al1 * dd))
+    AR = permute(reshape_f(R, (betabr, alphar * betakr)), (2, 1))
+    res = reshape_f(permute(reshape_f(AL, (betab, alpha, betak)), (1, 3, 2)), (betab * betak, alpha))
+    tmp = reshape_f(permute(reshape_f(Aop, (du, al2, al1, dd)), (3, 1, 4, 2)), (al1, du * dd * al2))
+    res = reshape_f(res @ tmp, (betab * betak * du * dd, al2))
+    tmp = reshape_f(AR, (alphar, betakr * betabr))
+    res = reshape_f(res @ tmp, (betab, betak, du, dd, betakr, betabr))
+    res = reshape_f(permute(res, (3, 1, 6, 4, 2, 5)), (du * betab * betabr, dd * betak * betakr))
+    if offset != 0.0:
+        res = res + offset * np.eye(res.shape[0])
+    return res
+
+
+def contract_left(L, ket, W, bra):
+    """New left environment L'[b_r, w_r, a_r].  Restates Operator::contractleftop
+    (src/core/cpp/Operator.cpp:273-322), the order after 'better order of contractions'."""
+    d, Dl, Dr = bra.shape
+    dp, Dlp, Drp = ket.shape
+    betab, alpha, betak = L.shape
+    du, al1, dd, al2 = W.shape
+    res = reshape_f(permute(ket, (2, 1, 3)), (Dlp, dp * Drp))
+    res = reshape_f(L, (betab * alpha, betak)) @ res
+    res = reshape_f(res, (betab, alpha * dp, Drp))
+    res = reshape_f(permute(res, (2, 1, 3)), (alpha * dp, betab * Drp))
+    wm = reshape_f(permute(reshape_f(W, (du, al1 * dd, al2)), (1, 3, 2)), (du * al2, al1 * dd))
+    res = wm @ res
+    res = reshape_f(res, (du, al2, betab, Drp))
+    res = reshape_f(permute(res, (1, 3, 2, 4)), (du * betab, al2 * Drp))
+    res = reshape_f(bra, (d * Dl, Dr)).conj().T @ res
+    return reshape_f(res, (Dr, al2, Drp))
+
+
+def contract_right(R, ket, W, bra):
+    """New right environment R'[b_l, w_l, a_l].  Restates Operator::contractrightop
+    (src/core/cpp/Operator.cpp:336-387)."""
+    d, Dl, Dr = bra.shape
+    dp, Dlp, Drp = ket.shape
+    betab, alpha, betak = R.shape
+    d1, al1, d2, al2 = W.shape
+    res = reshape_f(permute(ket, (3, 1, 2)), (Drp, dp * Dlp))
+    res = reshape_f(R, (betab * alpha, betak)) @ res
+    res = reshape_f(res, (betab, alpha, dp, Dlp))
+    res = reshape_f(permute(res, (3, 2, 1, 4)), (dp * alpha, betab * Dlp))
+    res = reshape_f(W, (d1 * al1, d2 * al2)) @ res
+    res = reshape_f(res, (d1, al1, betab, Dlp))
+    res = reshape_f(permute(res, (1, 3, 2, 4)), (d1 * betab, al1 * Dlp))
+    res = reshape_f(permute(bra.conj(), (2, 1, 3)), (Dl, d * Dr)) @ res
+    return reshape_f(res, (Dl, al1, Dlp))
+
+
+def matvec_flops(Dl, Dr, wl, wr, d=2):
+    """Algorithmic real flops of one matvec (or env update), complex MAC = 8 flops."""
+    return 8 * (d * wl * Dl * Dl * Dr + d * d * wl * wr * Dl * Dr + d * wr * Dl * Dr * Dr)
+
+
+# --------------------------------------------------------------------------- model MPOs
+
+_I2 = np.eye(2, dtype=np.complex128)
+_SX = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+_SY = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
+_SZ = np.array([[1, 0], [0, -1]], dtype=np.complex128)
+
+
+def _w_from_c(C, ops):
+    """W[du, Dl, dd, Dr] = sum_o C[Dl, Dr, o] op_o[du, dd]: the reshape C*Z then
+    permute(3,1,4,2) of IsingHamiltonian.cpp:110-115 (same step in the other models)."""
+    return np.einsum("lro,oud->uldr", C, np.stack(ops))
+
+
+def ising_mpo(N, J, g, h=0.0, K=0.0):
+    """H = sum J sz sz + g sx + h sz (+K).  Restates IsingHamiltonian::setHMPO
+    (src/models/cpp/IsingHamiltonian.cpp:70-121) with Z = {1, sz, sx} (:51-68); a negative
+    J enters as i*sqrt|J| on both sides (:84)."""
+    ops = [_I2, _SZ, _SX]
+    effJ = 1j * math.sqrt(abs(J)) if J < 0 else math.sqrt(J)
+    mpo = []
+    for k in range(N):
+        Dl = 1 if k == 0 else 3
+        Dr = 1 if k == N - 1 else 3
+        C = np.zeros((Dl, Dr, 3), dtype=np.complex128)
+        if k != N - 1:
+            C[0, 0, 0] = 1
+        if k != 0:
+            C[Dl - 1, Dr - 1, 0] = 1
+        C[0, Dr - 1, 2] = g
+        C[0, Dr - 1, 1] = h
+        C[0, Dr - 1, 0] = K / N
+        if k != N - 1:
+            C[0, 1, 1] = effJ
+        if k != 0:
+            C[1, Dr - 1, 1] = effJ
+        mpo.append(_w_from_c(C, ops))
+    return mpo
+
+
+def heisenberg_mpo(L, Jx, Jy, Jz, h=0.0, offset=0.0):
+    """H = sum Jx SxSx + Jy SySy + Jz SzSz + h Sz with S = sigma/2 (+offset).  Restates
+    HeisenbergHamiltonian::initZ/initHMPO (src/models/cpp/HeisenbergHamiltonian.cpp:266-331)
+    without the penalty term; couplings may be scalars or per-bond sequences."""
+    ops = [_I2, 0.5 * _SX, 0.5 * _SY, 0.5 * _SZ]
+    jx = np.broadcast_to(np.asarray(Jx, dtype=float), (max(L - 1, 1),))
+    jy = np.broadcast_to(np.asarray(Jy, dtype=float), (max(L - 1, 1),))
+    jz = np.broadcast_to(np.asarray(Jz, dtype=float), (max(L - 1, 1),))
+    hh = np.broadcast_to(np.asarray(h, dtype=float), (L,))
+    mpo = []
+    for k in range(L):
+        Dl = 1 if k == 0 else 5
+        Dr = 1 if k == L - 1 else 5
+        C = np.zeros((Dl, Dr, 4), dtype=np.complex128)
+        if k != L - 1:
+            C[0, 0, 0] = 1
+            C[0, 1, 1] = jx[k]
+            C[0, 2, 2] = jy[k]
+            C[0, 3, 3] = jz[k]
+        if k != 0:
+            C[Dl - 1, Dr - 1, 0] = 1
+            C[1, Dr - 1, 1] = 1
+            C[2, Dr - 1, 2] = 1
+            C[3, Dr - 1, 3] = 1
+        C[0, Dr - 1, 3] = hh[k]
+        if offset != 0:
+            C[0, Dr - 1, 0] = offset / L
+        mpo.append(_w_from_c(C, ops))
+    return mpo
+
+
+def schwinger_mpo(N, mu, x, l0=0.0, alpha=0.0, offset=0.0, nu=0.0, y=1.0):
+    """Schwinger model (staggered fermions, gauge field integrated out).  Restates
+    SchwingerHamiltonian::initZ/initHMPO (src/models/cpp/SchwingerHamiltonian.cpp:23-121);
+    mu = 2 (m/g) sqrt(x) as in src/programs/banulsm/schwinger/schwingerPD.cpp:51."""
+    ops = [_I2, _SX, _SY, _SZ]
+    a_ = alpha + l0
+    alphas = np.array([a_ - 0.25 * (1.0 - (1.0 if (k + 1) % 2 == 0 else -1.0)) for k in range(N)])
+    sumA = np.zeros(N)
+    tot = 0.0
+    for k in range(N - 2, -1, -1):
+        tot += alphas[k]
+        sumA[k] = tot
+    mpo = []
+    for k in range(N):
+        Dl = 1 if k == 0 else 5
+        Dr = 1 if k == N - 1 else 5
+        C = np.zeros((Dl, Dr, 4), dtype=np.complex128)
+        if k != N - 1:
+            C[0, 0, 0] = 1
+        if k != 0:
+            C[4, Dr - 1, 0] = 1
+        if 0 < k < N - 2:
+            C[3, 3, 0] = 1
+        ak = alphas[k]
+        sg = 1.0 if (k + 1) % 2 == 0 else -1.0
+        if k != N - 1:
+            C[0, Dr - 1, 0] = y * ak * ak + y * (k + 1) / 4.0 + mu / 2.0 + nu / 2.0 + offset / N
+            C[0, Dr - 1, 3] = sg * mu / 2.0 + nu / 2.0 + y * sumA[k]
+        else:
+            C[0, Dr - 1, 0] = mu / 2.0 + nu / 2.0 + offset / N
+            C[0, Dr - 1, 3] = sg * mu / 2.0 + nu / 2.0
+        if k != N - 1:
+            C[0, 1, 1] = math.sqrt(x / 2)
+            C[0, 2, 2] = math.sqrt(x / 2)
+            if k < N - 2:
+                C[0, 3, 3] = y
+            if k != 0:
+                C[1, 4, 1] = math.sqrt(x / 2)
+                C[2, 4, 2] = math.sqrt(x / 2)
+                C[3, 4, 3] = (N - k - 1) / 2.0
+        else:
+            C[1, 0, 1] = math.sqrt(x / 2)
+            C[2, 0, 2] = math.sqrt(x / 2)
+        mpo.append(_w_from_c(C, ops))
+    return mpo
+
+
+# --------------------------------------------------------------------------- MPS and gauge
+
+def bond_dims(L, D, d=2):
+    """Bond profile after MPS(L,D,d) and gaugeCond('R'),('L') (MPS.cpp:15-35, 306-364):
+    min(d^k, D, d^(L-k)) for k = 0..L."""
+    return [min(d ** min(k, 62), D, d ** min(L - k, 62)) for k in range(L + 1)]
+
+
+def random_mps(L, D, d=2, seed=1):
+    """Seeded complex-Gaussian MPS with the canonical bond profile; site k draws from seed
+    seed*1000003 + k.  The reference fills uniform [0,1) values from libc random()
+    (wrapper/mwArray.cpp:1595-1602): parity unpinned, only energies are compared."""
+    dims = bond_dims(L, D, d)
+    return [complex_gaussian(seed * 1000003 + k, (d, dims[k], dims[k + 1])) for k in range(L)]
+
+
+def _qr_site(A):
+    """Site::gaugeR (Site.cpp:179-208): QR of A as (d*Dl, Dr); returns (Q-site, R)."""
+    d, Dl, Dr = A.shape
+    q, r = np.linalg.qr(reshape_f(A, (d * Dl, Dr)))
+    return reshape_f(q, (d, Dl, q.shape[1])), r
+
+
+def _lq_site(A):
+    """Site::gaugeL (Site.cpp:210-238): LQ of A permuted to (Dl, d*Dr)."""
+    d, Dl, Dr = A.shape
+    m = reshape_f(permute(A, (2, 1, 3)), (Dl, d * Dr))
+    q, r = np.linalg.qr(m.conj().T)             # m = r^H q^H = L Q
+    Lm, Q = r.conj().T, q.conj().T
+    return permute(reshape_f(Q, (Q.shape[0], d, Dr)), (2, 1, 3)), Lm
+
+
+def _absorb_right(A, R):
+    """Site::reduceDimR (Site.cpp:240-261): A'[s, :, a_r] = R . A[s, :, a_r]."""
+    return np.einsum("ij,sjk->sik", R, A)
+
+
+def _absorb_left(A, Lm):
+    """Site::reduceDimL (Site.cpp:263-283): A'[s, a_l, :] = A[s, a_l, :] . L."""
+    return np.einsum("sij,jk->sik", A, Lm)
+
+
+def gauge_right_all(mps):
+    """MPS::gaugeCond('R', normal) (MPS.cpp:306-337): left-canonical sites 0..L-1, norm
+    dropped at the end."""
+    mps = [a.copy() for a in mps]
+    for k in range(len(mps)):
+        q, r = _qr_site(mps[k])
+        mps[k] = q
+        if k + 1 < len(mps):
+            mps[k + 1] = _absorb_right(mps[k + 1], r)
+        else:
+            mps[k] = mps[k] * (r[0, 0] / abs(r[0, 0]))
+    return mps
+
+
+def gauge_left_all(mps):
+    """MPS::gaugeCond('L', normal) (MPS.cpp:310-321): right-canonical sites L-1..0."""
+    mps = [a.copy() for a in mps]
+    for k in range(len(mps) - 1, -1, -1):
+        q, lm = _lq_site(mps[k])
+        mps[k] = q
+        if k > 0:
+            mps[k - 1] = _absorb_left(mps[k - 1], lm)
+        else:
+            mps[k] = mps[k] * (lm[0, 0] / abs(lm[0, 0]))
+    return mps
+
+
+# --------------------------------------------------------------------------- the sweep
+
+def _local_ground(L, W, R, x0, offset, dense_below=801):
+    n = x0.size
+    if n < dense_below:
+        # Contractor::solveEigs small-size switch (Contractor.cpp:3126-3133)
+        H = heff_full(L, W, R, offset)
+        H = 0.5 * (H + H.conj().T)
+        vals, vecs = np.linalg.eigh(H)
+        return float(vals[0]), vecs[:, 0]
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((n, n), matvec=lambda v: heff_product(L, W, R, v, offset), dtype=np.complex128)
+    vals, vecs = eigsh(op, k=1, which="SA", v0=x0, tol=0.0)
+    return float(vals[0]), vecs[:, 0]
+
+
+def environments(mps, mpo):
+    """All right environments of a right-canonical MPS plus the trivial left edge:
+    TmpManager edges are the scalar 1 (TmpManager.cpp:44-54), the rest follow
+    Contractor::calculateR (Contractor.cpp:744-764)."""
+    L = len(mps)
+    one = np.ones((1, 1, 1), dtype=np.complex128)
+    envL = [None] * L
+    envR = [None] * L
+    envL[0] = one
+    envR[L - 1] = one
+    for k in range(L - 2, -1, -1):
+        envR[k] = contract_right(envR[k + 1], mps[k + 1], mpo[k + 1], mps[k + 1])
+    return envL, envR
+
+
+def expectation(mps, mpo):
+    """<psi|H|psi> via a full left-environment sweep (Contractor::contract, Contractor.cpp:528)."""
+    e = np.ones((1, 1, 1), dtype=np.complex128)
+    for k in range(len(mps)):
+        e = contract_left(e, mps[k], mpo[k], mps[k])
+    return complex(e.reshape(-1)[0])
+
+
+def find_ground_state(mpo, mps, convtol=1e-12, offset=0.0, max_halfsweeps=200, dense_below=801):
+    """Single-site ground-state sweep.  Restates Contractor::findGroundState
+    (src/core/cpp/Contractor.cpp:1352-1533): right-canonicalise, sweep 0..L-1, L-2..0, ...
+    solving the lowest eigenpair of Heff at each site (warm start from the current site),
+    gauge-moving the centre with QR/LQ (MPS::gaugeCond(pos,dir,true), MPS.cpp:366-419) and
+    stopping when |E_prev - E| < convtol |E + offset| at the end of a half-sweep.
+    Returns (energy, mps, n_halfsweeps, energies_per_halfsweep)."""
+    Ls = len(mps)
+    mps = gauge_left_all(gauge_right_all(mps))
+    envL, envR = environments(mps, mpo)
+    energy = expectation(mps, mpo).real
+    val = energy
+    pos, right, done, rounds = 0, True, False, 0
+    history = []
+    while not done:
+        if pos > 0 and envL[pos] is None:
+            for k in range(1, pos + 1):
+                if envL[k] is None:
+                    envL[k] = contract_left(envL[k - 1], mps[k - 1], mpo[k - 1], mps[k - 1])
+        if pos < Ls - 1 and envR[pos] is None:
+            for k in range(Ls - 2, pos - 1, -1):
+                if envR[k] is None:
+                    envR[k] = contract_right(envR[k + 1], mps[k + 1], mpo[k + 1], mps[k + 1])
+        A = mps[pos]
+        val, vec = _local_ground(envL[pos], mpo[pos], envR[pos], to_colmajor(A), offset, dense_below)
+        val -= offset
+        mps[pos] = from_colmajor(vec, A.shape)
+        if right:
+            q, r = _qr_site(mps[pos])
+            mps[pos] = q
+            if pos + 1 < Ls:
+                mps[pos + 1] = _absorb_right(mps[pos + 1], r)
+            else:
+                mps[pos] = q * (r[0, 0] / abs(r[0, 0]))
+        else:
+            q, lm = _lq_site(mps[pos])
+            mps[pos] = q
+            if pos > 0:
+                mps[pos - 1] = _absorb_left(mps[pos - 1], lm)
+            else:
+                mps[pos] = q * (lm[0, 0] / abs(lm[0, 0]))
+        for l in range(pos + 1, Ls):
+            envL[l] = None
+        for l in range(pos - 1, -1, -1):
+            envR[l] = None
+        if right:
+            if pos == Ls - 1:
+                right = False
+                done = abs(energy - val) < convtol * abs(energy + offset) or rounds + 1 >= max_halfsweeps
+                energy = val
+                history.append(val)
+                if not done:
+                    q, lm = _lq_site(mps[pos])
+                    mps[pos] = q
+                    mps[pos - 1] = _absorb_left(mps[pos - 1], lm)
+                    pos -= 1
+                rounds += 1
+            else:
+                pos += 1
+        else:
+            if pos == 0:
+                right = True
+                done = abs(energy - val) < convtol * abs(energy + offset) or rounds + 1 >= max_halfsweeps
+                energy = val
+                history.append(val)
+                if not done:
+                    q, r = _qr_site(mps[pos])
+                    mps[pos] = q
+                    mps[pos + 1] = _absorb_right(mps[pos + 1], r)
+                    pos += 1
+                rounds += 1
+            else:
+                pos -= 1
+    return energy, mps, rounds, history
