@@ -1,0 +1,139 @@
+/*
+ * mpsd.h -- C-ABI of the MI355X device library for the DMRG effective-Hamiltonian hot path.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types cross this boundary.
+ * All tensors are complex128 stored as interleaved (re, im) double pairs in COLUMN-MAJOR
+ * order (first index fastest), bit-compatible with the reference's mwArray buffers
+ * (src/wrapper/mwArray.h:29-31, src/wrapper/mwArray.cpp:452-466):
+ *   site tensor    A[d, Dl, Dr]                 (src/core/headers/Site.h:49)
+ *   MPO tensor     W[d_up, wl, d_down, wr]      (src/core/headers/Operator.h:37-43)
+ *   environment    E[D_bra, w, D_ket]           (TmpManager operL/operR, TmpManager.h:39-52)
+ *
+ * Every function returns MPSD_OK (0) or a negative MPSD_ERR_* code; the message of the
+ * last failure on a context is available from mpsd_last_error().  The reference exits the
+ * process on the same conditions (exit(212) for shape errors, Operator.cpp:283-289).
+ *
+ * Threading: a context owns one HIP stream and its scratch memory and is not re-entrant.
+ * Concurrent solves use one context each; device tensors may be shared between contexts
+ * of the same device when the caller orders the work (mpsd_ctx_sync).
+ */
+#ifndef MPSD_H
+#define MPSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPSD_OK 0
+#define MPSD_ERR_ARG (-1)      /* null pointer, bad rank or size */
+#define MPSD_ERR_SHAPE (-2)    /* operand dimensions do not chain */
+#define MPSD_ERR_HIP (-3)      /* HIP runtime or device failure */
+#define MPSD_ERR_NOMEM (-4)    /* device allocation failed */
+#define MPSD_ERR_NUMERIC (-5)  /* NaN/Inf or breakdown detected */
+#define MPSD_ERR_UNSUPPORTED (-6)
+
+/* Arithmetic of a matvec.  FP64 is the drop-in; the others run the two GEMM stages in
+ * lower precision with FP32 accumulation (the eigensolver around them stays FP64). */
+#define MPSD_PREC_FP64 0
+#define MPSD_PREC_FP32 1    /* complex FP32 MFMA, FP32 accumulate */
+#define MPSD_PREC_BF16X3 2  /* split-BF16 (hi/lo, 3 products) MFMA, FP32 accumulate */
+
+typedef struct mpsd_ctx mpsd_ctx;
+typedef struct mpsd_tensor mpsd_tensor;
+typedef struct mpsd_heff mpsd_heff;
+
+/* ---- context --------------------------------------------------------------------------
+ * Replaces the process-wide state of the reference (the Contractor singleton,
+ * src/core/cpp/Contractor.cpp:49-53) with an explicit per-device, per-stream handle. */
+int mpsd_ctx_create(int device, mpsd_ctx** out);
+int mpsd_ctx_destroy(mpsd_ctx* ctx);
+int mpsd_ctx_sync(mpsd_ctx* ctx);
+const char* mpsd_last_error(const mpsd_ctx* ctx);
+/* The HIP stream (hipStream_t) the context launches on, as an opaque pointer. */
+void* mpsd_ctx_stream(mpsd_ctx* ctx);
+/* Device memory currently held by the context's tensors and scratch, in bytes. */
+int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx);
+
+/* ---- timing ---------------------------------------------------------------------------
+ * HIP events around every kernel launch of the context, accumulated per kernel kind.  The
+ * reference's equivalent is the -DTESTINGTIMES gettimeofday instrumentation of
+ * TensorMultiplier::product (src/wrapper/multipliers/TensorMultiplier.cpp:57-99).
+ * set_timing(on) also resets the totals; timing() synchronises the stream first. */
+#define MPSD_KERNEL_STAGE1 0      /* environment x site GEMM (first stage, both sides) */
+#define MPSD_KERNEL_MIX 1         /* MPO contraction of the middle stage */
+#define MPSD_KERNEL_HEFF_STAGE3 2 /* last stage of the matvec */
+#define MPSD_KERNEL_ENV_STAGE3 3  /* last stage of an environment update */
+#define MPSD_KERNEL_KINDS 4
+int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on);
+int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches);
+/* Force the BRGEMM tile (tm, tn in {32, 64}); 0, 0 restores the built-in choice. */
+int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
+
+/* ---- device tensors -------------------------------------------------------------------
+ * HBM-resident counterpart of an mwArray (src/wrapper/mwArray.h:23-498): rank <= 8,
+ * zero-initialised.  upload/download move the whole buffer (mwArray::getComponents()
+ * layout, n complex numbers = 2n doubles). */
+int mpsd_tensor_create(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** out);
+int mpsd_tensor_destroy(mpsd_tensor* t);
+int mpsd_tensor_rank(const mpsd_tensor* t);
+int mpsd_tensor_dims(const mpsd_tensor* t, int* dims);
+int64_t mpsd_tensor_size(const mpsd_tensor* t); /* number of complex elements */
+/* Change the logical shape; reallocates only if the new size exceeds the capacity.
+ * Replaces mwArray::reshape/resize of TmpManager slots (TmpManager.cpp:111-211). */
+int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims);
+int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex);
+int mpsd_tensor_download(const mpsd_tensor* t, double* host, int64_t n_complex);
+int mpsd_tensor_copy(mpsd_tensor* dst, const mpsd_tensor* src); /* same size, stream-ordered */
+/* Device address of the buffer (complex128 interleaved), for callers that keep vectors
+ * on the device (mpsd_heff_apply with on_device = 1). */
+void* mpsd_tensor_devptr(mpsd_tensor* t);
+
+/* ---- environment updates ---------------------------------------------------------------
+ * out[Dr_bra, wr, Dr_ket] = sum conj(bra[s', b, c]) envL[b, wl, a] W[s', wl, s, wr] ket[s, a, a_r]
+ * Replaces Operator::contractL -> contractleftop (src/core/cpp/Operator.cpp:273-334),
+ * called from Contractor::calculateL (src/core/cpp/Contractor.cpp:723-742).
+ * `out` is reshaped to [Dr_bra, wr, Dr_ket]; it must not alias an input. */
+int mpsd_env_left(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* ket,
+                  const mpsd_tensor* W, const mpsd_tensor* bra, mpsd_tensor* out);
+/* out[Dl_bra, wl, Dl_ket] = sum conj(bra[s', b, c]) W[s', wl, s, wr] envR[c, wr, a] ket[s, a_l, a]
+ * Replaces Operator::contractR -> contractrightop (src/core/cpp/Operator.cpp:336-398),
+ * called from Contractor::calculateR (src/core/cpp/Contractor.cpp:744-764). */
+int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ket,
+                   const mpsd_tensor* W, const mpsd_tensor* bra, mpsd_tensor* out);
+
+/* ---- effective Hamiltonian ---------------------------------------------------------------
+ * Heff = (L . W . R) with y[s', b_l, b_r] = sum L[b_l,a,a_l] W[s',a,s,b] R[b_r,b,a_r] x[s,a_l,a_r]
+ * (+ offset * x).  create replaces the TensorMultiplierOffset constructor
+ * (src/wrapper/multipliers/TensorMultiplier.cpp:8-39, TensorMultiplierOffset.cpp:8-12) built
+ * in Contractor::findGroundState (src/core/cpp/Contractor.cpp:1400-1406); the handle keeps
+ * references to the three device tensors (no copies), which must outlive it. */
+int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* W,
+                     const mpsd_tensor* envR, double offset, int precision, mpsd_heff** out);
+int mpsd_heff_destroy(mpsd_heff* h);
+/* Multiplier::getSize (src/wrapper/multipliers/Multiplier.h:18-49): d * Dl_ket * Dr_ket. */
+int64_t mpsd_heff_size(const mpsd_heff* h);
+/* y = Heff x for nvec column vectors.  Replaces TensorMultiplierOffset::product
+ * (TensorMultiplierOffset.cpp:14-17 -> TensorMultiplier.cpp:61-93) as called by the PRIMME
+ * matvec callback matrixMatvec(x, ldx, y, ldy, blockSize, ...) (src/wrapper/eigs.cpp:296-302,
+ * primme_headers_v3/primme_eigs.h:170-172) and by ARPACK reverse communication
+ * (src/wrapper/eigs.cpp:125-138).  ldx/ldy are in complex elements (>= size).  With
+ * on_device = 0, x and y are host buffers (copied over PCIe); with on_device = 1 they are
+ * device addresses (e.g. mpsd_tensor_devptr).  The call is stream-ordered on the context
+ * and, for host buffers, returns after y has been written. */
+int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx,
+                    int64_t ldy, int on_device);
+/* Dense Heff (size x size, column-major) into a host buffer.  Replaces
+ * TensorMultiplier::getFullTensor (TensorMultiplier.cpp:102-117) / the Offset variant
+ * (TensorMultiplierOffset.cpp:21-25), used by Contractor::solveEigs below 801
+ * (src/core/cpp/Contractor.cpp:3126-3133). */
+int mpsd_heff_full(mpsd_heff* h, double* host_out);
+/* Change the arithmetic of an existing handle (MPSD_PREC_*). */
+int mpsd_heff_set_precision(mpsd_heff* h, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSD_H */

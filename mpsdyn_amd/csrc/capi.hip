@@ -1,0 +1,675 @@
+// C-ABI of the device library: contexts, HBM tensors, environment updates and the
+// effective-Hamiltonian matvec.  Shapes and their chaining are checked on the host before
+// any launch; every kernel below only ever sees operands whose sizes match its grid.
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mpsd_internal.h"
+
+namespace mpsd {
+
+int set_err(mpsd_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_err(mpsd_ctx* ctx, hipError_t e, const char* where) {
+  if (e == hipSuccess) return MPSD_OK;
+  return set_err(ctx, e == hipErrorOutOfMemory ? MPSD_ERR_NOMEM : MPSD_ERR_HIP,
+                 std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes) {
+  if (ctx->scratch_bytes[slot] >= bytes) return MPSD_OK;
+  if (ctx->scratch[slot]) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "scratch sync");
+    (void)hipFree(ctx->scratch[slot]);
+    ctx->scratch[slot] = nullptr;
+    ctx->scratch_bytes[slot] = 0;
+  }
+  size_t want = bytes + bytes / 4;  // headroom so a sweep's growing bonds reallocate rarely
+  hipError_t e = hipMalloc(&ctx->scratch[slot], want);
+  if (e != hipSuccess) return hip_err(ctx, e, "scratch alloc");
+  ctx->scratch_bytes[slot] = want;
+  return MPSD_OK;
+}
+
+static hipEvent_t pool_event(mpsd_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch) {
+  if (!ctx->timing) return launch();
+  hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+  (void)hipEventRecord(a, ctx->stream);
+  hipError_t e = launch();
+  (void)hipEventRecord(b, ctx->stream);
+  ctx->pending.push_back({kind, a, b});
+  return e;
+}
+
+int flush_timing(mpsd_ctx* ctx) {
+  if (ctx->pending.empty()) return MPSD_OK;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "timing sync");
+  for (auto& p : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ctx->total_ms[p.kind] += ms;
+      ctx->launches[p.kind] += 1;
+    }
+    ctx->event_pool.push_back(p.a);
+    ctx->event_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+  return MPSD_OK;
+}
+
+static int ensure_stage(mpsd_ctx* ctx, int slot, size_t bytes) {
+  if (ctx->stage_bytes[slot] >= bytes) return MPSD_OK;
+  if (ctx->stage[slot]) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->stage[slot]);
+    ctx->stage[slot] = nullptr;
+    ctx->stage_bytes[slot] = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->stage[slot], bytes);
+  if (e != hipSuccess) return hip_err(ctx, e, "staging alloc");
+  ctx->stage_bytes[slot] = bytes;
+  return MPSD_OK;
+}
+
+static std::string dims_str(const mpsd_tensor* t) {
+  std::string s = "[";
+  for (int i = 0; i < t->rank; ++i) s += (i ? "," : "") + std::to_string(t->dims[i]);
+  return s + "]";
+}
+
+// Dimension i of a tensor viewed with at least `r` indices (trailing ones), as mwArray and
+// Operator pad a lower rank with unit dimensions (Operator.cpp:37-48).
+static int dim(const mpsd_tensor* t, int i) { return i < t->rank ? t->dims[i] : 1; }
+
+static bool rank_ok(const mpsd_tensor* t, int r) {
+  for (int i = r; i < t->rank; ++i)
+    if (t->dims[i] != 1) return false;
+  return true;
+}
+
+// Stage 1 + 2 of a left-type contraction into T2 = scratch[1]:
+//   T1[b, al, s, ar] = sum_a E[b, al, a] X[s, a, ar]           (BRGEMM, batch s)
+//   T2[b, s', be, ar] = sum_{al,s} W[s', al, s, be] T1[b, al, s, ar]   (mix)
+static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, const double2* X,
+                       int d, int Dkr, const double2* mix, int wr) {
+  const size_t t1 = sizeof(double2) * (size_t)Db * wl * d * Dkr;
+  const size_t t2 = sizeof(double2) * (size_t)Db * d * wr * Dkr;
+  int rc = ensure_scratch(ctx, 0, t1);
+  if (rc) return rc;
+  rc = ensure_scratch(ctx, 1, t2);
+  if (rc) return rc;
+  ZgemmDesc g;
+  g.A = E;
+  g.B = X;
+  g.C = (double2*)ctx->scratch[0];
+  g.M = Db * wl;
+  g.N = Dkr;
+  g.K = Dk;
+  g.NB = d;
+  g.sAm = 1;
+  g.sAk = (int64_t)Db * wl;
+  g.sAb = 0;
+  g.sBk = d;
+  g.sBn = (int64_t)d * Dk;
+  g.sBb = 1;
+  g.sCm = 1;
+  g.sCn = (int64_t)Db * wl * d;
+  g.sCb = (int64_t)Db * wl;
+  g.a_mfast = 1;
+  g.b_kfast = 1;
+  hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
+                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "stage1 launch");
+  e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+    return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Db,
+                           wl * d, d * wr, Dkr, ctx->stream);
+  });
+  return hip_err(ctx, e, "mix launch");
+}
+
+// Mix matrix of a left-type contraction: M[o = s' + d*be, i = al + wl*s] = W[s', al, s, be].
+static void mix_left(const std::vector<double2>& W, int d, int wl, int wr, std::vector<double2>& m) {
+  const int nout = d * wr, nin = wl * d;
+  m.assign((size_t)nout * nin, double2{0.0, 0.0});
+  for (int sp = 0; sp < d; ++sp)
+    for (int al = 0; al < wl; ++al)
+      for (int s = 0; s < d; ++s)
+        for (int be = 0; be < wr; ++be)
+          m[(sp + d * be) + (size_t)nout * (al + wl * s)] =
+              W[sp + d * al + (size_t)d * wl * s + (size_t)d * wl * d * be];
+}
+
+// Mix matrix of a right-type contraction: M[o = s' + d*al, i = be + wr*s] = W[s', al, s, be].
+static void mix_right(const std::vector<double2>& W, int d, int wl, int wr, std::vector<double2>& m) {
+  const int nout = d * wl, nin = wr * d;
+  m.assign((size_t)nout * nin, double2{0.0, 0.0});
+  for (int sp = 0; sp < d; ++sp)
+    for (int al = 0; al < wl; ++al)
+      for (int s = 0; s < d; ++s)
+        for (int be = 0; be < wr; ++be)
+          m[(sp + d * al) + (size_t)nout * (be + wr * s)] =
+              W[sp + d * al + (size_t)d * wl * s + (size_t)d * wl * d * be];
+}
+
+static int fetch_w(mpsd_ctx* ctx, const mpsd_tensor* W, std::vector<double2>& h) {
+  h.resize(W->n);
+  hipError_t e = hipMemcpyAsync(h.data(), W->ptr, sizeof(double2) * W->n, hipMemcpyDeviceToHost,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return hip_err(ctx, e, "fetch W");
+}
+
+static int upload_mix(mpsd_ctx* ctx, const std::vector<double2>& m, double2** dst) {
+  hipError_t e = hipMalloc((void**)dst, sizeof(double2) * m.size());
+  if (e != hipSuccess) return hip_err(ctx, e, "mix alloc");
+  e = hipMemcpyAsync(*dst, m.data(), sizeof(double2) * m.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return hip_err(ctx, e, "mix upload");
+}
+
+}  // namespace mpsd
+
+using namespace mpsd;
+
+namespace {
+// The mix matrices derived from a W tensor are cached on the tensor and rebuilt only when
+// its contents change (every host upload or device copy bumps its version).
+int get_mix(mpsd_ctx* ctx, const mpsd_tensor* W, int kind, int d, int wl, int wr, double2** out) {
+  if (W->mix[kind] && W->mix_version[kind] == W->version) {
+    *out = W->mix[kind];
+    return MPSD_OK;
+  }
+  std::vector<double2> hw, hm;
+  int rc = fetch_w(ctx, W, hw);
+  if (rc) return rc;
+  if (kind == 0)
+    mix_left(hw, d, wl, wr, hm);
+  else
+    mix_right(hw, d, wl, wr, hm);
+  if (W->mix[kind]) {
+    (void)hipFree(W->mix[kind]);
+    W->mix[kind] = nullptr;
+  }
+  rc = upload_mix(ctx, hm, &W->mix[kind]);
+  if (rc) return rc;
+  W->mix_version[kind] = W->version;
+  *out = W->mix[kind];
+  return MPSD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mpsd_ctx_create(int device, mpsd_ctx** out) {
+  if (!out) return MPSD_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MPSD_ERR_HIP;
+  if (device < 0 || device >= n) return MPSD_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return MPSD_ERR_HIP;
+  mpsd_ctx* c = new mpsd_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MPSD_ERR_HIP;
+  }
+  *out = c;
+  return MPSD_OK;
+}
+
+int mpsd_ctx_destroy(mpsd_ctx* ctx) {
+  if (!ctx) return MPSD_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)flush_timing(ctx);
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+    if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
+  }
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MPSD_OK;
+}
+
+int mpsd_ctx_sync(mpsd_ctx* ctx) {
+  if (!ctx) return MPSD_ERR_ARG;
+  return hip_err(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on) {
+  if (!ctx) return MPSD_ERR_ARG;
+  int rc = flush_timing(ctx);
+  for (int k = 0; k < MPSD_KERNEL_KINDS; ++k) {
+    ctx->total_ms[k] = 0.0;
+    ctx->launches[k] = 0;
+  }
+  ctx->timing = on != 0;
+  return rc;
+}
+
+int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches) {
+  if (!ctx || kind < 0 || kind >= MPSD_KERNEL_KINDS || !total_ms || !launches) return MPSD_ERR_ARG;
+  int rc = flush_timing(ctx);
+  *total_ms = ctx->total_ms[kind];
+  *launches = ctx->launches[kind];
+  return rc;
+}
+
+int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn) {
+  if (!ctx) return MPSD_ERR_ARG;
+  const bool ok = (tm == 0 && tn == 0) || ((tm == 32 || tm == 64) && (tn == 32 || tn == 64));
+  if (!ok) return set_err(ctx, MPSD_ERR_ARG, "set_tile: tm, tn must be 32 or 64 (or both 0)");
+  ctx->tile_override = mpsd::TileCfg{tm, tn};
+  return MPSD_OK;
+}
+
+const char* mpsd_last_error(const mpsd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* mpsd_ctx_stream(mpsd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx) {
+  if (!ctx) return 0;
+  return ctx->tensor_bytes + (int64_t)ctx->scratch_bytes[0] + (int64_t)ctx->scratch_bytes[1] +
+         (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1];
+}
+
+int mpsd_tensor_create(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** out) {
+  if (!ctx || !out || rank < 0 || rank > 8 || (rank > 0 && !dims)) return set_err(ctx, MPSD_ERR_ARG, "tensor_create: bad arguments");
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] <= 0) return set_err(ctx, MPSD_ERR_ARG, "tensor_create: non-positive dimension");
+    n *= dims[i];
+  }
+  mpsd_tensor* t = new mpsd_tensor();
+  t->ctx = ctx;
+  t->rank = rank;
+  for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
+  t->n = n;
+  t->cap = n;
+  hipError_t e = hipMalloc((void**)&t->ptr, sizeof(double2) * (size_t)n);
+  if (e == hipSuccess) e = hipMemsetAsync(t->ptr, 0, sizeof(double2) * (size_t)n, ctx->stream);
+  if (e != hipSuccess) {
+    if (t->ptr) (void)hipFree(t->ptr);
+    delete t;
+    return hip_err(ctx, e, "tensor_create");
+  }
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * n;
+  *out = t;
+  return MPSD_OK;
+}
+
+int mpsd_tensor_destroy(mpsd_tensor* t) {
+  if (!t) return MPSD_ERR_ARG;
+  (void)hipStreamSynchronize(t->ctx->stream);
+  if (t->ptr) (void)hipFree(t->ptr);
+  for (int k = 0; k < 2; ++k)
+    if (t->mix[k]) (void)hipFree(t->mix[k]);
+  t->ctx->tensor_bytes -= (int64_t)sizeof(double2) * t->cap;
+  delete t;
+  return MPSD_OK;
+}
+
+int mpsd_tensor_rank(const mpsd_tensor* t) { return t ? t->rank : MPSD_ERR_ARG; }
+
+int mpsd_tensor_dims(const mpsd_tensor* t, int* dims) {
+  if (!t || !dims) return MPSD_ERR_ARG;
+  for (int i = 0; i < t->rank; ++i) dims[i] = t->dims[i];
+  return MPSD_OK;
+}
+
+int64_t mpsd_tensor_size(const mpsd_tensor* t) { return t ? t->n : MPSD_ERR_ARG; }
+
+int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims) {
+  if (!t || rank < 0 || rank > 8 || (rank > 0 && !dims)) return MPSD_ERR_ARG;
+  mpsd_ctx* ctx = t->ctx;
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] <= 0) return set_err(ctx, MPSD_ERR_ARG, "tensor_reshape: non-positive dimension");
+    n *= dims[i];
+  }
+  if (n > t->cap) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "reshape sync");
+    (void)hipFree(t->ptr);
+    ctx->tensor_bytes -= (int64_t)sizeof(double2) * t->cap;
+    t->ptr = nullptr;
+    t->cap = 0;
+    e = hipMalloc((void**)&t->ptr, sizeof(double2) * (size_t)n);
+    if (e != hipSuccess) return hip_err(ctx, e, "reshape alloc");
+    t->cap = n;
+    ctx->tensor_bytes += (int64_t)sizeof(double2) * n;
+  }
+  t->rank = rank;
+  for (int i = 0; i < 8; ++i) t->dims[i] = i < rank ? dims[i] : 0;
+  t->n = n;
+  return MPSD_OK;
+}
+
+int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex) {
+  if (!t || !host) return MPSD_ERR_ARG;
+  if (n_complex != t->n) return set_err(t->ctx, MPSD_ERR_SHAPE, "tensor_upload: size mismatch");
+  hipError_t e = hipMemcpyAsync(t->ptr, host, sizeof(double2) * (size_t)t->n, hipMemcpyHostToDevice,
+                                t->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+  ++t->version;
+  return hip_err(t->ctx, e, "tensor_upload");
+}
+
+int mpsd_tensor_download(const mpsd_tensor* t, double* host, int64_t n_complex) {
+  if (!t || !host) return MPSD_ERR_ARG;
+  if (n_complex != t->n) return set_err(t->ctx, MPSD_ERR_SHAPE, "tensor_download: size mismatch");
+  hipError_t e = hipMemcpyAsync(host, t->ptr, sizeof(double2) * (size_t)t->n, hipMemcpyDeviceToHost,
+                                t->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+  return hip_err(t->ctx, e, "tensor_download");
+}
+
+int mpsd_tensor_copy(mpsd_tensor* dst, const mpsd_tensor* src) {
+  if (!dst || !src) return MPSD_ERR_ARG;
+  if (dst->n != src->n) return set_err(dst->ctx, MPSD_ERR_SHAPE, "tensor_copy: size mismatch");
+  ++dst->version;
+  return hip_err(dst->ctx,
+                 hipMemcpyAsync(dst->ptr, src->ptr, sizeof(double2) * (size_t)src->n,
+                                hipMemcpyDeviceToDevice, dst->ctx->stream),
+                 "tensor_copy");
+}
+
+void* mpsd_tensor_devptr(mpsd_tensor* t) { return t ? (void*)t->ptr : nullptr; }
+
+int mpsd_env_left(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* ket,
+                  const mpsd_tensor* W, const mpsd_tensor* bra, mpsd_tensor* out) {
+  if (!ctx || !envL || !ket || !W || !bra || !out) return set_err(ctx, MPSD_ERR_ARG, "env_left: null argument");
+  if (!rank_ok(envL, 3) || !rank_ok(ket, 3) || !rank_ok(W, 4) || !rank_ok(bra, 3))
+    return set_err(ctx, MPSD_ERR_SHAPE, "env_left: rank too high");
+  if (out == envL || out == ket || out == bra || out == W)
+    return set_err(ctx, MPSD_ERR_ARG, "env_left: output aliases an input");
+  const int Db = dim(envL, 0), wl = dim(envL, 1), Dk = dim(envL, 2);
+  const int d = dim(bra, 0), Dbl = dim(bra, 1), Dbr = dim(bra, 2);
+  const int dp = dim(ket, 0), Dkl = dim(ket, 1), Dkr = dim(ket, 2);
+  const int du = dim(W, 0), al1 = dim(W, 1), dd = dim(W, 2), wr = dim(W, 3);
+  // Operator::contractleftop dimension check (Operator.cpp:283-289)
+  if (d != du || dp != dd || Dkl != Dk || Dbl != Db || al1 != wl || d != dp)
+    return set_err(ctx, MPSD_ERR_SHAPE,
+                   "env_left: wrong dimensions: bra " + dims_str(bra) + ", ket " + dims_str(ket) +
+                       ", env " + dims_str(envL) + ", W " + dims_str(W));
+  double2* mix = nullptr;
+  int rc = get_mix(ctx, W, 0, d, wl, wr, &mix);
+  if (rc) return rc;
+  const int odims[3] = {Dbr, wr, Dkr};
+  rc = mpsd_tensor_reshape(out, 3, odims);
+  if (rc) return rc;
+  rc = left_stages(ctx, envL->ptr, Db, wl, Dk, ket->ptr, d, Dkr, mix, wr);
+  if (rc) return rc;
+  // L'(c, n = be + wr*ar) = sum_{s'} sum_b conj(bra[s', b, c]) T2[b + Db*s' + Db*d*n]
+  ZgemmDesc g;
+  g.A = bra->ptr;
+  g.B = (const double2*)ctx->scratch[1];
+  g.C = out->ptr;
+  g.M = Dbr;
+  g.N = wr * Dkr;
+  g.K = Db;
+  g.R = d;
+  g.NB = 1;
+  g.sAm = (int64_t)d * Db;
+  g.sAk = d;
+  g.sAr = 1;
+  g.sBk = 1;
+  g.sBn = (int64_t)Db * d;
+  g.sBr = Db;
+  g.sCm = 1;
+  g.sCn = Dbr;
+  g.conjA = 1;
+  g.a_mfast = 0;
+  g.b_kfast = 1;
+  return hip_err(ctx,
+                 timed(ctx, MPSD_KERNEL_ENV_STAGE3,
+                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); }),
+                 "env_left stage3");
+}
+
+int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ket,
+                   const mpsd_tensor* W, const mpsd_tensor* bra, mpsd_tensor* out) {
+  if (!ctx || !envR || !ket || !W || !bra || !out) return set_err(ctx, MPSD_ERR_ARG, "env_right: null argument");
+  if (!rank_ok(envR, 3) || !rank_ok(ket, 3) || !rank_ok(W, 4) || !rank_ok(bra, 3))
+    return set_err(ctx, MPSD_ERR_SHAPE, "env_right: rank too high");
+  if (out == envR || out == ket || out == bra || out == W)
+    return set_err(ctx, MPSD_ERR_ARG, "env_right: output aliases an input");
+  const int Dbr = dim(envR, 0), wr = dim(envR, 1), Dkr = dim(envR, 2);
+  const int d = dim(bra, 0), Dbl = dim(bra, 1), Dbr2 = dim(bra, 2);
+  const int dp = dim(ket, 0), Dkl = dim(ket, 1), Dkr2 = dim(ket, 2);
+  const int d1 = dim(W, 0), wl = dim(W, 1), d2 = dim(W, 2), al2 = dim(W, 3);
+  // Operator::contractrightop dimension check (Operator.cpp:344-352)
+  if (dp != d2 || d != d1 || Dkr2 != Dkr || Dbr2 != Dbr || al2 != wr || d != dp)
+    return set_err(ctx, MPSD_ERR_SHAPE,
+                   "env_right: wrong dimensions: bra " + dims_str(bra) + ", ket " + dims_str(ket) +
+                       ", env " + dims_str(envR) + ", W " + dims_str(W));
+  double2* mix = nullptr;
+  int rc = get_mix(ctx, W, 1, d, wl, wr, &mix);
+  if (rc) return rc;
+  const int odims[3] = {Dbl, wl, Dkl};
+  rc = mpsd_tensor_reshape(out, 3, odims);
+  if (rc) return rc;
+  const size_t u1 = sizeof(double2) * (size_t)Dbr * wr * d * Dkl;
+  const size_t u2 = sizeof(double2) * (size_t)Dbr * d * wl * Dkl;
+  rc = ensure_scratch(ctx, 0, u1);
+  if (rc) return rc;
+  rc = ensure_scratch(ctx, 1, u2);
+  if (rc) return rc;
+  // U1[c, be, s, al] = sum_ar R[c, be, ar] ket[s, al, ar]
+  ZgemmDesc g;
+  g.A = envR->ptr;
+  g.B = ket->ptr;
+  g.C = (double2*)ctx->scratch[0];
+  g.M = Dbr * wr;
+  g.N = Dkl;
+  g.K = Dkr;
+  g.NB = d;
+  g.sAm = 1;
+  g.sAk = (int64_t)Dbr * wr;
+  g.sAb = 0;
+  g.sBk = (int64_t)d * Dkl;
+  g.sBn = d;
+  g.sBb = 1;
+  g.sCm = 1;
+  g.sCn = (int64_t)Dbr * wr * d;
+  g.sCb = (int64_t)Dbr * wr;
+  g.a_mfast = 1;
+  g.b_kfast = 0;
+  hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
+                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "env_right stage1");
+  // U2[c, s', al, a] = sum_{be,s} W[s', al, s, be] U1[c, be, s, a]
+  e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+    return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Dbr,
+                           wr * d, d * wl, Dkl, ctx->stream);
+  });
+  if (e != hipSuccess) return hip_err(ctx, e, "env_right mix");
+  // R'(b, n = al + wl*a) = sum_{s'} sum_c conj(bra[s', b, c]) U2[c + Dbr*s' + Dbr*d*n]
+  ZgemmDesc h;
+  h.A = bra->ptr;
+  h.B = (const double2*)ctx->scratch[1];
+  h.C = out->ptr;
+  h.M = Dbl;
+  h.N = wl * Dkl;
+  h.K = Dbr;
+  h.R = d;
+  h.NB = 1;
+  h.sAm = d;
+  h.sAk = (int64_t)d * Dbl;
+  h.sAr = 1;
+  h.sBk = 1;
+  h.sBn = (int64_t)Dbr * d;
+  h.sBr = Dbr;
+  h.sCm = 1;
+  h.sCn = Dbl;
+  h.conjA = 1;
+  h.a_mfast = 1;
+  h.b_kfast = 1;
+  return hip_err(ctx,
+                 timed(ctx, MPSD_KERNEL_ENV_STAGE3,
+                       [&] { return launch_zbrgemm_f64(h, ctx->tile_override, ctx->stream); }),
+                 "env_right stage3");
+}
+
+int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* W,
+                     const mpsd_tensor* envR, double offset, int precision, mpsd_heff** out) {
+  if (!ctx || !envL || !W || !envR || !out) return set_err(ctx, MPSD_ERR_ARG, "heff_create: null argument");
+  *out = nullptr;
+  if (!rank_ok(envL, 3) || !rank_ok(envR, 3) || !rank_ok(W, 4))
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_create: rank too high");
+  if (precision != MPSD_PREC_FP64)
+    return set_err(ctx, MPSD_ERR_UNSUPPORTED, "heff_create: precision not built");
+  mpsd_heff* h = new mpsd_heff();
+  h->ctx = ctx;
+  h->L = envL;
+  h->W = W;
+  h->R = envR;
+  h->offset = offset;
+  h->precision = precision;
+  h->Db = dim(envL, 0);
+  h->wl = dim(envL, 1);
+  h->Dk = dim(envL, 2);
+  h->d = dim(W, 0);
+  h->Dbr = dim(envR, 0);
+  h->wr = dim(envR, 1);
+  h->Dkr = dim(envR, 2);
+  // TensorMultiplier CHECKDIMS (TensorMultiplier.cpp:22-28) plus square physical index
+  if (dim(W, 1) != h->wl || dim(W, 3) != h->wr || dim(W, 2) != h->d) {
+    delete h;
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_create: wrong dimensions: L " + dims_str(envL) + ", W " +
+                                            dims_str(W) + ", R " + dims_str(envR));
+  }
+  if (offset != 0.0 && (h->Db != h->Dk || h->Dbr != h->Dkr)) {
+    delete h;
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_create: offset needs equal bra and ket dimensions");
+  }
+  int rc = get_mix(ctx, W, 0, h->d, h->wl, h->wr, &h->mix);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return MPSD_OK;
+}
+
+int mpsd_heff_destroy(mpsd_heff* h) {
+  if (!h) return MPSD_ERR_ARG;
+  (void)hipStreamSynchronize(h->ctx->stream);
+  delete h;
+  return MPSD_OK;
+}
+
+int64_t mpsd_heff_size(const mpsd_heff* h) {
+  return h ? (int64_t)h->d * h->Dk * h->Dkr : (int64_t)MPSD_ERR_ARG;
+}
+
+int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
+  if (!h) return MPSD_ERR_ARG;
+  if (precision != MPSD_PREC_FP64)
+    return set_err(h->ctx, MPSD_ERR_UNSUPPORTED, "heff_set_precision: precision not built");
+  h->precision = precision;
+  return MPSD_OK;
+}
+
+// One device matvec y = Heff x (+ offset x) on device pointers.
+static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
+  mpsd_ctx* ctx = h->ctx;
+  int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
+  if (rc) return rc;
+  rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr);
+  if (rc) return rc;
+  // y[s', b, c] = sum_{k = be + wr*ar} T2[b + Db*s' + Db*d*k] R[c + Dbr*k]
+  ZgemmDesc g;
+  g.A = (const double2*)ctx->scratch[1];
+  g.B = h->R->ptr;
+  g.C = y;
+  g.M = h->Db;
+  g.N = h->Dbr;
+  g.K = h->wr * h->Dkr;
+  g.NB = h->d;
+  g.sAm = 1;
+  g.sAk = (int64_t)h->Db * h->d;
+  g.sAb = h->Db;
+  g.sBk = h->Dbr;
+  g.sBn = 1;
+  g.sBb = 0;
+  g.sCm = h->d;
+  g.sCn = (int64_t)h->d * h->Db;
+  g.sCb = 1;
+  g.a_mfast = 1;
+  g.b_kfast = 0;
+  if (h->offset != 0.0) {
+    g.Dadd = x;
+    g.beta = h->offset;
+  }
+  return hip_err(ctx,
+                 timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
+                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); }),
+                 "heff stage3");
+}
+
+int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx, int64_t ldy,
+                    int on_device) {
+  if (!h || !x || !y || nvec < 0) return MPSD_ERR_ARG;
+  mpsd_ctx* ctx = h->ctx;
+  const int64_t n = mpsd_heff_size(h);
+  const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
+  if (ldx < n || ldy < ny) return set_err(ctx, MPSD_ERR_ARG, "heff_apply: leading dimension too small");
+  if (on_device) {
+    for (int v = 0; v < nvec; ++v) {
+      int rc = heff_apply_dev(h, (const double2*)x + v * ldx, (double2*)y + v * ldy);
+      if (rc) return rc;
+    }
+    return MPSD_OK;
+  }
+  int rc = ensure_stage(ctx, 0, sizeof(double2) * (size_t)n);
+  if (!rc) rc = ensure_stage(ctx, 1, sizeof(double2) * (size_t)ny);
+  if (rc) return rc;
+  for (int v = 0; v < nvec; ++v) {
+    hipError_t e = hipMemcpyAsync(ctx->stage[0], (const double2*)x + v * ldx, sizeof(double2) * n,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "heff_apply H2D");
+    rc = heff_apply_dev(h, (const double2*)ctx->stage[0], (double2*)ctx->stage[1]);
+    if (rc) return rc;
+    e = hipMemcpyAsync((double2*)y + v * ldy, ctx->stage[1], sizeof(double2) * ny,
+                       hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "heff_apply D2H");
+  }
+  return hip_err(ctx, hipStreamSynchronize(ctx->stream), "heff_apply sync");
+}
+
+int mpsd_heff_full(mpsd_heff* h, double* host_out) {
+  if (!h || !host_out) return MPSD_ERR_ARG;
+  mpsd_ctx* ctx = h->ctx;
+  const int64_t nr = (int64_t)h->d * h->Db * h->Dbr, nc = mpsd_heff_size(h);
+  int rc = ensure_stage(ctx, 0, sizeof(double2) * (size_t)(nr * nc));
+  if (rc) return rc;
+  hipError_t e = launch_heff_full_f64(h->L->ptr, h->W->ptr, h->R->ptr, (double2*)ctx->stage[0], h->d,
+                                      h->Db, h->wl, h->Dk, h->Dbr, h->wr, h->Dkr, h->offset, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host_out, ctx->stage[0], sizeof(double2) * (size_t)(nr * nc), hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return hip_err(ctx, e, "heff_full");
+}
+
+}  // extern "C"

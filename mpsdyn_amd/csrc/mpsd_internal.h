@@ -1,0 +1,109 @@
+// Internal definitions shared by the device library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "mpsd.h"
+
+namespace mpsd {
+
+// One batch-reduce complex GEMM over strided operands (strides in complex elements):
+//   C[b](m, n) = sum_{r < R} sum_{k < K} opA(A[b, r](m, k)) * B[b, r](k, n)  (+ beta * Dadd[b](m, n))
+// opA conjugates when conjA is set.  Every permute of the reference's reshape/permute/ZGEMM
+// chains becomes a choice of strides here, so nothing is materialised but the stage outputs.
+struct ZgemmDesc {
+  const double2* A = nullptr;
+  const double2* B = nullptr;
+  double2* C = nullptr;
+  const double2* Dadd = nullptr;  // same strides as C; may be null
+  double beta = 0.0;
+  int M = 0, N = 0, K = 0, R = 1, NB = 1;
+  int64_t sAm = 0, sAk = 0, sAr = 0, sAb = 0;
+  int64_t sBk = 0, sBn = 0, sBr = 0, sBb = 0;
+  int64_t sCm = 0, sCn = 0, sCb = 0;
+  int conjA = 0;
+  int a_mfast = 1;  // load order of the A tile: m fastest (1) or k fastest (0)
+  int b_kfast = 1;  // load order of the B tile: k fastest (1) or n fastest (0)
+};
+
+// Tile configuration of a BRGEMM launch (chosen by the tuner or by heuristics).
+struct TileCfg {
+  int tm = 32, tn = 32;
+};
+
+hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
+
+// Apply the MPO "mix" of the middle contraction stage:
+//   out[bx + Bx*o + Bx*nout*ax] = sum_i mix[o + nout*i] * in[bx + Bx*i + Bx*nin*ax]
+hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, int Bx, int nin,
+                           int nout, int64_t Ax, hipStream_t s);
+
+// Dense Heff (rows s',b_l,b_r ; cols s,a_l,a_r), column-major, plus offset on the diagonal.
+hipError_t launch_heff_full_f64(const double2* L, const double2* W, const double2* R, double2* H,
+                                int d, int Db, int wl, int Dk, int Dbr, int wr, int Dkr,
+                                double offset, hipStream_t s);
+
+}  // namespace mpsd
+
+struct mpsd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // Scratch for the two intermediates of a contraction (grown on demand, never shrunk).
+  void* scratch[2] = {nullptr, nullptr};
+  size_t scratch_bytes[2] = {0, 0};
+  // Staging for host-buffer matvecs.
+  void* stage[2] = {nullptr, nullptr};
+  size_t stage_bytes[2] = {0, 0};
+  int64_t tensor_bytes = 0;
+  mpsd::TileCfg tile_override{0, 0};
+  // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
+  bool timing = false;
+  struct Pending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double total_ms[MPSD_KERNEL_KINDS] = {0};
+  int64_t launches[MPSD_KERNEL_KINDS] = {0};
+};
+
+struct mpsd_tensor {
+  mpsd_ctx* ctx = nullptr;
+  int rank = 0;
+  int dims[8] = {0};
+  int64_t n = 0;    // complex elements
+  int64_t cap = 0;  // capacity in complex elements
+  double2* ptr = nullptr;
+  // Contents version (bumped by every host upload / device copy into the tensor) and the
+  // mix matrices derived from it when the tensor is used as an MPO tensor W.
+  uint64_t version = 0;
+  mutable double2* mix[2] = {nullptr, nullptr};
+  mutable uint64_t mix_version[2] = {0, 0};
+};
+
+struct mpsd_heff {
+  mpsd_ctx* ctx = nullptr;
+  const mpsd_tensor* L = nullptr;
+  const mpsd_tensor* W = nullptr;
+  const mpsd_tensor* R = nullptr;
+  double offset = 0.0;
+  int precision = MPSD_PREC_FP64;
+  int d = 0, Db = 0, wl = 0, Dk = 0, Dbr = 0, wr = 0, Dkr = 0;
+  double2* mix = nullptr;  // device (d*wr) x (wl*d) mix matrix from W (owned by W's cache)
+};
+
+namespace mpsd {
+int set_err(mpsd_ctx* ctx, int code, const std::string& msg);
+int hip_err(mpsd_ctx* ctx, hipError_t e, const char* where);
+int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes);
+// Run `launch` on the context's stream, bracketed by HIP events when timing is on.
+hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch);
+int flush_timing(mpsd_ctx* ctx);
+}  // namespace mpsd

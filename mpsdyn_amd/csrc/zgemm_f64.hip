@@ -1,0 +1,345 @@
+// FP64 complex contraction kernels for gfx950 (CDNA4).
+//
+// zbrgemm_f64: batch-reduce complex GEMM on v_mfma_f64_16x16x4f64.  A workgroup of four
+// 64-wide waves owns a TM x TN output tile for NBW batches; operand tiles are staged
+// through LDS as interleaved (re, im) pairs and every complex product is four real MFMAs
+// (re*re, -im*im into the real accumulator; re*im, im*re into the imaginary one).
+// The global->register load of tile t+1 is issued before the MFMAs of tile t, so HBM/L2
+// latency hides behind the matrix cores.  When one operand is shared by all batches of
+// the workgroup (the environment in stage 1, the right environment in stage 3) it is
+// staged once and reused NBW times.
+//
+// MFMA 16x16x4 f64 lane layout (wave64): A[m][k] comes from lane m + 16k, B[k][n] from
+// lane n + 16k, and result element (m, n) lands in lane n + 16*(m/4), register m%4.
+
+#include "mpsd_internal.h"
+
+namespace mpsd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double2 cload(const double2* p) { return *p; }
+
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
+__global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p) {
+  constexpr int NA = ASH ? 1 : NBW;
+  constexpr int NB_ = BSH ? 1 : NBW;
+  __shared__ double2 As[NA][TK][TM];
+  __shared__ double2 Bs[NB_][TK][TN];
+
+  constexpr int WTM = TM / 2, WTN = TN / 2;  // 2x2 waves
+  constexpr int BM = WTM / 16, BN = WTN / 16;
+  constexpr int AE = TM * TK / kThreads;
+  constexpr int BE = TK * TN / kThreads;
+  static_assert(AE >= 1 && BE >= 1, "tile too small for 256 threads");
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = blockIdx.x * TM;
+  const int n0 = blockIdx.y * TN;
+  const int b0 = blockIdx.z * NBW;
+  const int nbw = min(NBW, p.NB - b0);
+
+  d4 accr[NBW][BM][BN];
+  d4 acci[NBW][BM][BN];
+#pragma unroll
+  for (int j = 0; j < NBW; ++j)
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int q = 0; q < BN; ++q) {
+        accr[j][i][q] = d4{0.0, 0.0, 0.0, 0.0};
+        acci[j][i][q] = d4{0.0, 0.0, 0.0, 0.0};
+      }
+
+  double2 ra[NA][AE];
+  double2 rb[NB_][BE];
+
+  const int nk = (p.K + TK - 1) / TK;
+  const int ntiles = nk * p.R;
+
+  auto a_coord = [&](int e, int& m, int& k) {
+    const int idx = tid + kThreads * e;
+    if (p.a_mfast) {
+      m = idx % TM;
+      k = idx / TM;
+    } else {
+      k = idx % TK;
+      m = idx / TK;
+    }
+  };
+  auto b_coord = [&](int e, int& k, int& n) {
+    const int idx = tid + kThreads * e;
+    if (p.b_kfast) {
+      k = idx % TK;
+      n = idx / TK;
+    } else {
+      n = idx % TN;
+      k = idx / TN;
+    }
+  };
+
+  auto load_tile = [&](int t) {
+    const int r = t / nk;
+    const int k0 = (t - r * nk) * TK;
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      const int jb = (j < nbw) ? j : 0;
+      const double2* base = p.A + (int64_t)(b0 + jb) * p.sAb + (int64_t)r * p.sAr;
+#pragma unroll
+      for (int e = 0; e < AE; ++e) {
+        int m, k;
+        a_coord(e, m, k);
+        const int gm = m0 + m, gk = k0 + k;
+        ra[j][e] = (gm < p.M && gk < p.K) ? cload(base + gm * p.sAm + gk * p.sAk)
+                                          : double2{0.0, 0.0};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NB_; ++j) {
+      const int jb = (j < nbw) ? j : 0;
+      const double2* base = p.B + (int64_t)(b0 + jb) * p.sBb + (int64_t)r * p.sBr;
+#pragma unroll
+      for (int e = 0; e < BE; ++e) {
+        int k, n;
+        b_coord(e, k, n);
+        const int gk = k0 + k, gn = n0 + n;
+        rb[j][e] = (gk < p.K && gn < p.N) ? cload(base + gk * p.sBk + gn * p.sBn)
+                                          : double2{0.0, 0.0};
+      }
+    }
+  };
+
+  auto store_tile = [&]() {
+    const double csign = p.conjA ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < NA; ++j)
+#pragma unroll
+      for (int e = 0; e < AE; ++e) {
+        int m, k;
+        a_coord(e, m, k);
+        As[j][k][m] = double2{ra[j][e].x, csign * ra[j][e].y};
+      }
+#pragma unroll
+    for (int j = 0; j < NB_; ++j)
+#pragma unroll
+      for (int e = 0; e < BE; ++e) {
+        int k, n;
+        b_coord(e, k, n);
+        Bs[j][k][n] = rb[j][e];
+      }
+  };
+
+  if (ntiles > 0) load_tile(0);
+  for (int t = 0; t < ntiles; ++t) {
+    store_tile();
+    __syncthreads();
+    if (t + 1 < ntiles) load_tile(t + 1);
+#pragma unroll
+    for (int kk = 0; kk < TK / 4; ++kk) {
+      const int krow = kk * 4 + (lane >> 4);
+      const int lc = lane & 15;
+      double ar[NA][BM], ai[NA][BM], an[NA][BM];
+      double br[NB_][BN], bi[NB_][BN];
+#pragma unroll
+      for (int j = 0; j < NA; ++j)
+#pragma unroll
+        for (int i = 0; i < BM; ++i) {
+          const double2 v = As[j][krow][wm * WTM + i * 16 + lc];
+          ar[j][i] = v.x;
+          ai[j][i] = v.y;
+          an[j][i] = -v.y;
+        }
+#pragma unroll
+      for (int j = 0; j < NB_; ++j)
+#pragma unroll
+        for (int q = 0; q < BN; ++q) {
+          const double2 v = Bs[j][krow][wn * WTN + q * 16 + lc];
+          br[j][q] = v.x;
+          bi[j][q] = v.y;
+        }
+#pragma unroll
+      for (int j = 0; j < NBW; ++j) {
+        const int ja = ASH ? 0 : j;
+        const int jb = BSH ? 0 : j;
+#pragma unroll
+        for (int i = 0; i < BM; ++i)
+#pragma unroll
+          for (int q = 0; q < BN; ++q) {
+            accr[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ja][i], br[jb][q], accr[j][i][q], 0, 0, 0);
+            acci[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ja][i], bi[jb][q], acci[j][i][q], 0, 0, 0);
+          }
+#pragma unroll
+        for (int i = 0; i < BM; ++i)
+#pragma unroll
+          for (int q = 0; q < BN; ++q) {
+            accr[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[ja][i], bi[jb][q], accr[j][i][q], 0, 0, 0);
+            acci[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ja][i], br[jb][q], acci[j][i][q], 0, 0, 0);
+          }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: C = acc (+ beta * Dadd).
+  const int lc = lane & 15;
+  const int lr = (lane >> 4) * 4;
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    if (j >= nbw) break;
+    double2* cb = p.C + (int64_t)(b0 + j) * p.sCb;
+    const double2* db = p.Dadd ? p.Dadd + (int64_t)(b0 + j) * p.sCb : nullptr;
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int q = 0; q < BN; ++q) {
+        const int gn = n0 + wn * WTN + q * 16 + lc;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int gm = m0 + wm * WTM + i * 16 + lr + rr;
+          if (gm < p.M && gn < p.N) {
+            const int64_t off = gm * p.sCm + gn * p.sCn;
+            double2 v{accr[j][i][q][rr], acci[j][i][q][rr]};
+            if (db) {
+              const double2 x = db[off];
+              v.x += p.beta * x.x;
+              v.y += p.beta * x.y;
+            }
+            cb[off] = v;
+          }
+        }
+      }
+  }
+}
+
+template <int TM, int TN, int NBW, bool ASH, bool BSH>
+hipError_t launch_cfg(const ZgemmDesc& p, hipStream_t s) {
+  constexpr int TK = 16;
+  dim3 grid((p.M + TM - 1) / TM, (p.N + TN - 1) / TN, (p.NB + NBW - 1) / NBW);
+  hipLaunchKernelGGL((zbrgemm_f64_kernel<TM, TN, TK, NBW, ASH, BSH>), grid, dim3(kThreads), 0, s, p);
+  return hipGetLastError();
+}
+
+template <int TM, int TN>
+hipError_t launch_tile(const ZgemmDesc& p, hipStream_t s) {
+  if (p.NB >= 2 && p.sAb == 0) return launch_cfg<TM, TN, 2, true, false>(p, s);
+  if (p.NB >= 2 && p.sBb == 0) return launch_cfg<TM, TN, 2, false, true>(p, s);
+  return launch_cfg<TM, TN, 1, false, false>(p, s);
+}
+
+}  // namespace
+
+hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s) {
+  if (p.M <= 0 || p.N <= 0 || p.NB <= 0) return hipSuccess;
+  if (cfg.tm == 0 || cfg.tn == 0) {
+    // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.
+    const int64_t zb = (p.NB >= 2 && (p.sAb == 0 || p.sBb == 0)) ? (p.NB + 1) / 2 : p.NB;
+    auto wgs = [&](int tm, int tn) { return (int64_t)((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn) * zb; };
+    if (wgs(64, 64) >= 512) cfg = {64, 64};
+    else if (wgs(64, 32) >= 512) cfg = {64, 32};
+    else cfg = {32, 32};
+  }
+  if (cfg.tm == 64 && cfg.tn == 64) return launch_tile<64, 64>(p, s);
+  if (cfg.tm == 64 && cfg.tn == 32) return launch_tile<64, 32>(p, s);
+  if (cfg.tm == 32 && cfg.tn == 64) return launch_tile<32, 64>(p, s);
+  return launch_tile<32, 32>(p, s);
+}
+
+// ------------------------------------------------------------------------------ MPO mix
+
+namespace {
+
+constexpr int kMixMax = 16;
+
+__global__ void __launch_bounds__(kThreads) zmix_f64_kernel(const double2* __restrict__ in,
+                                                            double2* __restrict__ out,
+                                                            const double2* __restrict__ mix,
+                                                            int Bx, int nin, int nout, int ldmix,
+                                                            int ldout, int64_t total) {
+  const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (gid >= total) return;
+  const int bx = (int)(gid % Bx);
+  const int64_t ax = gid / Bx;
+  const double2* src = in + bx + (int64_t)Bx * nin * ax;
+  double2 acc[kMixMax];
+#pragma unroll
+  for (int o = 0; o < kMixMax; ++o) acc[o] = double2{0.0, 0.0};
+  for (int i = 0; i < nin; ++i) {
+    const double2 v = src[(int64_t)Bx * i];
+#pragma unroll
+    for (int o = 0; o < kMixMax; ++o) {
+      if (o < nout) {
+        const double2 w = mix[o + ldmix * i];
+        acc[o].x += w.x * v.x - w.y * v.y;
+        acc[o].y += w.x * v.y + w.y * v.x;
+      }
+    }
+  }
+  double2* dst = out + bx + (int64_t)Bx * ldout * ax;
+#pragma unroll
+  for (int o = 0; o < kMixMax; ++o)
+    if (o < nout) dst[(int64_t)Bx * o] = acc[o];
+}
+
+__global__ void __launch_bounds__(kThreads) heff_full_f64_kernel(
+    const double2* __restrict__ L, const double2* __restrict__ W, const double2* __restrict__ R,
+    double2* __restrict__ H, int d, int Db, int wl, int Dk, int Dbr, int wr, int Dkr,
+    double offset, int64_t nrows, int64_t total) {
+  const int64_t gid = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (gid >= total) return;
+  const int64_t row = gid % nrows, col = gid / nrows;
+  const int sp = (int)(row % d);
+  const int b = (int)((row / d) % Db);
+  const int c = (int)(row / ((int64_t)d * Db));
+  const int s = (int)(col % d);
+  const int a = (int)((col / d) % Dk);
+  const int a2 = (int)(col / ((int64_t)d * Dk));
+  double2 acc{0.0, 0.0};
+  for (int al = 0; al < wl; ++al) {
+    const double2 l = L[b + (int64_t)Db * al + (int64_t)Db * wl * a];
+    for (int be = 0; be < wr; ++be) {
+      const double2 w = W[sp + d * al + (int64_t)d * wl * s + (int64_t)d * wl * d * be];
+      const double2 r = R[c + (int64_t)Dbr * be + (int64_t)Dbr * wr * a2];
+      const double2 lw{l.x * w.x - l.y * w.y, l.x * w.y + l.y * w.x};
+      acc.x += lw.x * r.x - lw.y * r.y;
+      acc.y += lw.x * r.y + lw.y * r.x;
+    }
+  }
+  if (row == col) acc.x += offset;
+  H[gid] = acc;
+}
+
+}  // namespace
+
+hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, int Bx, int nin,
+                           int nout, int64_t Ax, hipStream_t s) {
+  const int64_t total = (int64_t)Bx * Ax;
+  if (total <= 0) return hipSuccess;
+  const int64_t blocks = (total + kThreads - 1) / kThreads;
+  for (int o0 = 0; o0 < nout; o0 += kMixMax) {
+    const int no = min(kMixMax, nout - o0);
+    hipLaunchKernelGGL(zmix_f64_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, s, in,
+                       out + (int64_t)Bx * o0, mix + o0, Bx, nin, no, nout, nout, total);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_heff_full_f64(const double2* L, const double2* W, const double2* R, double2* H,
+                                int d, int Db, int wl, int Dk, int Dbr, int wr, int Dkr,
+                                double offset, hipStream_t s) {
+  const int64_t nrows = (int64_t)d * Db * Dbr, ncols = (int64_t)d * Dk * Dkr;
+  const int64_t total = nrows * ncols;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(heff_full_f64_kernel, dim3((unsigned)((total + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, L, W, R, H, d, Db, wl, Dk, Dbr, wr, Dkr, offset, nrows,
+                     total);
+  return hipGetLastError();
+}
+
+}  // namespace mpsd

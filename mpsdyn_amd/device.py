@@ -1,0 +1,286 @@
+"""ctypes binding of the device library's C-ABI (include/mpsd.h).
+
+This is plumbing for tests, benchmarks and Python callers: every computation happens in
+libmpsd.so.  Loading fails loudly (RuntimeError) if the library has not been built;
+there is no CPU fallback.
+
+Arrays cross the boundary as complex128 numpy arrays indexed like the reference's mwArray
+(first index first); they are flattened column-major (Fortran order), which is the
+library's storage order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmpsd.so")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_BF16X3 = 2
+
+_lib = None
+
+
+class MpsdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpsd error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libmpsd.so (once).  Raises RuntimeError when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run `python -m mpsdyn_amd.build` first")
+    lib = C.CDLL(LIB_PATH)
+    P, I, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "mpsd_ctx_create": (I, [I, C.POINTER(P)]),
+        "mpsd_ctx_destroy": (I, [P]),
+        "mpsd_ctx_sync": (I, [P]),
+        "mpsd_last_error": (C.c_char_p, [P]),
+        "mpsd_ctx_stream": (P, [P]),
+        "mpsd_ctx_device_bytes": (I64, [P]),
+        "mpsd_tensor_create": (I, [P, I, C.POINTER(I), C.POINTER(P)]),
+        "mpsd_tensor_destroy": (I, [P]),
+        "mpsd_tensor_rank": (I, [P]),
+        "mpsd_tensor_dims": (I, [P, C.POINTER(I)]),
+        "mpsd_tensor_size": (I64, [P]),
+        "mpsd_tensor_reshape": (I, [P, I, C.POINTER(I)]),
+        "mpsd_tensor_upload": (I, [P, P, I64]),
+        "mpsd_tensor_download": (I, [P, P, I64]),
+        "mpsd_tensor_copy": (I, [P, P]),
+        "mpsd_tensor_devptr": (P, [P]),
+        "mpsd_env_left": (I, [P, P, P, P, P, P]),
+        "mpsd_env_right": (I, [P, P, P, P, P, P]),
+        "mpsd_heff_create": (I, [P, P, P, P, D, I, C.POINTER(P)]),
+        "mpsd_heff_destroy": (I, [P]),
+        "mpsd_heff_size": (I64, [P]),
+        "mpsd_heff_apply": (I, [P, P, P, I, I64, I64, I]),
+        "mpsd_heff_full": (I, [P, P]),
+        "mpsd_heff_set_precision": (I, [P, I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _optional(lib)
+    _lib = lib
+    return lib
+
+
+def _optional(lib):
+    """Bind entry points that later parts of the library add (absent ones are skipped)."""
+    P, I, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    extra = {
+        "mpsd_ctx_set_timing": (I, [P, I]),
+        "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
+        "mpsd_ctx_set_tile": (I, [P, I, I]),
+    }
+    for name, (res, args) in extra.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+
+
+def exported_symbols():
+    return [n for n in _declared_symbols() if hasattr(load(), n)]
+
+
+def _declared_symbols():
+    """Function names declared in include/*.h."""
+    import glob
+    import re
+    names = []
+    inc = os.path.join(os.path.dirname(_HERE), "include")
+    for h in sorted(glob.glob(os.path.join(inc, "*.h"))):
+        txt = open(h).read()
+        names += re.findall(r"^\s*(?:const\s+)?[a-zA-Z_][\w\s\*]*?\b(mpsd_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+class Context:
+    """One device context (mpsd_ctx): a HIP stream plus scratch memory."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.mpsd_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise MpsdError(rc, "mpsd_ctx_create failed (is a HIP device visible?)")
+        self._h = h
+        self.lib = lib
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc):
+        if rc != 0:
+            raise MpsdError(rc, self.lib.mpsd_last_error(self._h).decode())
+        return rc
+
+    def sync(self):
+        self.check(self.lib.mpsd_ctx_sync(self._h))
+
+    def stream(self):
+        return self.lib.mpsd_ctx_stream(self._h)
+
+    def device_bytes(self):
+        return int(self.lib.mpsd_ctx_device_bytes(self._h))
+
+    def tensor(self, arr=None, dims=None):
+        return Tensor(self, arr=arr, dims=dims)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.mpsd_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # timing (HIP events around each launch, accumulated per kernel kind)
+    def set_timing(self, on=True):
+        self.check(self.lib.mpsd_ctx_set_timing(self._h, 1 if on else 0))
+
+    def timing(self, kind):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self.check(self.lib.mpsd_ctx_timing(self._h, kind, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def set_tile(self, tm, tn):
+        self.check(self.lib.mpsd_ctx_set_tile(self._h, tm, tn))
+
+
+class Tensor:
+    """HBM-resident complex128 tensor (mpsd_tensor)."""
+
+    def __init__(self, ctx, arr=None, dims=None):
+        self.ctx = ctx
+        if arr is not None:
+            arr = np.asarray(arr, dtype=np.complex128)
+            dims = arr.shape if arr.ndim else (1,)
+        dims = tuple(int(x) for x in dims)
+        cd = (C.c_int * max(1, len(dims)))(*dims)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.mpsd_tensor_create(ctx.handle, len(dims), cd, C.byref(h)))
+        self._h = h
+        if arr is not None:
+            self.upload(arr)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dims(self):
+        lib = self.ctx.lib
+        r = lib.mpsd_tensor_rank(self._h)
+        out = (C.c_int * 8)()
+        lib.mpsd_tensor_dims(self._h, out)
+        return tuple(out[i] for i in range(r))
+
+    @property
+    def size(self):
+        return int(self.ctx.lib.mpsd_tensor_size(self._h))
+
+    def devptr(self):
+        return self.ctx.lib.mpsd_tensor_devptr(self._h)
+
+    def upload(self, arr):
+        buf = np.ascontiguousarray(np.reshape(np.asarray(arr, dtype=np.complex128), -1, order="F"))
+        self.ctx.check(self.ctx.lib.mpsd_tensor_upload(self._h, buf.ctypes.data, buf.size))
+
+    def download(self):
+        buf = np.empty(self.size, dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.mpsd_tensor_download(self._h, buf.ctypes.data, buf.size))
+        return np.reshape(buf, self.dims, order="F")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx.lib.mpsd_tensor_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def env_left(ctx, envL, ket, W, bra, out):
+    ctx.check(ctx.lib.mpsd_env_left(ctx.handle, envL.handle, ket.handle, W.handle, bra.handle, out.handle))
+    return out
+
+
+def env_right(ctx, envR, ket, W, bra, out):
+    ctx.check(ctx.lib.mpsd_env_right(ctx.handle, envR.handle, ket.handle, W.handle, bra.handle, out.handle))
+    return out
+
+
+class Heff:
+    """Effective Hamiltonian handle (mpsd_heff): y = (L W R) x + offset x."""
+
+    def __init__(self, ctx, L, W, R, offset=0.0, precision=PREC_FP64):
+        self.ctx = ctx
+        self._keep = (L, W, R)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.mpsd_heff_create(ctx.handle, L.handle, W.handle, R.handle, float(offset),
+                                           int(precision), C.byref(h)))
+        self._h = h
+
+    @property
+    def size(self):
+        return int(self.ctx.lib.mpsd_heff_size(self._h))
+
+    def set_precision(self, p):
+        self.ctx.check(self.ctx.lib.mpsd_heff_set_precision(self._h, int(p)))
+
+    def apply(self, x):
+        """Host-buffer matvec (x flat or [n, nvec] in column-major)."""
+        x = np.asarray(x, dtype=np.complex128)
+        n = self.size
+        nvec = 1 if x.ndim == 1 else x.shape[1]
+        xb = np.asfortranarray(x.reshape(n, nvec, order="F"))
+        dims = self._out_len()
+        yb = np.empty((dims, nvec), dtype=np.complex128, order="F")
+        self.ctx.check(self.ctx.lib.mpsd_heff_apply(self._h, xb.ctypes.data, yb.ctypes.data, nvec, n, dims, 0))
+        return yb[:, 0] if x.ndim == 1 else yb
+
+    def apply_device(self, x_ptr, y_ptr, nvec=1, ldx=None, ldy=None):
+        n = self.size
+        self.ctx.check(self.ctx.lib.mpsd_heff_apply(self._h, x_ptr, y_ptr, nvec, ldx or n,
+                                                    ldy or self._out_len(), 1))
+
+    def _out_len(self):
+        L, W, R = self._keep
+        return W.dims[0] * L.dims[0] * R.dims[0]
+
+    def full(self):
+        nr = self._out_len()
+        out = np.empty((nr, self.size), dtype=np.complex128, order="F")
+        self.ctx.check(self.ctx.lib.mpsd_heff_full(self._h, out.ctypes.data))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx.lib.mpsd_heff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

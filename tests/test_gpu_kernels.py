@@ -1,0 +1,140 @@
+"""GPU parity of the device matvec, environment updates and dense Heff against the CPU
+oracle, through the C-ABI.  Tolerance (FP64 drop-in): relative 2-norm error <= 1e-12."""
+import numpy as np
+import pytest
+
+from oracle import mpsd_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from mpsdyn_amd import device as dev
+    c = dev.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def _operands(Db, Dbr, wl, wr, d=2, seed=0, Dk=None, Dkr=None):
+    Dk = Db if Dk is None else Dk
+    Dkr = Dbr if Dkr is None else Dkr
+    L = orc.complex_gaussian(seed + 1, (Db, wl, Dk))
+    W = orc.complex_gaussian(seed + 2, (d, wl, d, wr))
+    R = orc.complex_gaussian(seed + 3, (Dbr, wr, Dkr))
+    x = orc.complex_gaussian(seed + 4, (d * Dk * Dkr,))
+    return L, W, R, x
+
+
+SHAPES = [
+    # (Db, Dbr, wl, wr): edges, config-1 shape, ragged tiles, real DMRG shapes
+    (1, 2, 1, 3), (2, 1, 3, 1), (1, 1, 1, 1), (20, 20, 3, 3), (2, 4, 3, 3), (37, 50, 5, 5),
+    (64, 64, 3, 3), (128, 128, 3, 3), (100, 128, 5, 5), (16, 33, 5, 3),
+]
+
+
+@pytest.mark.parametrize("Db,Dbr,wl,wr", SHAPES)
+def test_heff_matches_oracle(ctx, Db, Dbr, wl, wr):
+    from mpsdyn_amd import device as dev
+    print(f"heff Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
+    L, W, R, x = _operands(Db, Dbr, wl, wr, seed=Db + 7 * Dbr)
+    tL, tW, tR = ctx.tensor(L), ctx.tensor(W), ctx.tensor(R)
+    h = dev.Heff(ctx, tL, tW, tR)
+    y = h.apply(x)
+    assert _rel(y, orc.heff_product(L, W, R, x)) <= TOL
+    h.close()
+
+
+def test_heff_offset_and_block(ctx):
+    from mpsdyn_amd import device as dev
+    L, W, R, _ = _operands(24, 24, 3, 3, seed=5)
+    X = np.stack([orc.complex_gaussian(40 + j, (2 * 24 * 24,)) for j in range(3)], axis=1)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R), offset=1.75)
+    Y = h.apply(X)
+    for j in range(3):
+        assert _rel(Y[:, j], orc.heff_product(L, W, R, X[:, j], offset=1.75)) <= TOL
+    h.close()
+
+
+def test_heff_known_answer_2x2(ctx):
+    """The reference's 2x2 complex product (testmwArray_new.cpp:176-246) as an Heff with
+    d = 1, W = 1, R = 1: y = L x, exact in FP64."""
+    from mpsdyn_amd import device as dev
+    A1 = orc.from_colmajor(np.array([1, 3, 2, 4]) * (1 + 1j), (2, 2))
+    A2 = 2 * A1
+    L = A1.reshape(2, 1, 2)
+    W = np.ones((1, 1, 1, 1), dtype=complex)
+    R = np.ones((1, 1, 1), dtype=complex)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R))
+    Y = h.apply(A2)
+    assert np.array_equal(Y, orc.from_colmajor(np.array([28j, 60j, 40j, 88j]), (2, 2)))
+    h.close()
+
+
+@pytest.mark.parametrize("Dl,Dr,wl,wr", [(1, 2, 1, 3), (20, 20, 3, 3), (37, 50, 5, 5), (2, 1, 3, 1),
+                                         (128, 96, 3, 3), (64, 64, 5, 5)])
+def test_env_updates_match_oracle(ctx, Dl, Dr, wl, wr):
+    from mpsdyn_amd import device as dev
+    print(f"env Dl={Dl} Dr={Dr} wl={wl} wr={wr}", flush=True)
+    d = 2
+    L = orc.complex_gaussian(1, (Dl, wl, Dl))
+    R = orc.complex_gaussian(2, (Dr, wr, Dr))
+    W = orc.complex_gaussian(3, (d, wl, d, wr))
+    ket = orc.complex_gaussian(4, (d, Dl, Dr))
+    bra = orc.complex_gaussian(5, (d, Dl, Dr))
+    tk, tb, tW = ctx.tensor(ket), ctx.tensor(bra), ctx.tensor(W)
+    out = ctx.tensor(dims=(1,))
+    dev.env_left(ctx, ctx.tensor(L), tk, tW, tb, out)
+    assert out.dims == (Dr, wr, Dr)
+    assert _rel(out.download(), orc.contract_left(L, ket, W, bra)) <= TOL
+    dev.env_right(ctx, ctx.tensor(R), tk, tW, tb, out)
+    assert out.dims == (Dl, wl, Dl)
+    assert _rel(out.download(), orc.contract_right(R, ket, W, bra)) <= TOL
+
+
+def test_heff_full_matches_oracle(ctx):
+    from mpsdyn_amd import device as dev
+    L, W, R, _ = _operands(10, 20, 3, 3, seed=9)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R), offset=-0.5)
+    H = h.full()
+    assert _rel(H, orc.heff_full(L, W, R, offset=-0.5)) <= TOL
+    h.close()
+
+
+def test_shape_errors_are_reported(ctx):
+    from mpsdyn_amd import device as dev
+    L, W, R, _ = _operands(8, 8, 3, 3)
+    bad_R = orc.complex_gaussian(1, (8, 5, 8))
+    with pytest.raises(dev.MpsdError) as e:
+        dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(bad_R))
+    assert e.value.code == -2
+    ket = orc.complex_gaussian(2, (2, 7, 8))
+    with pytest.raises(dev.MpsdError):
+        dev.env_left(ctx, ctx.tensor(L), ctx.tensor(ket), ctx.tensor(W), ctx.tensor(ket), ctx.tensor(dims=(1,)))
+
+
+@pytest.mark.parametrize("D,w", [(256, 3), (512, 3), (512, 5)])
+def test_heff_full_size(ctx, D, w):
+    """Real DMRG shapes (config 2: D=256/512 Ising w=3, Heisenberg w=5) against the oracle,
+    plus Hermiticity <u, H v> = <H u, v> for a Hermitian-structured operator."""
+    from mpsdyn_amd import device as dev
+    print(f"full-size D={D} w={w}", flush=True)
+    L, W, R, x = _operands(D, D, w, w, seed=D + w)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R))
+    assert _rel(h.apply(x), orc.heff_product(L, W, R, x)) <= TOL
+    h.close()
+    # Hermitian operands: L, R Hermitian in (bra, ket) per MPO index, W Hermitian in (s', s)
+    Lh = 0.5 * (L + np.conj(np.transpose(L, (2, 1, 0))))
+    Rh = 0.5 * (R + np.conj(np.transpose(R, (2, 1, 0))))
+    Wh = 0.5 * (W + np.conj(np.transpose(W, (2, 1, 0, 3))))
+    h = dev.Heff(ctx, ctx.tensor(Lh), ctx.tensor(Wh), ctx.tensor(Rh))
+    u = orc.complex_gaussian(77, (x.size,))
+    Hu, Hx = h.apply(u), h.apply(x)
+    assert abs(np.vdot(u, Hx) - np.vdot(Hu, x)) <= 1e-11 * np.linalg.norm(Hu) * np.linalg.norm(x)
+    h.close()
