@@ -10,7 +10,8 @@
 // staged once and reused NBW times.
 //
 // MFMA 16x16x4 f64 lane layout (wave64): A[m][k] comes from lane m + 16k, B[k][n] from
-// lane n + 16k, and result element (m, n) lands in lane n + 16*(m/4), register m%4.
+// lane n + 16k, and result element (m, n) lands in lane n + 16*(m%4), register m/4
+// (the f64 form interleaves rows across the four lane groups, unlike the f32 forms).
 
 #include "mpsd_internal.h"
 
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
 
   // Epilogue: C = acc (+ beta * Dadd).
   const int lc = lane & 15;
-  const int lr = (lane >> 4) * 4;
+  const int lr = lane >> 4;
 #pragma unroll
   for (int j = 0; j < NBW; ++j) {
     if (j >= nbw) break;
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
         const int gn = n0 + wn * WTN + q * 16 + lc;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int gm = m0 + wm * WTM + i * 16 + lr + rr;
+          const int gm = m0 + wm * WTM + i * 16 + lr + 4 * rr;
           if (gm < p.M && gn < p.N) {
             const int64_t off = gm * p.sCm + gn * p.sCn;
             double2 v{accr[j][i][q][rr], acci[j][i][q][rr]};

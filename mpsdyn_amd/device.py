@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -117,10 +118,16 @@ class Context:
             raise MpsdError(rc, "mpsd_ctx_create failed (is a HIP device visible?)")
         self._h = h
         self.lib = lib
+        # Tensors and Heff handles of this context; closed before the context itself.
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
         return self._h
+
+    @property
+    def alive(self):
+        return getattr(self, "_h", None) is not None
 
     def check(self, rc):
         if rc != 0:
@@ -141,8 +148,20 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
+            kids = list(self._children)
+            for k in kids:
+                if isinstance(k, Heff):
+                    k.close()
+            for k in kids:
+                k.close()
             self.lib.mpsd_ctx_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -177,6 +196,7 @@ class Tensor:
         h = C.c_void_p()
         ctx.check(ctx.lib.mpsd_tensor_create(ctx.handle, len(dims), cd, C.byref(h)))
         self._h = h
+        ctx._children.add(self)
         if arr is not None:
             self.upload(arr)
 
@@ -209,9 +229,9 @@ class Tensor:
         return np.reshape(buf, self.dims, order="F")
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and self.ctx.alive:
             self.ctx.lib.mpsd_tensor_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -240,6 +260,7 @@ class Heff:
         ctx.check(ctx.lib.mpsd_heff_create(ctx.handle, L.handle, W.handle, R.handle, float(offset),
                                            int(precision), C.byref(h)))
         self._h = h
+        ctx._children.add(self)
 
     @property
     def size(self):
@@ -275,9 +296,9 @@ class Heff:
         return out
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and self.ctx.alive:
             self.ctx.lib.mpsd_heff_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
