@@ -132,6 +132,39 @@ int mpsd_heff_full(mpsd_heff* h, double* host_out);
 /* Change the arithmetic of an existing handle (MPSD_PREC_*). */
 int mpsd_heff_set_precision(mpsd_heff* h, int precision);
 
+/* ---- device vectors (FP64 eigensolver support) ------------------------------------------
+ * The orthogonalisation, Rayleigh-Ritz and residual arithmetic that PRIMME/ARPACK do on the
+ * host (src/wrapper/eigs.cpp:71-422) run on device vectors here.  Vectors are complex128
+ * device buffers of n complex elements; V holds m of them with leading dimension ldv.
+ * Reductions have a fixed order that depends only on n (bit-reproducible). */
+/* out_host[j] = sum_i conj(V[j*ldv + i]) * w[i], j < m (synchronises the stream). */
+int mpsd_vec_dots(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* w, int64_t n,
+                  double* out_host);
+/* w[i] += sum_j V[j*ldv + i] * c[j], j < m <= 1024, c a host array of m complex. */
+int mpsd_vec_gemv(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_host, double* w,
+                  int64_t n);
+/* y = a*x + b*y with complex scalars a, b given as {re, im}. */
+int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n);
+/* y = x ; x = 0 (plain stores: a NaN in the old y does not survive). */
+int mpsd_vec_copy(mpsd_ctx* ctx, const double* x, double* y, int64_t n);
+int mpsd_vec_zero(mpsd_ctx* ctx, double* x, int64_t n);
+/* out = sum |x_i|^2 (synchronises the stream). */
+int mpsd_vec_norm2(mpsd_ctx* ctx, const double* x, int64_t n, double* out);
+
+/* ---- gauge steps (kept in HBM) -----------------------------------------------------------
+ * site [d, Dl, Dr] := Q of the QR of site viewed (d*Dl) x Dr; R_out := R (k x Dr),
+ * k = min(d*Dl, Dr), and the site's right bond becomes k.  Replaces Site::gaugeR with
+ * cutD = 0 (src/core/cpp/Site.cpp:179-208) over wrapper::qr (src/wrapper/qr.cpp:33-111),
+ * including the edge phase convention for a 1x1 factor (Site.cpp:202-207). */
+int mpsd_site_gauge_r(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out);
+/* site := Q of the LQ of site permuted to Dl x (d*Dr); L_out := L (Dl x k).  Replaces
+ * Site::gaugeL (Site.cpp:210-238) over wrapper::lq (src/wrapper/qr.cpp:113-188). */
+int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out);
+/* site[s, :, c] := R . site[s, :, c]  -- Site::reduceDimR (Site.cpp:240-261). */
+int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R);
+/* site[s, a, :] := site[s, a, :] . L  -- Site::reduceDimL (Site.cpp:263-283). */
+int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L);
+
 #ifdef __cplusplus
 }
 #endif

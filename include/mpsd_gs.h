@@ -1,0 +1,77 @@
+/*
+ * mpsd_gs.h -- C-ABI of whole ground-state solves on the device (the C++ Contractor of
+ * include/mpsdyn/mpsdyn.h behind plain C).  One call = build the model MPO, draw a seeded
+ * random MPS, and run Contractor::findGroundState.  Mirrors the reference's ground-state
+ * drivers: src/programs/ising/testIsing.cpp:71-91, src/programs/heisenberg/testHeisenberg.cpp:84-110,
+ * src/programs/banulsm/schwinger/schwingerPD.cpp:51,93-102, calling
+ * Contractor::findGroundState (src/core/cpp/Contractor.cpp:1352-1533).
+ */
+#ifndef MPSD_GS_H
+#define MPSD_GS_H
+
+#include <stdint.h>
+
+#include "mpsd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPSD_MODEL_ISING 0      /* p = {J, g, h, K}: sum J ZZ + g X + h Z (IsingHamiltonian.h:19-25) */
+#define MPSD_MODEL_HEISENBERG 1 /* p = {Jx, Jy, Jz, h}: S = sigma/2 (HeisenbergHamiltonian.cpp:266-331) */
+#define MPSD_MODEL_SCHWINGER 2  /* p = {mu, x, l0, alpha} (SchwingerHamiltonian.cpp:13-21) */
+
+typedef struct mpsd_gs_config {
+  int model;
+  int L;              /* chain length */
+  int D;              /* MPS bond dimension */
+  double p[4];        /* model parameters, see MPSD_MODEL_* */
+  double offset;      /* findGroundState offset argument */
+  double convtol;     /* Contractor::setConvTol (the configs use 1e-12 or 1e-10) */
+  double eigtol;      /* residual tolerance of each local eigensolve (0: 1e-10) */
+  int max_halfsweeps; /* <= 0: until converged */
+  int precision;      /* MPSD_PREC_* of the matvecs */
+  uint64_t seed;      /* random initial MPS */
+} mpsd_gs_config;
+
+#define MPSD_GS_MAX_HISTORY 256
+
+typedef struct mpsd_gs_result {
+  double energy;          /* ground-state energy (findGroundState *lambda) */
+  double initial_energy;  /* <psi0|H|psi0> of the start state */
+  int halfsweeps;
+  int64_t matvecs;
+  int64_t env_updates;
+  double seconds;         /* wall time of findGroundState */
+  double residual;        /* last local eigen-residual */
+  int64_t device_bytes;   /* HBM held by the context after the solve */
+  int n_history;
+  double history[MPSD_GS_MAX_HISTORY]; /* energy at the end of each half-sweep */
+} mpsd_gs_result;
+
+/* Run one solve on ctx.  Returns MPSD_OK or an MPSD_ERR_* code (message: mpsd_last_error). */
+int mpsd_gs_solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, mpsd_gs_result* out);
+
+/* Same, and also write the final MPS with MPS::exportMPS's binary layout
+ * (src/core/cpp/MPS.cpp:440-459) to `export_path` (NULL: no export). */
+int mpsd_gs_solve_export(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path,
+                         mpsd_gs_result* out);
+
+/* <psi|H|psi> of an MPS file in exportMPS layout for the model of cfg (Contractor::contract,
+ * src/core/cpp/Contractor.cpp:528), e.g. to check a checkpoint. */
+int mpsd_gs_expectation(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mps_path, double* re,
+                        double* im);
+
+/* Lowest eigenpair of an Heff by the device FP64 Lanczos solver.  Replaces
+ * Contractor::solveEigs(multi, 1, "SR", ...) (src/core/cpp/Contractor.cpp:3067-3138) and the
+ * eigensolver wrappers behind it (src/wrapper/eigs.cpp:71-422).  x_dev: device vector of
+ * mpsd_heff_size() complex numbers, the start vector on entry (warm start), the unit
+ * eigenvector on exit.  tol: residual tolerance relative to max(1, |value|). */
+int mpsd_heff_eigs_lowest(mpsd_heff* h, double* x_dev, double tol, int max_krylov, double* value,
+                          double* residual, int* matvecs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSD_GS_H */

@@ -1,0 +1,357 @@
+// mpsdyn.h -- C++ host API of the MI355X ground-state path.
+//
+// Same class and method names, argument meaning and index conventions as the reference
+// library (mcbapo/mpsdyn), for the classes on the variational ground-state path:
+//   mwArray   (src/wrapper/mwArray.h)     host dense complex tensor, column-major
+//   Operator  (src/core/headers/Operator.h)   W[d_up, Dl, d_down, Dr]
+//   MPO       (src/core/headers/MPO.h)
+//   Site/MPS  (src/core/headers/Site.h, MPS.h)  -- site tensors live in HBM here
+//   TmpManager (src/core/headers/TmpManager.h) -- environments live in HBM here
+//   Multiplier / TensorMultiplierOffset (src/wrapper/multipliers/*.h)
+//   Contractor (src/core/headers/Contractor.h): findGroundState, contract, setters
+//   IsingHamiltonian, HeisenbergHamiltonian, SchwingerHamiltonian (src/models/headers/*.h)
+// Every device operation goes through the C-ABI of include/mpsd.h.  Errors raise
+// mpsdyn::Error (the reference calls exit(212)/exit(1) on the same conditions).
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpsd.h"
+
+namespace mpsdyn {
+
+typedef std::complex<double> complex_t;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Throw Error(rc, mpsd_last_error(ctx)) when rc != 0.
+void check(mpsd_ctx* ctx, int rc, const char* what);
+
+// Process-wide default device context (device 0), created on first use.  The reference
+// keeps its state in the Contractor singleton (Contractor.cpp:49-53); here every object
+// may instead be bound to an explicit context (one per concurrent solve).
+mpsd_ctx* default_ctx();
+
+// ------------------------------------------------------------------------------ mwArray
+class mwArray {
+ public:
+  mwArray() {}
+  explicit mwArray(const std::vector<int>& dims);
+  mwArray(const std::vector<int>& dims, const complex_t* data);
+  explicit mwArray(complex_t scalar);
+  int getRank() const { return (int)dims_.size(); }
+  int getDimension(int k) const { return k < (int)dims_.size() ? dims_[k] : 1; }
+  const std::vector<int>& getDimensions() const { return dims_; }
+  int64_t getNrComponents() const { return (int64_t)data_.size(); }
+  complex_t getElement(const std::vector<int>& idx) const;
+  void setElement(complex_t v, const std::vector<int>& idx);
+  complex_t getElement(int64_t k) const { return data_[k]; }
+  void reshape(const std::vector<int>& dims);  // column-major, free
+  void permute(const std::vector<int>& order, bool conj = false);  // 1-based order
+  void conjugate();
+  void fillWithZero();
+  const complex_t* data() const { return data_.data(); }
+  complex_t* data() { return data_.data(); }
+  const double* getComponents() const { return reinterpret_cast<const double*>(data_.data()); }
+  // binary layout of mwArray::save/load (src/wrapper/mwArray.cpp:382-443)
+  void save(std::ofstream& out) const;
+  void load(std::ifstream& in);
+
+ private:
+  std::vector<int> dims_;
+  std::vector<complex_t> data_;
+};
+
+// A device tensor owned by a host object.
+struct DevTensor {
+  mpsd_tensor* t = nullptr;
+  mpsd_ctx* ctx = nullptr;
+  DevTensor() {}
+  DevTensor(mpsd_ctx* c, const std::vector<int>& dims);
+  DevTensor(mpsd_ctx* c, const mwArray& a);
+  ~DevTensor();
+  DevTensor(const DevTensor&) = delete;
+  DevTensor& operator=(const DevTensor&) = delete;
+  DevTensor(DevTensor&& o) noexcept { std::swap(t, o.t), std::swap(ctx, o.ctx); }
+  DevTensor& operator=(DevTensor&& o) noexcept {
+    std::swap(t, o.t);
+    std::swap(ctx, o.ctx);
+    return *this;
+  }
+  std::vector<int> dims() const;
+  int64_t size() const { return t ? mpsd_tensor_size(t) : 0; }
+  double* ptr() const { return (double*)mpsd_tensor_devptr(t); }
+  mwArray download() const;
+  void upload(const mwArray& a);
+};
+
+// ------------------------------------------------------------------------------ Operator / MPO
+class Operator {
+ public:
+  explicit Operator(const mwArray& data);  // [du, Dl, dd, Dr]; lower ranks padded with 1
+  const mwArray& getFullData() const { return data_; }
+  int getDimension(int k) const { return data_.getDimension(k); }
+  // The operator's tensor on a context's device (uploaded once per context).
+  const mpsd_tensor* device(mpsd_ctx* ctx) const;
+
+ private:
+  mwArray data_;
+  mutable std::vector<std::pair<mpsd_ctx*, std::shared_ptr<DevTensor>>> dev_;
+};
+
+class MPO {
+ public:
+  explicit MPO(int length = 0) : ops_(length) {}
+  void initLength(int length) { ops_.assign(length, nullptr); }
+  int getLength() const { return (int)ops_.size(); }
+  // MPO::setOp (src/core/cpp/MPO.cpp:107-128): takes ownership when myop, else shares.
+  void setOp(int k, Operator* op, bool myop = true);
+  void setOp(int k, const Operator* shared) { ops_[k] = std::shared_ptr<const Operator>(shared, [](const Operator*) {}); }
+  void setOpShared(int k, const std::shared_ptr<const Operator>& op) { ops_[k] = op; }
+  const Operator& getOp(int k) const { return *ops_[k]; }
+  std::vector<int> getDimensions() const;  // physical dimension per site
+
+ private:
+  std::vector<std::shared_ptr<const Operator>> ops_;
+};
+
+// ------------------------------------------------------------------------------ Site / MPS
+class Site {
+ public:
+  Site(mpsd_ctx* ctx, int pos, const std::vector<int>& dims);  // pos is 1-based like Site_t
+  int getPos() const { return pos_; }
+  int getd() const { return dims()[0]; }
+  int getDl() const { return dims()[1]; }
+  int getDr() const { return dims()[2]; }
+  std::vector<int> dims() const;
+  mwArray getA() const { return dev_.download(); }  // host copy
+  void setA(const mwArray& a);
+  mpsd_tensor* device() const { return dev_.t; }
+  void gaugeR(DevTensor& rightterm);   // Site::gaugeR, cutD = 0 (Site.cpp:179-208)
+  void gaugeL(DevTensor& leftterm);    // Site::gaugeL (Site.cpp:210-238)
+  void reduceDimR(const DevTensor& leftterm);   // Site.cpp:240-261
+  void reduceDimL(const DevTensor& rightterm);  // Site.cpp:263-283
+
+ private:
+  mpsd_ctx* ctx_;
+  int pos_;
+  DevTensor dev_;
+};
+
+enum Gauge { gNone, gLeft, gRight, gBoth };
+
+class MPS {
+ public:
+  MPS() {}
+  // MPS(len, bond, d) (src/core/cpp/MPS.cpp:15-35): sites [d,1,D], [d,D,D]..., [d,D,1].
+  MPS(int length, int bond, int d, mpsd_ctx* ctx = nullptr);
+  int getLength() const { return (int)A_.size(); }
+  int getBond() const { return bond_; }
+  const Site& getA(int pos) const { return *A_[pos]; }
+  Site& site(int pos) { return *A_[pos]; }
+  void setA(int pos, const mwArray& a) { A_[pos]->setA(a); }
+  mpsd_ctx* ctx() const { return ctx_; }
+  // Seeded complex-Gaussian state (SplitMix64 + Box-Muller, site k draws from seed*1000003+k)
+  // at the canonical bond profile min(d^k, D, d^(L-k)), then right-gauged.  The reference
+  // fills uniform [0,1) values from libc random() (MPS.cpp:282-294).
+  void setRandomState(uint64_t seed = 1);
+  void gaugeCond(char dir, bool normal = true);        // MPS.cpp:306-364
+  void gaugeCond(int pos, char dir, bool next = false);  // MPS.cpp:366-419
+  Gauge getGauge() const { return gauge_; }
+  // Binary format of MPS::exportMPS/importMPS (MPS.cpp:442-509, Site.cpp:561-578).
+  void exportMPS(const char* filename) const;
+  void importMPS(const char* filename);
+
+ private:
+  DevTensor& factor();  // reusable device buffer for the QR/LQ factor of a gauge move
+  mpsd_ctx* ctx_ = nullptr;
+  int bond_ = 0;
+  double normfact_ = 1.0;
+  Gauge gauge_ = gNone;
+  std::vector<std::unique_ptr<Site>> A_;
+  std::unique_ptr<DevTensor> factor_;
+};
+
+// ------------------------------------------------------------------------------ environments
+// TmpManager (src/core/cpp/TmpManager.cpp:22-211): operL/operR slots with up-to-date flags;
+// edges are the scalar 1.  The slots are HBM tensors that keep their capacity.
+class TmpManager {
+ public:
+  TmpManager(int length, mpsd_ctx* ctx);
+  bool isuptodateL(int k) const { return upL_[k]; }
+  bool isuptodateR(int k) const { return upR_[k]; }
+  void outofdateL(int k) { upL_[k] = false; }
+  void outofdateR(int k) { upR_[k] = false; }
+  void storedL(int k) { upL_[k] = true; }
+  void storedR(int k) { upR_[k] = true; }
+  DevTensor& operL(int k) { return L_[k]; }
+  DevTensor& operR(int k) { return R_[k]; }
+  int64_t deviceBytes() const;
+
+ private:
+  std::vector<DevTensor> L_, R_;
+  std::vector<bool> upL_, upR_;
+};
+
+// ------------------------------------------------------------------------------ Multiplier
+// Multiplier (src/wrapper/multipliers/Multiplier.h:18-49) with a device product: x and y
+// are device vectors (complex128) of getSize() elements, ordered on the context's stream.
+class Multiplier {
+ public:
+  virtual ~Multiplier() {}
+  virtual int64_t getSize() const = 0;
+  virtual void product_device(const double* x, double* y) = 0;
+  // The reference's host form: product(const mwArray& x, mwArray& y).
+  virtual void product(const mwArray& x, mwArray& y);
+  virtual void getFullTensor(mwArray& result) = 0;
+  virtual mpsd_ctx* ctx() const = 0;
+};
+
+// TensorMultiplierOffset (TensorMultiplierOffset.cpp:8-25): Heff + offset, on the device.
+class TensorMultiplierOffset : public Multiplier {
+ public:
+  TensorMultiplierOffset(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W, const mpsd_tensor* R,
+                         double offset, int precision = MPSD_PREC_FP64);
+  ~TensorMultiplierOffset();
+  int64_t getSize() const override;
+  void product_device(const double* x, double* y) override;
+  void getFullTensor(mwArray& result) override;
+  mpsd_ctx* ctx() const override { return ctx_; }
+  void setPrecision(int p);
+
+ private:
+  mpsd_ctx* ctx_;
+  mpsd_heff* h_ = nullptr;
+  std::vector<int> outdims_;
+};
+
+// ------------------------------------------------------------------------------ eigensolver
+// Lowest eigenpair of a Hermitian Multiplier by restarted Lanczos with full (two-pass)
+// reorthogonalisation on device vectors; the tridiagonal Rayleigh-Ritz step is FP64 on the
+// host.  Stands in for PRIMME/ARPACK (src/wrapper/eigs.cpp:71-422), whose sources are not
+// part of the reference.
+struct EigStats {
+  double value = 0.0;
+  double residual = 0.0;  // ||H x - value x|| (Lanczos estimate)
+  int matvecs = 0;
+  int restarts = 0;
+  bool converged = false;
+};
+
+class LanczosSolver {
+ public:
+  explicit LanczosSolver(mpsd_ctx* ctx) : ctx_(ctx) {}
+  ~LanczosSolver();
+  // x: device vector, initial guess on entry (warm start), eigenvector on exit (unit norm).
+  EigStats lowest(Multiplier& H, double* x, double tol, int max_krylov = 30, int max_restarts = 40);
+
+ private:
+  mpsd_ctx* ctx_;
+  DevTensor V_;
+  int64_t cap_ = 0;
+};
+
+// Lowest eigenpair of a symmetric tridiagonal matrix (diag a[0..m), offdiag b[0..m-1)):
+// bisection on the Sturm count, then inverse iteration.  Returns theta; s gets the unit
+// eigenvector.
+double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, std::vector<double>& s);
+
+// ------------------------------------------------------------------------------ Contractor
+enum EigSolver { primme, arpack, fulleig, lanczos };
+
+struct SolveStats {
+  int halfsweeps = 0;
+  int64_t matvecs = 0;
+  int64_t env_updates = 0;
+  double final_residual = 0.0;
+  std::vector<double> energies;  // energy at the end of each half-sweep
+};
+
+class Contractor {
+ public:
+  explicit Contractor(mpsd_ctx* ctx = nullptr);
+  ~Contractor();
+  static Contractor& theContractor();  // Contractor.cpp:49-53
+  void setConvTol(double tol) { convtol_ = tol; }
+  void setEigTol(double tol) { eigtol_ = tol; }
+  void setEigenSolver(EigSolver s) { solver_ = s; }
+  double getConvTol() const { return convtol_; }
+  // Limit on half-sweeps (the reference has none); <= 0 means unlimited.
+  void setMaxHalfSweeps(int n) { max_halfsweeps_ = n; }
+  void setPrecision(int p) { precision_ = p; }
+  // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).
+  void findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset = 0.,
+                       int knr = 1, const std::string& tmpfile = "", int freqSv = 200, bool useLM = false);
+  // <bra|ops|ket> via a full left-environment sweep (Contractor.cpp:528).
+  complex_t contract(const MPS& ket, const MPO& ops, const MPS& bra);
+  const SolveStats& lastStats() const { return stats_; }
+
+ private:
+  void calculateL(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra);
+  void calculateR(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra);
+  mpsd_ctx* ctx_;
+  double convtol_ = 1e-5, eigtol_ = 0.0;
+  EigSolver solver_ = primme;
+  int max_halfsweeps_ = 0;
+  int precision_ = MPSD_PREC_FP64;
+  SolveStats stats_;
+  std::unique_ptr<LanczosSolver> eig_;
+};
+
+// ------------------------------------------------------------------------------ models
+class Hamiltonian {
+ public:
+  virtual ~Hamiltonian() {}
+  virtual const MPO& getHMPO() const = 0;
+};
+
+// H = sum J sz sz + g sx + h sz (+K) (IsingHamiltonian.cpp:17-121).
+class IsingHamiltonian : public Hamiltonian {
+ public:
+  IsingHamiltonian(int N, int d, double J, double g, double h = 0., double K = 0.);
+  const MPO& getHMPO() const override { return hamil_; }
+
+ private:
+  MPO hamil_;
+};
+
+// H = sum Jx SxSx + Jy SySy + Jz SzSz + h Sz, S = sigma/2 (HeisenbergHamiltonian.cpp:27-54,266-331).
+class HeisenbergHamiltonian : public Hamiltonian {
+ public:
+  HeisenbergHamiltonian(int L, double Jx, double Jy, double Jz, double h, int d = 2, double offset = 0.);
+  HeisenbergHamiltonian(int L, const std::vector<double>& Jx, const std::vector<double>& Jy,
+                        const std::vector<double>& Jz, const std::vector<double>& h, int d = 2,
+                        double offset = 0.);
+  const MPO& getHMPO() const override { return hamil_; }
+
+ private:
+  void init(const std::vector<double>& Jx, const std::vector<double>& Jy, const std::vector<double>& Jz,
+            const std::vector<double>& h, double offset);
+  MPO hamil_;
+};
+
+// Lattice Schwinger model (SchwingerHamiltonian.cpp:13-121).
+class SchwingerHamiltonian : public Hamiltonian {
+ public:
+  SchwingerHamiltonian(int N, double mu, double x, double l0, double alpha, double offset = 0,
+                       double nu = 0., double y = 1.);
+  const MPO& getHMPO() const override { return hamil_; }
+
+ private:
+  MPO hamil_;
+};
+
+// SplitMix64 + Box-Muller complex Gaussians, column-major over dims (same draws as the
+// Python test generator).
+mwArray complex_gaussian(uint64_t seed, const std::vector<int>& dims);
+
+}  // namespace mpsdyn

@@ -30,7 +30,11 @@ LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        # host-only C++ (no HIP headers): the system compiler
+        cmd = ["g++"] + [f for f in COMMON if f != "-Wno-unused-but-set-variable"] + ["-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "'-packed-fp32-ops' is not a recognized feature"
     err = "\n".join(l for l in r.stderr.splitlines() if noise not in l)

@@ -241,10 +241,16 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   (void)flush_timing(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < mpsd_ctx::kScratch; ++i)
     if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+  for (int i = 0; i < 2; ++i)
     if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
-  }
+  if (ctx->dots) (void)hipFree(ctx->dots);
+  if (ctx->coef) (void)hipFree(ctx->coef);
+  if (ctx->coef_host) (void)hipHostFree(ctx->coef_host);
+  for (hipEvent_t e : ctx->coef_ev)
+    if (e) (void)hipEventDestroy(e);
+  destroy_blas(ctx);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPSD_OK;
@@ -288,8 +294,10 @@ void* mpsd_ctx_stream(mpsd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr
 
 int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx) {
   if (!ctx) return 0;
-  return ctx->tensor_bytes + (int64_t)ctx->scratch_bytes[0] + (int64_t)ctx->scratch_bytes[1] +
-         (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1];
+  int64_t b = ctx->tensor_bytes + (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1] +
+              (int64_t)ctx->dots_bytes;
+  for (int i = 0; i < mpsd_ctx::kScratch; ++i) b += (int64_t)ctx->scratch_bytes[i];
+  return b;
 }
 
 int mpsd_tensor_create(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** out) {

@@ -1,0 +1,142 @@
+// C-ABI over the C++ Contractor (include/mpsd_gs.h).  C++ exceptions stop here and become
+// status codes plus the context's error message.
+
+#include <chrono>
+#include <cstring>
+
+#include "mpsd_gs.h"
+#include "mpsd_internal.h"
+#include "mpsdyn/mpsdyn.h"
+
+using namespace mpsdyn;
+
+namespace {
+
+std::unique_ptr<Hamiltonian> make_model(const mpsd_gs_config* c) {
+  switch (c->model) {
+    case MPSD_MODEL_ISING:
+      return std::unique_ptr<Hamiltonian>(new IsingHamiltonian(c->L, 2, c->p[0], c->p[1], c->p[2], c->p[3]));
+    case MPSD_MODEL_HEISENBERG:
+      return std::unique_ptr<Hamiltonian>(new HeisenbergHamiltonian(c->L, c->p[0], c->p[1], c->p[2], c->p[3]));
+    case MPSD_MODEL_SCHWINGER:
+      return std::unique_ptr<Hamiltonian>(new SchwingerHamiltonian(c->L, c->p[0], c->p[1], c->p[2], c->p[3]));
+    default:
+      throw Error(MPSD_ERR_ARG, "unknown model");
+  }
+}
+
+int fail(mpsd_ctx* ctx, const Error& e) {
+  if (ctx) ctx->err = e.what();
+  return e.code ? e.code : MPSD_ERR_ARG;
+}
+
+int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mpsd_gs_result* out) {
+  if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
+  if (cfg->L < 2 || cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need L >= 2 and D >= 1");
+  memset(out, 0, sizeof(*out));
+  try {
+    auto H = make_model(cfg);
+    MPS mps(cfg->L, cfg->D, 2, ctx);
+    mps.setRandomState(cfg->seed);
+    Contractor c(ctx);
+    c.setConvTol(cfg->convtol > 0 ? cfg->convtol : 1e-5);
+    c.setEigTol(cfg->eigtol);
+    c.setMaxHalfSweeps(cfg->max_halfsweeps);
+    c.setPrecision(cfg->precision);
+    out->initial_energy = c.contract(mps, H->getHMPO(), mps).real();
+    double E = 0.0;
+    auto t0 = std::chrono::steady_clock::now();
+    c.findGroundState(H->getHMPO(), cfg->D, &E, mps, cfg->offset);
+    check(ctx, mpsd_ctx_sync(ctx), "sync");
+    auto t1 = std::chrono::steady_clock::now();
+    const SolveStats& st = c.lastStats();
+    out->energy = E;
+    out->halfsweeps = st.halfsweeps;
+    out->matvecs = st.matvecs;
+    out->env_updates = st.env_updates;
+    out->residual = st.final_residual;
+    out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->device_bytes = mpsd_ctx_device_bytes(ctx);
+    out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
+    for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+    if (export_path) mps.exportMPS(export_path);
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+// Non-owning Multiplier over a C-ABI Heff handle.
+class HeffRef : public Multiplier {
+ public:
+  explicit HeffRef(mpsd_heff* h) : h_(h) {}
+  int64_t getSize() const override { return mpsd_heff_size(h_); }
+  void product_device(const double* x, double* y) override {
+    const int64_t n = getSize();
+    check(h_->ctx, mpsd_heff_apply(h_, x, y, 1, n, n, 1), "heff_apply");
+  }
+  void getFullTensor(mwArray& r) override {
+    const int n = (int)getSize();
+    r = mwArray(std::vector<int>{n, n});
+    check(h_->ctx, mpsd_heff_full(h_, (double*)r.data()), "heff_full");
+  }
+  mpsd_ctx* ctx() const override { return h_->ctx; }
+
+ private:
+  mpsd_heff* h_;
+};
+
+}  // namespace
+
+extern "C" {
+
+int mpsd_heff_eigs_lowest(mpsd_heff* h, double* x_dev, double tol, int max_krylov, double* value, double* residual,
+                          int* matvecs) {
+  if (!h || !x_dev || !value || !residual || !matvecs) return MPSD_ERR_ARG;
+  try {
+    HeffRef H(h);
+    LanczosSolver solver(h->ctx);
+    EigStats st = solver.lowest(H, x_dev, tol > 0 ? tol : 1e-10, max_krylov > 1 ? max_krylov : 30);
+    *value = st.value;
+    *residual = st.residual;
+    *matvecs = st.matvecs;
+    return st.converged ? MPSD_OK : mpsd::set_err(h->ctx, MPSD_ERR_NUMERIC, "eigensolver did not converge");
+  } catch (const Error& e) {
+    return fail(h->ctx, e);
+  } catch (const std::exception& e) {
+    h->ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_gs_solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, mpsd_gs_result* out) {
+  return solve(ctx, cfg, nullptr, out);
+}
+
+int mpsd_gs_solve_export(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mpsd_gs_result* out) {
+  return solve(ctx, cfg, export_path, out);
+}
+
+int mpsd_gs_expectation(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mps_path, double* re, double* im) {
+  if (!ctx || !cfg || !mps_path || !re || !im) return MPSD_ERR_ARG;
+  try {
+    auto H = make_model(cfg);
+    MPS mps(1, 1, 2, ctx);
+    mps.importMPS(mps_path);
+    Contractor c(ctx);
+    complex_t e = c.contract(mps, H->getHMPO(), mps);
+    *re = e.real();
+    *im = e.imag();
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+// Contractor: the single-site ground-state sweep over device-resident MPS and
+// environments, and the model MPO builders of the benchmark configurations.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+
+#include "mpsdyn/mpsdyn.h"
+
+namespace mpsdyn {
+
+Contractor::Contractor(mpsd_ctx* ctx) : ctx_(ctx ? ctx : default_ctx()), eig_(new LanczosSolver(ctx_)) {}
+
+Contractor::~Contractor() {}
+
+Contractor& Contractor::theContractor() {
+  static Contractor c(nullptr);
+  return c;
+}
+
+void Contractor::calculateL(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra) {
+  // Contractor::calculateL (Contractor.cpp:723-742), iterative instead of recursive.
+  int first = pos;
+  while (first > 0 && !mgr.isuptodateL(first)) --first;
+  for (int k = first + 1; k <= pos; ++k) {
+    check(ctx_,
+          mpsd_env_left(ctx_, mgr.operL(k - 1).t, ket.getA(k - 1).device(), ops.getOp(k - 1).device(ctx_),
+                        bra.getA(k - 1).device(), mgr.operL(k).t),
+          "env_left");
+    ++stats_.env_updates;
+    mgr.storedL(k);
+  }
+}
+
+void Contractor::calculateR(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra) {
+  // Contractor::calculateR (Contractor.cpp:744-764), iterative instead of recursive.
+  const int L = ket.getLength();
+  int last = pos;
+  while (last < L - 1 && !mgr.isuptodateR(last)) ++last;
+  for (int k = last - 1; k >= pos; --k) {
+    check(ctx_,
+          mpsd_env_right(ctx_, mgr.operR(k + 1).t, ket.getA(k + 1).device(), ops.getOp(k + 1).device(ctx_),
+                         bra.getA(k + 1).device(), mgr.operR(k).t),
+          "env_right");
+    ++stats_.env_updates;
+    mgr.storedR(k);
+  }
+}
+
+complex_t Contractor::contract(const MPS& ket, const MPO& ops, const MPS& bra) {
+  const int L = ket.getLength();
+  TmpManager mgr(L + 1, ctx_);
+  for (int k = 1; k <= L; ++k)
+    check(ctx_,
+          mpsd_env_left(ctx_, mgr.operL(k - 1).t, ket.getA(k - 1).device(), ops.getOp(k - 1).device(ctx_),
+                        bra.getA(k - 1).device(), mgr.operL(k).t),
+          "contract");
+  mwArray r = mgr.operL(L).download();
+  return r.getElement(0);
+}
+
+void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset, int knr,
+                                 const std::string& tmpfile, int freqSv, bool useLM) {
+  (void)knr;
+  (void)D;
+  if (useLM) throw Error(MPSD_ERR_UNSUPPORTED, "findGroundState: useLM is not part of this path");
+  const int L = ops.getLength();
+  if (init.getLength() != L) throw Error(MPSD_ERR_SHAPE, "findGroundState: MPS and MPO lengths differ");
+  if (init.ctx() != ctx_) throw Error(MPSD_ERR_ARG, "findGroundState: MPS lives on another context");
+  stats_ = SolveStats();
+  // Contractor.cpp:1371-1373: right then left gauge
+  init.gaugeCond('R', true);
+  init.gaugeCond('L', true);
+  TmpManager mgr(L, ctx_);
+  double energy = contract(init, ops, init).real();
+  double val = energy;
+  int pos = 0, round = 0;
+  bool right = true, done = false;
+  const double tol = eigtol_ > 0 ? eigtol_ : 1e-10;
+  while (!done) {
+    calculateL(pos, mgr, ops, init, init);
+    calculateR(pos, mgr, ops, init, init);
+    Site& site = init.site(pos);
+    {
+      TensorMultiplierOffset multi(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset,
+                                   precision_);
+      EigStats es = eig_->lowest(multi, (double*)mpsd_tensor_devptr(site.device()), tol);
+      stats_.matvecs += es.matvecs;
+      stats_.final_residual = es.residual;
+      val = es.value - offset;
+    }
+    // energy-increase guard (Contractor.cpp:1417-1434)
+    if ((val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
+        (val - energy > std::fabs(energy + offset) * convtol_)) {
+      char buf[256];
+      snprintf(buf, sizeof buf, "energy increasing in this step: old %.15g new %.15g (round %d, pos %d)", energy,
+               val, round, pos);
+      throw Error(MPSD_ERR_NUMERIC, buf);
+    }
+    init.gaugeCond(pos, right ? 'R' : 'L', true);
+    for (int l = pos + 1; l < L; ++l) mgr.outofdateL(l);
+    for (int l = pos - 1; l >= 0; --l) mgr.outofdateR(l);
+    const bool endRight = right && pos == L - 1;
+    const bool endLeft = !right && pos == 0;
+    if (endRight || endLeft) {
+      done = std::fabs(energy - val) < convtol_ * std::fabs(energy + offset);
+      if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
+      energy = val;
+      stats_.energies.push_back(val);
+      ++round;
+      if (max_halfsweeps_ > 0 && round >= max_halfsweeps_) done = true;
+      if (!done) {
+        if (endRight) {
+          init.gaugeCond(pos, 'L', true);
+          --pos;
+        } else {
+          init.gaugeCond(pos, 'R', true);
+          ++pos;
+        }
+        right = !right;
+        if (!tmpfile.empty() && freqSv > 0 && round % freqSv == 0) init.exportMPS(tmpfile.c_str());
+      }
+    } else {
+      pos += right ? 1 : -1;
+    }
+  }
+  stats_.halfsweeps = round;
+  *lambda = energy;
+}
+
+// ------------------------------------------------------------------------------ models
+static const complex_t I_c(0.0, 1.0);
+
+// W[du, Dl, dd, Dr] = sum_o C[Dl, Dr, o] op_o[du, dd] -- the C*Z reshape and
+// permute(3,1,4,2) of the reference's model builders (e.g. IsingHamiltonian.cpp:110-115).
+static mwArray w_from_c(const std::vector<complex_t>& C, int Dl, int Dr, const std::vector<std::vector<complex_t>>& ops,
+                        int d) {
+  const int no = (int)ops.size();
+  mwArray W(std::vector<int>{d, Dl, d, Dr});
+  for (int l = 0; l < Dl; ++l)
+    for (int r = 0; r < Dr; ++r)
+      for (int o = 0; o < no; ++o) {
+        const complex_t c = C[l + Dl * r + Dl * Dr * o];
+        if (c == complex_t(0, 0)) continue;
+        for (int u = 0; u < d; ++u)
+          for (int dn = 0; dn < d; ++dn) {
+            const complex_t v = ops[o][u + d * dn];  // column-major d x d
+            if (v == complex_t(0, 0)) continue;
+            W.data()[u + d * l + d * Dl * dn + d * Dl * d * r] += c * v;
+          }
+      }
+  return W;
+}
+
+// column-major 2x2 operators
+static const std::vector<complex_t> kI = {1, 0, 0, 1};
+static const std::vector<complex_t> kX = {0, 1, 1, 0};
+static const std::vector<complex_t> kY = {0, I_c, -I_c, 0};
+static const std::vector<complex_t> kZ = {1, 0, 0, -1};
+
+IsingHamiltonian::IsingHamiltonian(int N, int d, double J, double g, double h, double K) : hamil_(N) {
+  if (d != 2) throw Error(MPSD_ERR_ARG, "IsingHamiltonian only defined for d=2");
+  const std::vector<std::vector<complex_t>> ops = {kI, kZ, kX};
+  const complex_t effJ = J < 0 ? std::sqrt(std::fabs(J)) * I_c : complex_t(std::sqrt(J), 0);
+  std::shared_ptr<const Operator> bulk;
+  for (int k = 0; k < N; ++k) {
+    if (k > 1 && k < N - 1 && bulk) {  // IsingHamiltonian.cpp:74-75: bulk sites share one operator
+      hamil_.setOpShared(k, bulk);
+      continue;
+    }
+    const int Dl = k == 0 ? 1 : 3, Dr = k == N - 1 ? 1 : 3;
+    std::vector<complex_t> C(Dl * Dr * 3, 0.0);
+    auto at = [&](int l, int r, int o) -> complex_t& { return C[l + Dl * r + Dl * Dr * o]; };
+    if (k != N - 1) at(0, 0, 0) = 1.0;
+    if (k != 0) at(Dl - 1, Dr - 1, 0) = 1.0;
+    at(0, Dr - 1, 2) = g;
+    at(0, Dr - 1, 1) = h;
+    at(0, Dr - 1, 0) = K / N;
+    if (k != N - 1) at(0, 1, 1) = effJ;
+    if (k != 0) at(1, Dr - 1, 1) = effJ;
+    auto op = std::make_shared<const Operator>(w_from_c(C, Dl, Dr, ops, 2));
+    hamil_.setOpShared(k, op);
+    if (k == 1) bulk = op;
+  }
+}
+
+HeisenbergHamiltonian::HeisenbergHamiltonian(int L, double Jx, double Jy, double Jz, double h, int d, double offset)
+    : hamil_(L) {
+  if (d != 2) throw Error(MPSD_ERR_ARG, "HeisenbergHamiltonian only supports d=2");
+  init(std::vector<double>(std::max(L - 1, 1), Jx), std::vector<double>(std::max(L - 1, 1), Jy),
+       std::vector<double>(std::max(L - 1, 1), Jz), std::vector<double>(L, h), offset);
+}
+
+HeisenbergHamiltonian::HeisenbergHamiltonian(int L, const std::vector<double>& Jx, const std::vector<double>& Jy,
+                                             const std::vector<double>& Jz, const std::vector<double>& h, int d,
+                                             double offset)
+    : hamil_(L) {
+  if (d != 2) throw Error(MPSD_ERR_ARG, "HeisenbergHamiltonian only supports d=2");
+  if ((int)Jx.size() != L - 1 || (int)Jy.size() != L - 1 || (int)Jz.size() != L - 1 || (int)h.size() != L)
+    throw Error(MPSD_ERR_ARG, "HeisenbergHamiltonian: coefficient vectors of the wrong length");
+  init(Jx, Jy, Jz, h, offset);
+}
+
+void HeisenbergHamiltonian::init(const std::vector<double>& Jx, const std::vector<double>& Jy,
+                                 const std::vector<double>& Jz, const std::vector<double>& h, double offset) {
+  // HeisenbergHamiltonian::initZ/initHMPO (HeisenbergHamiltonian.cpp:266-331), no penalty term.
+  const int L = hamil_.getLength();
+  std::vector<complex_t> sx(4), sy(4), sz(4);
+  for (int i = 0; i < 4; ++i) {
+    sx[i] = 0.5 * kX[i];
+    sy[i] = 0.5 * kY[i];
+    sz[i] = 0.5 * kZ[i];
+  }
+  const std::vector<std::vector<complex_t>> ops = {kI, sx, sy, sz};
+  for (int k = 0; k < L; ++k) {
+    const int Dl = k == 0 ? 1 : 5, Dr = k == L - 1 ? 1 : 5;
+    std::vector<complex_t> C(Dl * Dr * 4, 0.0);
+    auto at = [&](int l, int r, int o) -> complex_t& { return C[l + Dl * r + Dl * Dr * o]; };
+    if (k != L - 1) {
+      at(0, 0, 0) = 1.0;
+      at(0, 1, 1) = Jx[k];
+      at(0, 2, 2) = Jy[k];
+      at(0, 3, 3) = Jz[k];
+    }
+    if (k != 0) {
+      at(Dl - 1, Dr - 1, 0) = 1.0;
+      at(1, Dr - 1, 1) = 1.0;
+      at(2, Dr - 1, 2) = 1.0;
+      at(3, Dr - 1, 3) = 1.0;
+    }
+    at(0, Dr - 1, 3) = h[k];
+    if (offset != 0) at(0, Dr - 1, 0) = offset / L;
+    hamil_.setOpShared(k, std::make_shared<const Operator>(w_from_c(C, Dl, Dr, ops, 2)));
+  }
+}
+
+SchwingerHamiltonian::SchwingerHamiltonian(int N, double mu, double x, double l0, double alpha, double offset,
+                                           double nu, double y)
+    : hamil_(N) {
+  // SchwingerHamiltonian::initZ/initHMPO (SchwingerHamiltonian.cpp:23-121)
+  const std::vector<std::vector<complex_t>> ops = {kI, kX, kY, kZ};
+  const double a_ = alpha + l0;
+  std::vector<double> alphas(N), sumA(N, 0.0);
+  for (int k = 0; k < N; ++k) {
+    const double sg = (k + 1) % 2 == 0 ? 1.0 : -1.0;
+    alphas[k] = a_ - 0.25 * (1.0 - sg);
+  }
+  double tot = 0.0;
+  for (int k = N - 2; k >= 0; --k) {
+    tot += alphas[k];
+    sumA[k] = tot;
+  }
+  for (int k = 0; k < N; ++k) {
+    const int Dl = k == 0 ? 1 : 5, Dr = k == N - 1 ? 1 : 5;
+    std::vector<complex_t> C(Dl * Dr * 4, 0.0);
+    auto at = [&](int l, int r, int o) -> complex_t& { return C[l + Dl * r + Dl * Dr * o]; };
+    if (k != N - 1) at(0, 0, 0) = 1.0;
+    if (k != 0) at(4, Dr - 1, 0) = 1.0;
+    if (k > 0 && k < N - 2) at(3, 3, 0) = 1.0;
+    const double ak = alphas[k];
+    const double sg = (k + 1) % 2 == 0 ? 1.0 : -1.0;
+    if (k != N - 1) {
+      at(0, Dr - 1, 0) = y * ak * ak + y * (k + 1) / 4.0 + mu / 2.0 + nu / 2.0 + offset / N;
+      at(0, Dr - 1, 3) = sg * mu / 2.0 + nu / 2.0 + y * sumA[k];
+    } else {
+      at(0, Dr - 1, 0) = mu / 2.0 + nu / 2.0 + offset / N;
+      at(0, Dr - 1, 3) = sg * mu / 2.0 + nu / 2.0;
+    }
+    if (k != N - 1) {
+      at(0, 1, 1) = std::sqrt(x / 2);
+      at(0, 2, 2) = std::sqrt(x / 2);
+      if (k < N - 2) at(0, 3, 3) = y;
+      if (k != 0) {
+        at(1, 4, 1) = std::sqrt(x / 2);
+        at(2, 4, 2) = std::sqrt(x / 2);
+        at(3, 4, 3) = (N - k - 1) / 2.0;
+      }
+    } else {
+      at(1, 0, 1) = std::sqrt(x / 2);
+      at(2, 0, 2) = std::sqrt(x / 2);
+    }
+    hamil_.setOpShared(k, std::make_shared<const Operator>(w_from_c(C, Dl, Dr, ops, 2)));
+  }
+}
+
+}  // namespace mpsdyn

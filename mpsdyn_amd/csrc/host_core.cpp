@@ -1,0 +1,365 @@
+// Host C++ data model of the ground-state path: mwArray, device tensors, Operator/MPO,
+// Site/MPS (site tensors in HBM) and TmpManager (environments in HBM).
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "mpsdyn/mpsdyn.h"
+
+namespace mpsdyn {
+
+void check(mpsd_ctx* ctx, int rc, const char* what) {
+  if (rc != 0) throw Error(rc, std::string(what) + ": " + (ctx ? mpsd_last_error(ctx) : "error"));
+}
+
+mpsd_ctx* default_ctx() {
+  static std::once_flag once;
+  static mpsd_ctx* ctx = nullptr;
+  std::call_once(once, [] {
+    int rc = mpsd_ctx_create(0, &ctx);
+    if (rc) throw Error(rc, "mpsd_ctx_create(0) failed: no HIP device");
+  });
+  return ctx;
+}
+
+// ------------------------------------------------------------------------------ mwArray
+static int64_t prod(const std::vector<int>& d) {
+  int64_t n = 1;
+  for (int x : d) n *= x;
+  return n;
+}
+
+mwArray::mwArray(const std::vector<int>& dims) : dims_(dims), data_(prod(dims)) {}
+
+mwArray::mwArray(const std::vector<int>& dims, const complex_t* data) : dims_(dims), data_(data, data + prod(dims)) {}
+
+mwArray::mwArray(complex_t scalar) : dims_{1}, data_{scalar} {}
+
+static int64_t lin(const std::vector<int>& dims, const std::vector<int>& idx) {
+  int64_t off = 0, stride = 1;
+  for (size_t k = 0; k < dims.size(); ++k) {
+    const int i = k < idx.size() ? idx[k] : 0;
+    if (i < 0 || i >= dims[k]) throw Error(MPSD_ERR_ARG, "mwArray index out of range");
+    off += stride * i;
+    stride *= dims[k];
+  }
+  return off;
+}
+
+complex_t mwArray::getElement(const std::vector<int>& idx) const { return data_[lin(dims_, idx)]; }
+
+void mwArray::setElement(complex_t v, const std::vector<int>& idx) { data_[lin(dims_, idx)] = v; }
+
+void mwArray::reshape(const std::vector<int>& dims) {
+  std::vector<int> nd = dims;
+  int64_t known = 1;
+  int neg = -1;
+  for (size_t k = 0; k < nd.size(); ++k) {
+    if (nd[k] < 0) neg = (int)k;
+    else known *= nd[k];
+  }
+  if (neg >= 0) nd[neg] = (int)(data_.size() / std::max<int64_t>(known, 1));
+  if (prod(nd) != (int64_t)data_.size()) throw Error(MPSD_ERR_SHAPE, "mwArray::reshape: size mismatch");
+  dims_ = nd;
+}
+
+void mwArray::permute(const std::vector<int>& order, bool conj) {
+  const int r = getRank();
+  if ((int)order.size() != r) throw Error(MPSD_ERR_SHAPE, "mwArray::permute: wrong order length");
+  std::vector<int> nd(r);
+  std::vector<int64_t> ostride(r), nstride(r);
+  int64_t s = 1;
+  for (int k = 0; k < r; ++k) {
+    ostride[k] = s;
+    s *= dims_[k];
+  }
+  for (int k = 0; k < r; ++k) nd[k] = dims_[order[k] - 1];
+  std::vector<complex_t> out(data_.size());
+  std::vector<int> idx(r, 0);
+  // iterate over new layout in column-major order
+  for (int64_t t = 0; t < (int64_t)out.size(); ++t) {
+    int64_t src = 0;
+    for (int k = 0; k < r; ++k) src += idx[k] * ostride[order[k] - 1];
+    out[t] = conj ? std::conj(data_[src]) : data_[src];
+    for (int k = 0; k < r; ++k) {
+      if (++idx[k] < nd[k]) break;
+      idx[k] = 0;
+    }
+  }
+  data_.swap(out);
+  dims_ = nd;
+}
+
+void mwArray::conjugate() {
+  for (auto& v : data_) v = std::conj(v);
+}
+
+void mwArray::fillWithZero() { std::fill(data_.begin(), data_.end(), complex_t(0.0, 0.0)); }
+
+void mwArray::save(std::ofstream& out) const {
+  int rank = getRank();
+  out.write((const char*)&rank, sizeof(int));
+  out.write((const char*)dims_.data(), sizeof(int) * rank);
+  out.write((const char*)data_.data(), sizeof(complex_t) * data_.size());
+  if (!out) throw Error(MPSD_ERR_ARG, "mwArray::save: write failed");
+}
+
+void mwArray::load(std::ifstream& in) {
+  int rank = 0;
+  in.read((char*)&rank, sizeof(int));
+  if (!in || rank < 0 || rank > 64) throw Error(MPSD_ERR_ARG, "mwArray::load: bad rank");
+  dims_.assign(rank, 0);
+  in.read((char*)dims_.data(), sizeof(int) * rank);
+  if (!in) throw Error(MPSD_ERR_ARG, "mwArray::load: bad dimensions");
+  data_.assign(prod(dims_), complex_t(0, 0));
+  in.read((char*)data_.data(), sizeof(complex_t) * data_.size());
+  if (!in) throw Error(MPSD_ERR_ARG, "mwArray::load: truncated data");
+}
+
+// ------------------------------------------------------------------------------ DevTensor
+DevTensor::DevTensor(mpsd_ctx* c, const std::vector<int>& dims) : ctx(c) {
+  std::vector<int> d = dims.empty() ? std::vector<int>{1} : dims;
+  check(ctx, mpsd_tensor_create(ctx, (int)d.size(), d.data(), &t), "tensor_create");
+}
+
+DevTensor::DevTensor(mpsd_ctx* c, const mwArray& a) : DevTensor(c, a.getDimensions()) { upload(a); }
+
+DevTensor::~DevTensor() {
+  if (t) mpsd_tensor_destroy(t);
+}
+
+std::vector<int> DevTensor::dims() const {
+  int r = mpsd_tensor_rank(t);
+  std::vector<int> d(std::max(r, 0));
+  if (r > 0) mpsd_tensor_dims(t, d.data());
+  return d;
+}
+
+mwArray DevTensor::download() const {
+  mwArray a(dims());
+  check(ctx, mpsd_tensor_download(t, (double*)a.data(), a.getNrComponents()), "tensor_download");
+  return a;
+}
+
+void DevTensor::upload(const mwArray& a) {
+  if (dims() != a.getDimensions()) {
+    const auto& d = a.getDimensions();
+    check(ctx, mpsd_tensor_reshape(t, (int)d.size(), d.data()), "tensor_reshape");
+  }
+  check(ctx, mpsd_tensor_upload(t, a.getComponents(), a.getNrComponents()), "tensor_upload");
+}
+
+// ------------------------------------------------------------------------------ Operator / MPO
+Operator::Operator(const mwArray& data) : data_(data) {
+  std::vector<int> d = data.getDimensions();
+  if (d.size() > 4) throw Error(MPSD_ERR_SHAPE, "Operator: rank above 4");
+  while (d.size() < 4) d.push_back(1);
+  data_.reshape(d);
+}
+
+const mpsd_tensor* Operator::device(mpsd_ctx* ctx) const {
+  for (auto& p : dev_)
+    if (p.first == ctx) return p.second->t;
+  dev_.emplace_back(ctx, std::make_shared<DevTensor>(ctx, data_));
+  return dev_.back().second->t;
+}
+
+void MPO::setOp(int k, Operator* op, bool myop) {
+  if (myop)
+    ops_[k] = std::shared_ptr<const Operator>(op);
+  else
+    ops_[k] = std::shared_ptr<const Operator>(op, [](const Operator*) {});
+}
+
+std::vector<int> MPO::getDimensions() const {
+  std::vector<int> d;
+  for (auto& o : ops_) d.push_back(o->getDimension(0));
+  return d;
+}
+
+// ------------------------------------------------------------------------------ RNG
+static inline uint64_t splitmix(uint64_t& state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+mwArray complex_gaussian(uint64_t seed, const std::vector<int>& dims) {
+  mwArray a(dims);
+  uint64_t st = seed;
+  const double inv53 = 1.0 / 9007199254740992.0;
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t i = 0; i < a.getNrComponents(); ++i) {
+    const double u1 = ((double)(splitmix(st) >> 11) + 1.0) * inv53;
+    const double u2 = (double)(splitmix(st) >> 11) * inv53;
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    a.data()[i] = complex_t(r * std::cos(two_pi * u2), r * std::sin(two_pi * u2));
+  }
+  return a;
+}
+
+// ------------------------------------------------------------------------------ Site
+Site::Site(mpsd_ctx* ctx, int pos, const std::vector<int>& dims) : ctx_(ctx), pos_(pos), dev_(ctx, dims) {}
+
+std::vector<int> Site::dims() const {
+  std::vector<int> d = dev_.dims();
+  while (d.size() < 3) d.push_back(1);
+  return d;
+}
+
+void Site::setA(const mwArray& a) {
+  mwArray b = a;
+  std::vector<int> d = b.getDimensions();
+  if (d.size() == 2 || (d.size() == 1 && d[0] == dev_.size())) b.reshape(dims());  // vector from the eigensolver
+  dev_.upload(b);
+}
+
+void Site::gaugeR(DevTensor& rightterm) { check(ctx_, mpsd_site_gauge_r(ctx_, dev_.t, rightterm.t), "gaugeR"); }
+
+void Site::gaugeL(DevTensor& leftterm) { check(ctx_, mpsd_site_gauge_l(ctx_, dev_.t, leftterm.t), "gaugeL"); }
+
+void Site::reduceDimR(const DevTensor& leftterm) {
+  check(ctx_, mpsd_site_reduce_dim_r(ctx_, dev_.t, leftterm.t), "reduceDimR");
+}
+
+void Site::reduceDimL(const DevTensor& rightterm) {
+  check(ctx_, mpsd_site_reduce_dim_l(ctx_, dev_.t, rightterm.t), "reduceDimL");
+}
+
+// ------------------------------------------------------------------------------ MPS
+MPS::MPS(int length, int bond, int d, mpsd_ctx* ctx) : ctx_(ctx ? ctx : default_ctx()), bond_(bond) {
+  for (int k = 0; k < length; ++k) {
+    const int Dl = (k == 0) ? 1 : bond;
+    const int Dr = (k == length - 1) ? 1 : bond;
+    A_.emplace_back(new Site(ctx_, k + 1, {d, Dl, Dr}));
+  }
+}
+
+void MPS::setRandomState(uint64_t seed) {
+  const int L = getLength();
+  std::vector<int> bd(L + 1);
+  for (int k = 0; k <= L; ++k) {
+    double cap = (double)bond_;
+    const int d = A_[std::min(k, L - 1)]->getd();
+    cap = std::min(cap, std::pow((double)d, std::min(k, 60)));
+    cap = std::min(cap, std::pow((double)d, std::min(L - k, 60)));
+    bd[k] = (int)cap;
+  }
+  for (int k = 0; k < L; ++k) {
+    const int d = A_[k]->getd();
+    A_[k]->setA(complex_gaussian(seed * 1000003ull + (uint64_t)k, {d, bd[k], bd[k + 1]}));
+  }
+  gaugeCond('R', true);
+}
+
+DevTensor& MPS::factor() {
+  if (!factor_) factor_.reset(new DevTensor(ctx_, std::vector<int>{1, 1}));
+  return *factor_;
+}
+
+void MPS::gaugeCond(char dir, bool normal) {
+  DevTensor& tmp = factor();
+  const int L = getLength();
+  if (dir == 'L' || dir == 'l') {
+    for (int pos = L - 1; pos >= 0; --pos) {
+      A_[pos]->gaugeL(tmp);
+      if (pos > 0) A_[pos - 1]->reduceDimL(tmp);
+    }
+    gauge_ = gLeft;
+  } else if (dir == 'R' || dir == 'r') {
+    for (int pos = 0; pos < L; ++pos) {
+      A_[pos]->gaugeR(tmp);
+      if (pos < L - 1) A_[pos + 1]->reduceDimR(tmp);
+    }
+    gauge_ = gRight;
+  } else {
+    throw Error(MPSD_ERR_ARG, "MPS::gaugeCond: unknown direction");
+  }
+  int maxb = 1;
+  for (auto& s : A_) maxb = std::max(maxb, std::max(s->getDl(), s->getDr()));
+  bond_ = std::min(bond_, maxb);
+  if (normal) normfact_ = 1.0;
+}
+
+void MPS::gaugeCond(int pos, char dir, bool next) {
+  DevTensor& tmp = factor();
+  const int L = getLength();
+  if (dir == 'L' || dir == 'l') {
+    const int oldDl = A_[pos]->getDl();
+    A_[pos]->gaugeL(tmp);
+    if ((A_[pos]->getDl() != oldDl || next) && pos > 0) A_[pos - 1]->reduceDimL(tmp);
+  } else {
+    const int oldDr = A_[pos]->getDr();
+    A_[pos]->gaugeR(tmp);
+    if (pos < L - 1 && (A_[pos]->getDr() != oldDr || next)) A_[pos + 1]->reduceDimR(tmp);
+  }
+  gauge_ = gNone;
+}
+
+void MPS::exportMPS(const char* filename) const {
+  std::ofstream out(filename, std::ios::binary);
+  if (!out.is_open()) throw Error(MPSD_ERR_ARG, std::string("exportMPS: cannot open ") + filename);
+  int length = getLength();
+  int g = (int)gauge_;
+  out.write((const char*)&length, sizeof(int));
+  out.write((const char*)&bond_, sizeof(int));
+  out.write((const char*)&normfact_, sizeof(double));
+  out.write((const char*)&g, sizeof(int));
+  for (auto& s : A_) {
+    int pos = s->getPos();
+    out.write((const char*)&pos, sizeof(int));
+    s->getA().save(out);
+  }
+  if (!out) throw Error(MPSD_ERR_ARG, "exportMPS: write failed");
+}
+
+void MPS::importMPS(const char* filename) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in.is_open()) throw Error(MPSD_ERR_ARG, std::string("importMPS: cannot open ") + filename);
+  if (!ctx_) ctx_ = default_ctx();
+  int length = 0, g = 0;
+  in.read((char*)&length, sizeof(int));
+  in.read((char*)&bond_, sizeof(int));
+  in.read((char*)&normfact_, sizeof(double));
+  in.read((char*)&g, sizeof(int));
+  if (!in || length < 0) throw Error(MPSD_ERR_ARG, "importMPS: bad header");
+  gauge_ = (Gauge)g;
+  A_.clear();
+  for (int k = 0; k < length; ++k) {
+    int pos = 0;
+    in.read((char*)&pos, sizeof(int));
+    mwArray a;
+    a.load(in);
+    std::vector<int> d = a.getDimensions();
+    while (d.size() < 3) d.push_back(1);
+    a.reshape(d);
+    A_.emplace_back(new Site(ctx_, pos, d));
+    A_.back()->setA(a);
+  }
+}
+
+// ------------------------------------------------------------------------------ TmpManager
+TmpManager::TmpManager(int length, mpsd_ctx* ctx) : upL_(length, false), upR_(length, false) {
+  const complex_t one(1.0, 0.0);
+  for (int k = 0; k < length; ++k) {
+    L_.emplace_back(ctx, std::vector<int>{1, 1, 1});
+    R_.emplace_back(ctx, std::vector<int>{1, 1, 1});
+  }
+  mwArray edge(std::vector<int>{1, 1, 1}, &one);
+  L_[0].upload(edge);
+  R_[length - 1].upload(edge);
+  upL_[0] = true;
+  upR_[length - 1] = true;
+}
+
+int64_t TmpManager::deviceBytes() const {
+  int64_t b = 0;
+  for (auto& t : L_) b += 16 * t.size();
+  for (auto& t : R_) b += 16 * t.size();
+  return b;
+}
+
+}  // namespace mpsdyn

@@ -1,0 +1,226 @@
+// Device matvec multiplier and the FP64 Lanczos eigensolver of the sweep.
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "mpsdyn/mpsdyn.h"
+
+namespace mpsdyn {
+
+// ------------------------------------------------------------------------------ Multiplier
+void Multiplier::product(const mwArray& x, mwArray& y) {
+  mpsd_ctx* c = ctx();
+  const int64_t n = getSize();
+  if (x.getNrComponents() != n) throw Error(MPSD_ERR_SHAPE, "Multiplier::product: wrong input size");
+  DevTensor dx(c, x), dy(c, std::vector<int>{(int)n});
+  product_device(dx.ptr(), dy.ptr());
+  y = dy.download();
+  y.reshape({(int)n, 1});
+}
+
+TensorMultiplierOffset::TensorMultiplierOffset(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W,
+                                               const mpsd_tensor* R, double offset, int precision)
+    : ctx_(ctx) {
+  check(ctx_, mpsd_heff_create(ctx_, L, W, R, offset, precision, &h_), "heff_create");
+}
+
+TensorMultiplierOffset::~TensorMultiplierOffset() {
+  if (h_) mpsd_heff_destroy(h_);
+}
+
+int64_t TensorMultiplierOffset::getSize() const { return mpsd_heff_size(h_); }
+
+void TensorMultiplierOffset::product_device(const double* x, double* y) {
+  const int64_t n = getSize();
+  check(ctx_, mpsd_heff_apply(h_, x, y, 1, n, n, 1), "heff_apply");
+}
+
+void TensorMultiplierOffset::getFullTensor(mwArray& result) {
+  const int n = (int)getSize();
+  result = mwArray(std::vector<int>{n, n});
+  check(ctx_, mpsd_heff_full(h_, (double*)result.data()), "heff_full");
+}
+
+void TensorMultiplierOffset::setPrecision(int p) { check(ctx_, mpsd_heff_set_precision(h_, p), "set_precision"); }
+
+// ------------------------------------------------------------------------------ tridiagonal
+// Number of eigenvalues of T below x (Sturm count via the LDL^T pivots).
+static int sturm_count(const std::vector<double>& a, const std::vector<double>& b, double x) {
+  const double tiny = std::numeric_limits<double>::min() * 1e10;
+  int cnt = 0;
+  double q = a[0] - x;
+  if (q < 0) ++cnt;
+  for (size_t i = 1; i < a.size(); ++i) {
+    if (std::fabs(q) < tiny) q = q < 0 ? -tiny : tiny;
+    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+// Solve (T - sigma I) y = r by Gaussian elimination with partial pivoting (tridiagonal).
+static void tridiag_solve(const std::vector<double>& a, const std::vector<double>& b, double sigma,
+                          std::vector<double>& r) {
+  const int m = (int)a.size();
+  if (m == 1) {
+    double p = a[0] - sigma;
+    if (std::fabs(p) < 1e-300) p = 1e-300;
+    r[0] /= p;
+    return;
+  }
+  // rows: lower l[i] (i>=1), diag d[i], upper u[i], second upper v[i] (from pivoting)
+  std::vector<double> d(m), u(m, 0.0), v(m, 0.0), l(m, 0.0);
+  for (int i = 0; i < m; ++i) d[i] = a[i] - sigma;
+  for (int i = 0; i + 1 < m; ++i) {
+    u[i] = b[i];
+    l[i + 1] = b[i];
+  }
+  std::vector<double> rr = r;
+  for (int i = 0; i + 1 < m; ++i) {
+    if (std::fabs(l[i + 1]) > std::fabs(d[i])) {  // swap rows i and i+1
+      std::swap(d[i], l[i + 1]);
+      std::swap(u[i], d[i + 1]);
+      std::swap(v[i], u[i + 1]);
+      std::swap(rr[i], rr[i + 1]);
+    }
+    double piv = d[i];
+    if (std::fabs(piv) < 1e-300) piv = d[i] = 1e-300;
+    const double f = l[i + 1] / piv;
+    d[i + 1] -= f * u[i];
+    u[i + 1] -= f * v[i];
+    rr[i + 1] -= f * rr[i];
+  }
+  if (std::fabs(d[m - 1]) < 1e-300) d[m - 1] = 1e-300;
+  rr[m - 1] /= d[m - 1];
+  rr[m - 2] = (rr[m - 2] - u[m - 2] * rr[m - 1]) / d[m - 2];
+  for (int i = m - 3; i >= 0; --i) rr[i] = (rr[i] - u[i] * rr[i + 1] - v[i] * rr[i + 2]) / d[i];
+  r = rr;
+}
+
+double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b, std::vector<double>& s) {
+  const int m = (int)a.size();
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < m; ++i) {
+    const double rad = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - rad);
+    hi = std::max(hi, a[i] + rad);
+  }
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (hi - lo <= 2 * eps * std::max(std::fabs(lo), std::fabs(hi)) || mid == lo || mid == hi) break;
+    if (sturm_count(a, b, mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double theta = 0.5 * (lo + hi);
+  s.assign(m, 1.0);
+  const double shift = theta - 4 * eps * std::max(1.0, std::fabs(theta));
+  for (int it = 0; it < 3; ++it) {
+    tridiag_solve(a, b, shift, s);
+    double nrm = 0;
+    for (double v : s) nrm += v * v;
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0) || !std::isfinite(nrm)) {
+      s.assign(m, 0.0);
+      s[0] = 1.0;
+      break;
+    }
+    for (double& v : s) v /= nrm;
+  }
+  return theta;
+}
+
+// ------------------------------------------------------------------------------ Lanczos
+LanczosSolver::~LanczosSolver() {}
+
+EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_krylov, int max_restarts) {
+  EigStats st;
+  const int64_t n = H.getSize();
+  const int m = (int)std::max<int64_t>(2, std::min<int64_t>(max_krylov, n));
+  if ((m + 1) * n > cap_) {
+    V_ = DevTensor(ctx_, std::vector<int>{(int)n, m + 1});
+    cap_ = (int64_t)(m + 1) * n;
+  }
+  double* V = V_.ptr();
+  const int64_t ldv = n;
+  auto vec = [&](int j) { return V + 2 * (int64_t)j * ldv; };
+  const double zero[2] = {0.0, 0.0}, one[2] = {1.0, 0.0};
+
+  // start vector
+  double nx = 0.0;
+  check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+  if (!(nx > 0.0) || !std::isfinite(nx)) {
+    DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));
+    check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
+    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+  }
+  std::vector<double> alpha, beta, s, h;
+  for (int restart = 0; restart <= max_restarts; ++restart) {
+    st.restarts = restart;
+    const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
+    check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, vec(0), n), "normalise");
+    alpha.clear();
+    beta.clear();
+    double theta = 0.0, res = 0.0;
+    int j = 0;
+    bool stop = false;
+    for (; j < m; ++j) {
+      double* w = vec(j + 1);
+      H.product_device(vec(j), w);
+      ++st.matvecs;
+      // first Gram-Schmidt pass: h = V^H w, w -= V h
+      h.assign(2 * (j + 1), 0.0);
+      check(ctx_, mpsd_vec_dots(ctx_, V, ldv, j + 1, w, n, h.data()), "dots");
+      double a = h[2 * j];
+      for (double& v : h) v = -v;
+      check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, j + 1, h.data(), w, n), "gemv");
+      // second pass (DGKS) fused with the norm of w: dots over [V_0..V_j, w]
+      std::vector<double> h2(2 * (j + 2), 0.0);
+      check(ctx_, mpsd_vec_dots(ctx_, V, ldv, j + 2, w, n, h2.data()), "dots2");
+      double nw2 = h2[2 * (j + 1)];
+      double corr2 = 0.0;
+      for (int i = 0; i <= j; ++i) corr2 += h2[2 * i] * h2[2 * i] + h2[2 * i + 1] * h2[2 * i + 1];
+      a += h2[2 * j];
+      h2.resize(2 * (j + 1));
+      for (double& v : h2) v = -v;
+      check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, j + 1, h2.data(), w, n), "gemv2");
+      const double b = std::sqrt(std::max(nw2 - corr2, 0.0));
+      alpha.push_back(a);
+      theta = tridiag_lowest(alpha, beta, s);
+      res = b * std::fabs(s[j]);
+      const double scale = std::max(1.0, std::fabs(theta));
+      if (!std::isfinite(theta) || !std::isfinite(b))
+        throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
+      if (res <= tol * scale || b <= 1e-14 * scale || j + 1 >= n) {
+        stop = res <= tol * scale || b <= 1e-14 * scale || j + 1 >= n;
+        ++j;
+        break;
+      }
+      beta.push_back(b);
+      const double ib[2] = {1.0 / b, 0.0};
+      check(ctx_, mpsd_vec_axpby(ctx_, ib, w, zero, w, n), "scale");
+    }
+    // Ritz vector x = V s
+    const int k = (int)alpha.size();
+    std::vector<double> c(2 * k, 0.0);
+    for (int i = 0; i < k; ++i) c[2 * i] = s[i];
+    check(ctx_, mpsd_vec_zero(ctx_, x, n), "zero");
+    check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, k, c.data(), x, n), "ritz");
+    st.value = theta;
+    st.residual = res;
+    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+    if (stop) {
+      st.converged = true;
+      break;
+    }
+  }
+  // unit norm on exit
+  const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
+  check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, x, n), "normalise");
+  return st;
+}
+
+}  // namespace mpsdyn

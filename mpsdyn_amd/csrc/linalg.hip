@@ -1,0 +1,412 @@
+// Device vector primitives for the FP64 eigensolver and the gauge steps of the sweep.
+//
+// Reductions use a fixed grid and a fixed summation order that depend only on the vector
+// length, so a result is bit-identical whatever else runs on the device (a solve gives the
+// same numbers alone and inside a mix of concurrent solves).
+//
+// The QR/LQ gauge moves keep the MPS in HBM: rocSOLVER's Householder QR/LQ runs on the
+// site buffer in place and the triangular factor is multiplied into the neighbour with the
+// BRGEMM kernel.
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mpsd_internal.h"
+
+namespace mpsd {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kDotBlocks = 128;  // partial sums per vector (fixed => fixed order)
+
+__device__ __forceinline__ double2 cmul_conj(double2 a, double2 b) {  // conj(a) * b
+  return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+
+// partial[j * kDotBlocks + blk] = sum over this block's fixed slice of conj(V_j) * w
+__global__ void __launch_bounds__(kT) dots_partial_kernel(const double2* __restrict__ V, int64_t ldv,
+                                                          const double2* __restrict__ w, int64_t n,
+                                                          double2* __restrict__ partial) {
+  const int j = blockIdx.y;
+  const int blk = blockIdx.x;
+  const double2* v = V + (int64_t)j * ldv;
+  const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
+  const int64_t lo = blk * chunk;
+  const int64_t hi = min(n, lo + chunk);
+  double2 acc{0.0, 0.0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kT) {
+    const double2 p = cmul_conj(v[i], w[i]);
+    acc.x += p.x;
+    acc.y += p.y;
+  }
+  __shared__ double2 red[kT];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x].x += red[threadIdx.x + s].x;
+      red[threadIdx.x].y += red[threadIdx.x + s].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)j * kDotBlocks + blk] = red[0];
+}
+
+__global__ void __launch_bounds__(kDotBlocks) dots_final_kernel(const double2* __restrict__ partial,
+                                                               double2* __restrict__ out) {
+  const int j = blockIdx.x;
+  __shared__ double2 red[kDotBlocks];
+  red[threadIdx.x] = partial[(int64_t)j * kDotBlocks + threadIdx.x];
+  __syncthreads();
+  for (int s = kDotBlocks / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x].x += red[threadIdx.x + s].x;
+      red[threadIdx.x].y += red[threadIdx.x + s].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// w[i] += sum_j V_j[i] * c[j]   (c on the device, m <= 1024)
+__global__ void __launch_bounds__(kT) gemv_update_kernel(const double2* __restrict__ V, int64_t ldv, int m,
+                                                         const double2* __restrict__ c,
+                                                         double2* __restrict__ w, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = w[i];
+  for (int j = 0; j < m; ++j) {
+    const double2 v = V[(int64_t)j * ldv + i];
+    const double2 cj = c[j];
+    acc.x += v.x * cj.x - v.y * cj.y;
+    acc.y += v.x * cj.y + v.y * cj.x;
+  }
+  w[i] = acc;
+}
+
+// y = a x + b y
+__global__ void __launch_bounds__(kT) axpby_kernel(double2 a, const double2* __restrict__ x, double2 b,
+                                                   double2* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double2 xv = x[i], yv = y[i];
+  y[i] = double2{a.x * xv.x - a.y * xv.y + b.x * yv.x - b.y * yv.y,
+                 a.x * xv.y + a.y * xv.x + b.x * yv.y + b.y * yv.x};
+}
+
+__global__ void __launch_bounds__(kT) fill_kernel(double2* __restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) x[i] = double2{0.0, 0.0};
+}
+
+// Upper-trapezoidal R (k x n) out of a geqrf result (lda = m), zeros below the diagonal.
+__global__ void __launch_bounds__(kT) extract_r_kernel(const double2* __restrict__ A, int m, int k, int n,
+                                                       double2* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)k * n) return;
+  const int i = (int)(t % k), j = (int)(t / k);
+  R[t] = (i <= j) ? A[i + (int64_t)m * j] : double2{0.0, 0.0};
+}
+
+// Lower-trapezoidal L (m x k) out of a gelqf result (lda = m), zeros above the diagonal.
+__global__ void __launch_bounds__(kT) extract_l_kernel(const double2* __restrict__ A, int m, int k,
+                                                       double2* __restrict__ L) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)m * k) return;
+  const int i = (int)(t % m), j = (int)(t / m);
+  L[t] = (i >= j) ? A[i + (int64_t)m * j] : double2{0.0, 0.0};
+}
+
+// out[a, s, c] = in[s, a, c] for in = [d, Dl, Dr]; ldo = leading dim (rows) of out.
+__global__ void __launch_bounds__(kT) permute_sad_kernel(const double2* __restrict__ in, int d, int Dl, int Dr,
+                                                         double2* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t total = (int64_t)d * Dl * Dr;
+  if (t >= total) return;
+  const int s = (int)(t % d);
+  const int a = (int)((t / d) % Dl);
+  const int64_t c = t / ((int64_t)d * Dl);
+  out[a + (int64_t)Dl * s + (int64_t)Dl * d * c] = in[t];
+}
+
+// site[s, i, c] = Q[i, s + d*c] with Q stored with leading dimension ldq (k rows used).
+__global__ void __launch_bounds__(kT) unpermute_q_kernel(const double2* __restrict__ Q, int ldq, int k, int d,
+                                                         int Dr, double2* __restrict__ site) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t total = (int64_t)d * k * Dr;
+  if (t >= total) return;
+  const int s = (int)(t % d);
+  const int i = (int)((t / d) % k);
+  const int64_t c = t / ((int64_t)d * k);
+  site[t] = Q[i + (int64_t)ldq * (s + (int64_t)d * c)];
+}
+
+// Edge phase of Site::gaugeR/gaugeL (Site.cpp:202-207, 232-237): the 1x1 factor f becomes
+// |f| and its phase f/|f| is multiplied into the site tensor.
+__global__ void __launch_bounds__(kT) edge_phase_kernel(double2* __restrict__ site, int64_t n,
+                                                        double2* __restrict__ f) {
+  const double2 v = *f;
+  const double a = sqrt(v.x * v.x + v.y * v.y);
+  const double2 ph = a > 0.0 ? double2{v.x / a, v.y / a} : double2{1.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double2 x = site[i];
+    site[i] = double2{x.x * ph.x - x.y * ph.y, x.x * ph.y + x.y * ph.x};
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *f = double2{a, 0.0};
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+}  // namespace
+
+// Per-context rocBLAS handle bound to the context's stream (created on first use).
+static int handle_of(mpsd_ctx* ctx, rocblas_handle* h) {
+  if (!ctx->blas) {
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success)
+      return set_err(ctx, MPSD_ERR_HIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(b, ctx->stream) != rocblas_status_success) {
+      rocblas_destroy_handle(b);
+      return set_err(ctx, MPSD_ERR_HIP, "rocblas_set_stream failed");
+    }
+    ctx->blas = b;
+  }
+  *h = (rocblas_handle)ctx->blas;
+  return MPSD_OK;
+}
+
+void destroy_blas(mpsd_ctx* ctx) {
+  if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
+  ctx->blas = nullptr;
+}
+
+static int solver_err(mpsd_ctx* ctx, rocblas_status st, const char* what) {
+  if (st == rocblas_status_success) return MPSD_OK;
+  return set_err(ctx, MPSD_ERR_HIP, std::string(what) + ": " + rocblas_status_to_string(st));
+}
+
+}  // namespace mpsd
+
+using namespace mpsd;
+
+extern "C" {
+
+int mpsd_vec_dots(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* w, int64_t n,
+                  double* out_host) {
+  if (!ctx || !V || !w || !out_host || m < 0 || n < 0 || (m > 1 && ldv < n)) return MPSD_ERR_ARG;
+  if (m == 0) return MPSD_OK;
+  const size_t need = sizeof(double2) * ((size_t)m * kDotBlocks + m);
+  if (ctx->dots_bytes < need) {
+    if (ctx->dots) (void)hipFree(ctx->dots);
+    ctx->dots = nullptr;
+    ctx->dots_bytes = 0;
+    hipError_t e = hipMalloc(&ctx->dots, need);
+    if (e != hipSuccess) return hip_err(ctx, e, "dots alloc");
+    ctx->dots_bytes = need;
+  }
+  double2* partial = (double2*)ctx->dots;
+  double2* out = partial + (size_t)m * kDotBlocks;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks, m), dim3(kT), 0, ctx->stream,
+                     (const double2*)V, ldv, (const double2*)w, n, partial);
+  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, out);
+  hipError_t e = hipMemcpyAsync(out_host, out, sizeof(double2) * m, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return hip_err(ctx, e, "vec_dots");
+}
+
+int mpsd_vec_gemv(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_host, double* w,
+                  int64_t n) {
+  if (!ctx || !V || !w || !c_host || m < 0 || m > 1024 || n < 0) return MPSD_ERR_ARG;
+  if (m == 0 || n == 0) return MPSD_OK;
+  // Coefficients go through a pinned host ring and a device ring of 8 slots; a slot is
+  // reused only after the event of its previous copy has completed.
+  if (!ctx->coef) {
+    hipError_t e = hipMalloc(&ctx->coef, sizeof(double2) * 1024 * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->coef_host, sizeof(double2) * 1024 * 8, hipHostMallocDefault);
+    for (int i = 0; e == hipSuccess && i < 8; ++i) e = hipEventCreateWithFlags(&ctx->coef_ev[i], hipEventDisableTiming);
+    if (e != hipSuccess) return hip_err(ctx, e, "coef alloc");
+  }
+  const unsigned slot = ctx->coef_slot++ & 7;
+  hipError_t e = hipEventSynchronize(ctx->coef_ev[slot]);
+  if (e != hipSuccess) return hip_err(ctx, e, "coef slot wait");
+  double2* ch = (double2*)ctx->coef_host + 1024 * slot;
+  double2* c = (double2*)ctx->coef + 1024 * slot;
+  memcpy(ch, c_host, sizeof(double2) * m);
+  e = hipMemcpyAsync(c, ch, sizeof(double2) * m, hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "vec_gemv H2D");
+  hipLaunchKernelGGL(gemv_update_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)V, ldv, m,
+                     c, (double2*)w, n);
+  e = hipEventRecord(ctx->coef_ev[slot], ctx->stream);
+  return hip_err(ctx, e, "vec_gemv");
+}
+
+int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n) {
+  if (!ctx || !a || !x || !b || !y || n < 0) return MPSD_ERR_ARG;
+  if (n == 0) return MPSD_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, double2{a[0], a[1]},
+                     (const double2*)x, double2{b[0], b[1]}, (double2*)y, n);
+  return hip_err(ctx, hipGetLastError(), "vec_axpby");
+}
+
+int mpsd_vec_copy(mpsd_ctx* ctx, const double* x, double* y, int64_t n) {
+  if (!ctx || !x || !y || n < 0) return MPSD_ERR_ARG;
+  if (n == 0 || x == y) return MPSD_OK;
+  return hip_err(ctx, hipMemcpyAsync(y, x, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream), "vec_copy");
+}
+
+int mpsd_vec_zero(mpsd_ctx* ctx, double* x, int64_t n) {
+  if (!ctx || !x || n < 0) return MPSD_ERR_ARG;
+  if (n == 0) return MPSD_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (double2*)x, n);
+  return hip_err(ctx, hipGetLastError(), "vec_zero");
+}
+
+int mpsd_vec_norm2(mpsd_ctx* ctx, const double* x, int64_t n, double* out) {
+  if (!out) return MPSD_ERR_ARG;
+  double h[2] = {0.0, 0.0};
+  int rc = mpsd_vec_dots(ctx, x, n, 1, x, n, h);
+  *out = h[0];
+  return rc;
+}
+
+int mpsd_site_gauge_r(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out) {
+  if (!ctx || !site || !R_out || site == R_out) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = d * Dl, n = Dr, k = m < n ? m : n;
+  rocblas_handle h;
+  int rc = handle_of(ctx, &h);
+  if (rc) return rc;
+  rc = ensure_scratch(ctx, 2, sizeof(double2) * (size_t)k);
+  if (rc) return rc;
+  rocblas_double_complex* A = (rocblas_double_complex*)site->ptr;
+  rocblas_double_complex* tau = (rocblas_double_complex*)ctx->scratch[2];
+  rc = solver_err(ctx, rocsolver_zgeqrf(h, m, n, A, m, tau), "zgeqrf");
+  if (rc) return rc;
+  const int rd[2] = {k, n};
+  rc = mpsd_tensor_reshape(R_out, 2, rd);
+  if (rc) return rc;
+  hipLaunchKernelGGL(extract_r_kernel, dim3(nblk((int64_t)k * n)), dim3(kT), 0, ctx->stream, site->ptr, m, k, n,
+                     R_out->ptr);
+  rc = solver_err(ctx, rocsolver_zungqr(h, m, k, k, A, m, tau), "zungqr");
+  if (rc) return rc;
+  const int sd[3] = {d, Dl, k};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  if (k == 1 && n == 1)
+    hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, R_out->ptr);
+  ++site->version;
+  ++R_out->version;
+  return hip_err(ctx, hipGetLastError(), "gauge_r");
+}
+
+int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out) {
+  if (!ctx || !site || !L_out || site == L_out) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = Dl, n = d * Dr, k = m < n ? m : n;
+  rocblas_handle h;
+  int rc = handle_of(ctx, &h);
+  if (rc) return rc;
+  rc = ensure_scratch(ctx, 2, sizeof(double2) * (size_t)k);
+  if (!rc) rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * n);
+  if (rc) return rc;
+  double2* T = (double2*)ctx->scratch[3];
+  hipLaunchKernelGGL(permute_sad_kernel, dim3(nblk((int64_t)m * n)), dim3(kT), 0, ctx->stream, site->ptr, d, Dl,
+                     Dr, T);
+  rocblas_double_complex* tau = (rocblas_double_complex*)ctx->scratch[2];
+  rc = solver_err(ctx, rocsolver_zgelqf(h, m, n, (rocblas_double_complex*)T, m, tau), "zgelqf");
+  if (rc) return rc;
+  const int ld[2] = {m, k};
+  rc = mpsd_tensor_reshape(L_out, 2, ld);
+  if (rc) return rc;
+  hipLaunchKernelGGL(extract_l_kernel, dim3(nblk((int64_t)m * k)), dim3(kT), 0, ctx->stream, T, m, k, L_out->ptr);
+  rc = solver_err(ctx, rocsolver_zunglq(h, k, n, k, (rocblas_double_complex*)T, m, tau), "zunglq");
+  if (rc) return rc;
+  const int sd[3] = {d, k, Dr};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  hipLaunchKernelGGL(unpermute_q_kernel, dim3(nblk((int64_t)d * k * Dr)), dim3(kT), 0, ctx->stream, T, m, k, d,
+                     Dr, site->ptr);
+  if (k == 1 && m == 1)
+    hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, L_out->ptr);
+  ++site->version;
+  ++L_out->version;
+  return hip_err(ctx, hipGetLastError(), "gauge_l");
+}
+
+int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R) {
+  // site[s, i, c] = sum_j R[i, j] site_old[s, j, c]   (Site::reduceDimR, Site.cpp:240-261)
+  if (!ctx || !site || !R || site == R) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int Rm = R->dims[0], Rn = R->rank > 1 ? R->dims[1] : 1;
+  if (Rn != Dl) return set_err(ctx, MPSD_ERR_SHAPE, "reduce_dim_r: factor does not match the left bond");
+  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)site->n);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(ctx->scratch[3], site->ptr, sizeof(double2) * site->n, hipMemcpyDeviceToDevice,
+                                ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "reduce_dim_r copy");
+  const int sd[3] = {d, Rm, Dr};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  ZgemmDesc g;
+  g.A = R->ptr;
+  g.B = (const double2*)ctx->scratch[3];
+  g.C = site->ptr;
+  g.M = Rm;
+  g.N = Dr;
+  g.K = Dl;
+  g.NB = d;
+  g.sAm = 1;
+  g.sAk = Rm;
+  g.sAb = 0;
+  g.sBk = d;
+  g.sBn = (int64_t)d * Dl;
+  g.sBb = 1;
+  g.sCm = d;
+  g.sCn = (int64_t)d * Rm;
+  g.sCb = 1;
+  g.a_mfast = 1;
+  g.b_kfast = 1;
+  ++site->version;
+  return hip_err(ctx, launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream), "reduce_dim_r");
+}
+
+int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L) {
+  // site[s, a, j] = sum_c site_old[s, a, c] L[c, j]   (Site::reduceDimL, Site.cpp:263-283)
+  if (!ctx || !site || !L || site == L) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int Lm = L->dims[0], Ln = L->rank > 1 ? L->dims[1] : 1;
+  if (Lm != Dr) return set_err(ctx, MPSD_ERR_SHAPE, "reduce_dim_l: factor does not match the right bond");
+  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)site->n);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(ctx->scratch[3], site->ptr, sizeof(double2) * site->n, hipMemcpyDeviceToDevice,
+                                ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "reduce_dim_l copy");
+  const int sd[3] = {d, Dl, Ln};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  ZgemmDesc g;
+  g.A = (const double2*)ctx->scratch[3];
+  g.B = L->ptr;
+  g.C = site->ptr;
+  g.M = d * Dl;
+  g.N = Ln;
+  g.K = Dr;
+  g.NB = 1;
+  g.sAm = 1;
+  g.sAk = (int64_t)d * Dl;
+  g.sBk = 1;
+  g.sBn = Lm;
+  g.sCm = 1;
+  g.sCn = (int64_t)d * Dl;
+  g.a_mfast = 1;
+  g.b_kfast = 1;
+  ++site->version;
+  return hip_err(ctx, launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream), "reduce_dim_l");
+}
+
+}  // extern "C"

@@ -54,9 +54,19 @@ struct mpsd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  // Scratch for the two intermediates of a contraction (grown on demand, never shrunk).
-  void* scratch[2] = {nullptr, nullptr};
-  size_t scratch_bytes[2] = {0, 0};
+  // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
+  // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step.
+  static constexpr int kScratch = 4;
+  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_bytes[kScratch] = {0, 0, 0, 0};
+  // Eigensolver support: rocBLAS handle (for rocSOLVER), reduction partials, coefficient ring.
+  void* blas = nullptr;
+  void* dots = nullptr;
+  size_t dots_bytes = 0;
+  void* coef = nullptr;       // device ring, 8 slots x 1024 complex
+  void* coef_host = nullptr;  // pinned host ring, same layout
+  hipEvent_t coef_ev[8] = {nullptr};
+  unsigned coef_slot = 0;
   // Staging for host-buffer matvecs.
   void* stage[2] = {nullptr, nullptr};
   size_t stage_bytes[2] = {0, 0};
@@ -106,4 +116,5 @@ int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes);
 // Run `launch` on the context's stream, bracketed by HIP events when timing is on.
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch);
 int flush_timing(mpsd_ctx* ctx);
+void destroy_blas(mpsd_ctx* ctx);
 }  // namespace mpsd

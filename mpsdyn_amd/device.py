@@ -80,6 +80,20 @@ def _optional(lib):
     """Bind entry points that later parts of the library add (absent ones are skipped)."""
     P, I, I64, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
     extra = {
+        "mpsd_vec_dots": (I, [P, P, I64, I, P, I64, P]),
+        "mpsd_vec_gemv": (I, [P, P, I64, I, P, P, I64]),
+        "mpsd_vec_axpby": (I, [P, P, P, P, P, I64]),
+        "mpsd_vec_copy": (I, [P, P, P, I64]),
+        "mpsd_vec_zero": (I, [P, P, I64]),
+        "mpsd_vec_norm2": (I, [P, P, I64, C.POINTER(D)]),
+        "mpsd_site_gauge_r": (I, [P, P, P]),
+        "mpsd_site_gauge_l": (I, [P, P, P]),
+        "mpsd_site_reduce_dim_r": (I, [P, P, P]),
+        "mpsd_site_reduce_dim_l": (I, [P, P, P]),
+        "mpsd_gs_solve": (I, [P, C.POINTER(GsConfig), C.POINTER(GsResult)]),
+        "mpsd_gs_solve_export": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(GsResult)]),
+        "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
+        "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
         "mpsd_ctx_set_timing": (I, [P, I]),
         "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
         "mpsd_ctx_set_tile": (I, [P, I, I]),
@@ -89,6 +103,41 @@ def _optional(lib):
             f = getattr(lib, name)
             f.restype = res
             f.argtypes = args
+
+
+MODEL_ISING, MODEL_HEISENBERG, MODEL_SCHWINGER = 0, 1, 2
+
+
+class GsConfig(C.Structure):
+    """mpsd_gs_config (include/mpsd_gs.h)."""
+    _fields_ = [("model", C.c_int), ("L", C.c_int), ("D", C.c_int), ("p", C.c_double * 4),
+                ("offset", C.c_double), ("convtol", C.c_double), ("eigtol", C.c_double),
+                ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64)]
+
+
+class GsResult(C.Structure):
+    """mpsd_gs_result (include/mpsd_gs.h)."""
+    _fields_ = [("energy", C.c_double), ("initial_energy", C.c_double), ("halfsweeps", C.c_int),
+                ("matvecs", C.c_int64), ("env_updates", C.c_int64), ("seconds", C.c_double),
+                ("residual", C.c_double), ("device_bytes", C.c_int64), ("n_history", C.c_int),
+                ("history", C.c_double * 256)]
+
+
+def gs_config(model, L, D, p, convtol=1e-10, eigtol=0.0, max_halfsweeps=0, precision=PREC_FP64,
+              seed=1, offset=0.0):
+    pp = list(p) + [0.0] * (4 - len(p))
+    return GsConfig(model, L, D, (C.c_double * 4)(*pp), offset, convtol, eigtol, max_halfsweeps,
+                    precision, seed)
+
+
+def gs_solve(ctx, cfg, export_path=None):
+    """Run one ground-state solve on the device; returns GsResult."""
+    res = GsResult()
+    if export_path is None:
+        ctx.check(ctx.lib.mpsd_gs_solve(ctx.handle, C.byref(cfg), C.byref(res)))
+    else:
+        ctx.check(ctx.lib.mpsd_gs_solve_export(ctx.handle, C.byref(cfg), export_path.encode(), C.byref(res)))
+    return res
 
 
 def exported_symbols():
@@ -288,6 +337,14 @@ class Heff:
     def _out_len(self):
         L, W, R = self._keep
         return W.dims[0] * L.dims[0] * R.dims[0]
+
+    def eigs_lowest(self, x_tensor, tol=1e-10, max_krylov=30):
+        """Lowest eigenpair with the device Lanczos solver; x_tensor holds the start vector
+        and receives the eigenvector.  Returns (value, residual, matvecs)."""
+        v, r, m = C.c_double(), C.c_double(), C.c_int()
+        self.ctx.check(self.ctx.lib.mpsd_heff_eigs_lowest(self._h, x_tensor.devptr(), tol, max_krylov,
+                                                          C.byref(v), C.byref(r), C.byref(m)))
+        return v.value, r.value, m.value
 
     def full(self):
         nr = self._out_len()

@@ -1,0 +1,150 @@
+"""GPU checks of the eigensolver support (multi-dot, GEMV update, axpby), the gauge steps
+(QR/LQ on the device, neighbour absorption) and the device Lanczos solver, through the
+C-ABI, against numpy."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mpsd_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from mpsdyn_amd import device as dev
+    c = dev.Context(0)
+    yield c
+    c.close()
+
+
+def test_vec_dots_gemv_axpby(ctx):
+    n, m = 10007, 7
+    V = orc.complex_gaussian(1, (n, m))
+    w = orc.complex_gaussian(2, (n,))
+    tV, tw = ctx.tensor(V), ctx.tensor(w)
+    out = np.zeros(m, dtype=np.complex128)
+    ctx.check(ctx.lib.mpsd_vec_dots(ctx.handle, tV.devptr(), n, m, tw.devptr(), n, out.ctypes.data))
+    assert np.allclose(out, V.conj().T @ w, rtol=1e-13, atol=1e-10)
+    c = orc.complex_gaussian(3, (m,))
+    ctx.check(ctx.lib.mpsd_vec_gemv(ctx.handle, tV.devptr(), n, m, c.ctypes.data, tw.devptr(), n))
+    assert np.allclose(tw.download(), w + V @ c, rtol=1e-13, atol=1e-12)
+    a = np.array([0.5, -1.0])
+    b = np.array([2.0, 0.25])
+    x = orc.complex_gaussian(4, (n,))
+    tx, ty = ctx.tensor(x), ctx.tensor(w)
+    ctx.check(ctx.lib.mpsd_vec_axpby(ctx.handle, a.ctypes.data, tx.devptr(), b.ctypes.data, ty.devptr(), n))
+    assert np.allclose(ty.download(), (0.5 - 1j) * x + (2 + 0.25j) * w, rtol=1e-14, atol=1e-13)
+    nr = C.c_double()
+    ctx.check(ctx.lib.mpsd_vec_norm2(ctx.handle, tx.devptr(), n, C.byref(nr)))
+    assert np.isclose(nr.value, np.vdot(x, x).real, rtol=1e-13)
+
+
+def test_dots_are_bit_reproducible(ctx):
+    n = 300001
+    V = orc.complex_gaussian(5, (n, 3))
+    w = orc.complex_gaussian(6, (n,))
+    tV, tw = ctx.tensor(V), ctx.tensor(w)
+    outs = []
+    for _ in range(3):
+        out = np.zeros(3, dtype=np.complex128)
+        ctx.check(ctx.lib.mpsd_vec_dots(ctx.handle, tV.devptr(), n, 3, tw.devptr(), n, out.ctypes.data))
+        outs.append(out)
+    assert all(np.array_equal(outs[0], o) for o in outs)
+
+
+@pytest.mark.parametrize("d,Dl,Dr", [(2, 8, 16), (2, 16, 8), (2, 1, 2), (2, 64, 64), (2, 4, 1), (3, 5, 7)])
+def test_gauge_r_and_reduce(ctx, d, Dl, Dr):
+    A = orc.complex_gaussian(7 + Dl, (d, Dl, Dr))
+    tA = ctx.tensor(A)
+    tR = ctx.tensor(dims=(1, 1))
+    ctx.check(ctx.lib.mpsd_site_gauge_r(ctx.handle, tA.handle, tR.handle))
+    Q = tA.download()
+    R = tR.download()
+    k = min(d * Dl, Dr)
+    assert Q.shape == (d, Dl, k) and R.shape == (k, Dr)
+    Qm = orc.reshape_f(Q, (d * Dl, k))
+    assert np.allclose(Qm.conj().T @ Qm, np.eye(k), atol=1e-13)
+    assert np.allclose(Qm @ R, orc.reshape_f(A, (d * Dl, Dr)), atol=1e-12)
+    assert np.allclose(np.tril(R, -1), 0)
+    # absorb R into a neighbour: B'[s, :, c] = R . B[s, :, c]
+    B = orc.complex_gaussian(9, (d, Dr, 5))
+    tB = ctx.tensor(B)
+    ctx.check(ctx.lib.mpsd_site_reduce_dim_r(ctx.handle, tB.handle, tR.handle))
+    assert np.allclose(tB.download(), orc._absorb_right(B, R), atol=1e-12)
+
+
+@pytest.mark.parametrize("d,Dl,Dr", [(2, 8, 16), (2, 16, 8), (2, 2, 1), (2, 64, 64), (2, 1, 4), (3, 7, 5)])
+def test_gauge_l_and_reduce(ctx, d, Dl, Dr):
+    A = orc.complex_gaussian(17 + Dr, (d, Dl, Dr))
+    tA = ctx.tensor(A)
+    tL = ctx.tensor(dims=(1, 1))
+    ctx.check(ctx.lib.mpsd_site_gauge_l(ctx.handle, tA.handle, tL.handle))
+    Q = tA.download()
+    Lm = tL.download()
+    k = min(Dl, d * Dr)
+    assert Q.shape == (d, k, Dr) and Lm.shape == (Dl, k)
+    Qm = orc.reshape_f(orc.permute(Q, (2, 1, 3)), (k, d * Dr))
+    assert np.allclose(Qm @ Qm.conj().T, np.eye(k), atol=1e-13)
+    assert np.allclose(Lm @ Qm, orc.reshape_f(orc.permute(A, (2, 1, 3)), (Dl, d * Dr)), atol=1e-12)
+    B = orc.complex_gaussian(19, (d, 3, Dl))
+    tB = ctx.tensor(B)
+    ctx.check(ctx.lib.mpsd_site_reduce_dim_l(ctx.handle, tB.handle, tL.handle))
+    assert np.allclose(tB.download(), orc._absorb_left(B, Lm), atol=1e-12)
+
+
+def test_edge_phase_convention(ctx):
+    """Site::gaugeR on a last site (Dr = 1) leaves a real positive factor (Site.cpp:202-207)."""
+    A = orc.complex_gaussian(21, (2, 4, 1))
+    tA, tR = ctx.tensor(A), ctx.tensor(dims=(1, 1))
+    ctx.check(ctx.lib.mpsd_site_gauge_r(ctx.handle, tA.handle, tR.handle))
+    r = tR.download().ravel()[0]
+    assert abs(r.imag) < 1e-14 and r.real > 0
+    assert np.isclose(r.real, np.linalg.norm(A))
+    assert np.allclose(tA.download() * r, A, atol=1e-13)
+
+
+@pytest.mark.parametrize("D,w", [(6, 3), (20, 3), (32, 5)])
+def test_lanczos_lowest_matches_dense(ctx, D, w):
+    """Device Lanczos on a Hermitian Heff against numpy's eigh of the dense oracle Heff."""
+    from mpsdyn_amd import device as dev
+    L_ = 8
+    mpo = orc.heisenberg_mpo(L_, 1.0, 1.0, 0.7, 0.1) if w == 5 else orc.ising_mpo(L_, -1.0, -1.0, -0.5)
+    mps = orc.gauge_left_all(orc.gauge_right_all(orc.random_mps(L_, D, seed=D)))
+    k = 4
+    for j in range(k):
+        q, r = orc._qr_site(mps[j])
+        mps[j] = q
+        mps[j + 1] = orc._absorb_right(mps[j + 1], r)
+    envL = np.ones((1, 1, 1), dtype=complex)
+    for j in range(k):
+        envL = orc.contract_left(envL, mps[j], mpo[j], mps[j])
+    envR = np.ones((1, 1, 1), dtype=complex)
+    for j in range(L_ - 1, k, -1):
+        envR = orc.contract_right(envR, mps[j], mpo[j], mps[j])
+    H = orc.heff_full(envL, mpo[k], envR)
+    evals = np.linalg.eigvalsh(0.5 * (H + H.conj().T))
+    h = dev.Heff(ctx, ctx.tensor(envL), ctx.tensor(mpo[k]), ctx.tensor(envR))
+    x = ctx.tensor(mps[k])
+    val, res, nmv = h.eigs_lowest(x, tol=1e-11)
+    assert abs(val - evals[0]) <= 1e-10 * max(1, abs(evals[0]))
+    v = x.download().reshape(-1, order="F")
+    assert np.isclose(np.linalg.norm(v), 1.0)
+    assert np.linalg.norm(H @ v - val * v) <= 1e-9 * max(1, abs(val))
+    h.close()
+
+
+def test_lanczos_reports_nan(ctx):
+    """A NaN in an environment is reported as MPSD_ERR_NUMERIC, not a crash or hang."""
+    from mpsdyn_amd import device as dev
+    L = orc.complex_gaussian(1, (4, 3, 4))
+    L[1, 1, 1] = np.nan
+    W = orc.ising_mpo(6, -1, -1, 0)[2]
+    R = orc.complex_gaussian(2, (4, 3, 4))
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R))
+    x = ctx.tensor(orc.complex_gaussian(3, (2, 4, 4)))
+    with pytest.raises(dev.MpsdError) as e:
+        h.eigs_lowest(x)
+    assert e.value.code == -5
+    h.close()
