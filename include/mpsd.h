@@ -70,6 +70,10 @@ int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on);
 int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 /* Force the BRGEMM tile (tm, tn in {32, 64}); 0, 0 restores the built-in choice. */
 int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
+/* Other launch options of the context (used by the tile tuner). */
+#define MPSD_OPT_TILE_K 1       /* K depth of a BRGEMM LDS stage: 16 (default) or 32 */
+#define MPSD_OPT_SPLIT_STAGE3 2 /* matvec last stage split over the MPO bond: -1 auto, 0 off, 1 on */
+int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
 
 /* ---- device tensors -------------------------------------------------------------------
  * HBM-resident counterpart of an mwArray (src/wrapper/mwArray.h:23-498): rank <= 8,

@@ -284,8 +284,25 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn) {
   if (!ctx) return MPSD_ERR_ARG;
   const bool ok = (tm == 0 && tn == 0) || ((tm == 32 || tm == 64) && (tn == 32 || tn == 64));
   if (!ok) return set_err(ctx, MPSD_ERR_ARG, "set_tile: tm, tn must be 32 or 64 (or both 0)");
-  ctx->tile_override = mpsd::TileCfg{tm, tn};
+  ctx->tile_override.tm = tm;
+  ctx->tile_override.tn = tn;
   return MPSD_OK;
+}
+
+int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
+  if (!ctx) return MPSD_ERR_ARG;
+  switch (option) {
+    case MPSD_OPT_TILE_K:
+      if (value != 16 && value != 32) return set_err(ctx, MPSD_ERR_ARG, "set_option: tile k must be 16 or 32");
+      ctx->tile_override.tk = value;
+      return MPSD_OK;
+    case MPSD_OPT_SPLIT_STAGE3:
+      if (value < -1 || value > 1) return set_err(ctx, MPSD_ERR_ARG, "set_option: split must be -1, 0 or 1");
+      ctx->split3 = value;
+      return MPSD_OK;
+    default:
+      return set_err(ctx, MPSD_ERR_ARG, "set_option: unknown option");
+  }
 }
 
 const char* mpsd_last_error(const mpsd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -605,6 +622,54 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   if (rc) return rc;
   rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr);
   if (rc) return rc;
+  const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
+  // Split the last stage over the MPO bond when its plain grid would leave CUs idle: each
+  // beta gives a partial product of the same shape as stage 1, summed in a fixed order.
+  bool split = false;
+  if (h->d == 2 && h->wr >= 2) {
+    if (ctx->split3 == 1)
+      split = true;
+    else if (ctx->split3 == -1) {
+      const int64_t tiles = (int64_t)((h->Db + 31) / 32) * ((h->Dbr + 31) / 32);
+      split = tiles < 512;
+    }
+  }
+  if (split) {
+    rc = ensure_scratch(ctx, 0, sizeof(double2) * (size_t)(ny * h->wr));
+    if (rc) return rc;
+    double2* P = (double2*)ctx->scratch[0];  // T1 is dead once the mix has run
+    // P[be][s' + d*b + d*Db*c] = sum_ar T2[b + Db*s' + Db*d*be + Db*d*wr*ar] R[c + Dbr*be + Dbr*wr*ar]
+    ZgemmDesc g;
+    g.A = (const double2*)ctx->scratch[1];
+    g.B = h->R->ptr;
+    g.C = P;
+    g.M = h->Db;
+    g.N = h->Dbr;
+    g.K = h->Dkr;
+    g.NB = h->d * h->wr;
+    g.group = h->d;
+    g.sAm = 1;
+    g.sAk = (int64_t)h->Db * h->d * h->wr;
+    g.sAb = h->Db;
+    g.sAz = (int64_t)h->Db * h->d;
+    g.sBk = (int64_t)h->Dbr * h->wr;
+    g.sBn = 1;
+    g.sBb = 0;
+    g.sBz = h->Dbr;
+    g.sCm = h->d;
+    g.sCn = (int64_t)h->d * h->Db;
+    g.sCb = 1;
+    g.sCz = ny;
+    g.a_mfast = 1;
+    g.b_kfast = 0;
+    hipError_t e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
+                         [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+    if (e != hipSuccess) return hip_err(ctx, e, "heff stage3 split");
+    e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+      return launch_zsum_parts_f64(P, h->wr, ny, h->offset != 0.0 ? x : nullptr, h->offset, y, ny, ctx->stream);
+    });
+    return hip_err(ctx, e, "heff stage3 sum");
+  }
   // y[s', b, c] = sum_{k = be + wr*ar} T2[b + Db*s' + Db*d*k] R[c + Dbr*k]
   ZgemmDesc g;
   g.A = (const double2*)ctx->scratch[1];

@@ -26,6 +26,11 @@ struct ZgemmDesc {
   int64_t sAm = 0, sAk = 0, sAr = 0, sAb = 0;
   int64_t sBk = 0, sBn = 0, sBr = 0, sBb = 0;
   int64_t sCm = 0, sCn = 0, sCb = 0;
+  // Batches come in groups: batch = j + group * z, j < group.  With group = 0 the launcher
+  // picks the group size and sets the z strides to group * s*b; with group > 0 (1 or 2)
+  // the z strides below are used as given.
+  int group = 0;
+  int64_t sAz = 0, sBz = 0, sCz = 0;
   int conjA = 0;
   int a_mfast = 1;  // load order of the A tile: m fastest (1) or k fastest (0)
   int b_kfast = 1;  // load order of the B tile: k fastest (1) or n fastest (0)
@@ -33,10 +38,14 @@ struct ZgemmDesc {
 
 // Tile configuration of a BRGEMM launch (chosen by the tuner or by heuristics).
 struct TileCfg {
-  int tm = 32, tn = 32;
+  int tm = 0, tn = 0, tk = 16;  // tm = tn = 0: built-in choice
 };
 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
+
+// y[i] = sum_{p < np} P[p * stride + i] (+ beta * x[i]) in a fixed order (x may be null).
+hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const double2* x, double beta, double2* y,
+                                 int64_t n, hipStream_t s);
 
 // Apply the MPO "mix" of the middle contraction stage:
 //   out[bx + Bx*o + Bx*nout*ax] = sum_i mix[o + nout*i] * in[bx + Bx*i + Bx*nin*ax]
@@ -71,7 +80,8 @@ struct mpsd_ctx {
   void* stage[2] = {nullptr, nullptr};
   size_t stage_bytes[2] = {0, 0};
   int64_t tensor_bytes = 0;
-  mpsd::TileCfg tile_override{0, 0};
+  mpsd::TileCfg tile_override{0, 0, 16};
+  int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   struct Pending {

@@ -25,8 +25,30 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ double2 cload(const double2* p) { return *p; }
 
+// Workgroup -> tile mapping.  The hardware hands linear workgroup ids to the 8 XCDs round
+// robin; the ids are first regrouped so each XCD owns a contiguous range, and that range is
+// walked in groups of kGroupM row-tiles so the tiles resident in one XCD's L2 share operand
+// panels.  z (the batch group) is slowest.
+constexpr int kGroupM = 8;
+
+__device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, int& z) {
+  const int total = gridDim.x;
+  int id = blockIdx.x;
+  if ((total & 7) == 0) id = (id & 7) * (total >> 3) + (id >> 3);
+  const int per_z = gm * gn;
+  z = id / per_z;
+  const int r = id - z * per_z;
+  const int in_group = kGroupM * gn;
+  const int g = r / in_group;
+  const int first_m = g * kGroupM;
+  const int gsize = min(gm - first_m, kGroupM);
+  const int rr = r - g * in_group;
+  tm_i = first_m + rr % gsize;
+  tn_i = rr / gsize;
+}
+
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p) {
+__global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p, int gm, int gn) {
   constexpr int NA = ASH ? 1 : NBW;
   constexpr int NB_ = BSH ? 1 : NBW;
   __shared__ double2 As[NA][TK][TM];
@@ -42,10 +64,11 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = blockIdx.x * TM;
-  const int n0 = blockIdx.y * TN;
-  const int b0 = blockIdx.z * NBW;
-  const int nbw = min(NBW, p.NB - b0);
+  int tm_i, tn_i, z;
+  tile_of(gm, gn, tm_i, tn_i, z);
+  const int m0 = tm_i * TM;
+  const int n0 = tn_i * TN;
+  const int nbw = min(NBW, p.NB - z * NBW);
 
   d4 accr[NBW][BM][BN];
   d4 acci[NBW][BM][BN];
@@ -92,7 +115,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       const int jb = (j < nbw) ? j : 0;
-      const double2* base = p.A + (int64_t)(b0 + jb) * p.sAb + (int64_t)r * p.sAr;
+      const double2* base = p.A + (int64_t)z * p.sAz + (int64_t)jb * p.sAb + (int64_t)r * p.sAr;
 #pragma unroll
       for (int e = 0; e < AE; ++e) {
         int m, k;
@@ -105,7 +128,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
 #pragma unroll
     for (int j = 0; j < NB_; ++j) {
       const int jb = (j < nbw) ? j : 0;
-      const double2* base = p.B + (int64_t)(b0 + jb) * p.sBb + (int64_t)r * p.sBr;
+      const double2* base = p.B + (int64_t)z * p.sBz + (int64_t)jb * p.sBb + (int64_t)r * p.sBr;
 #pragma unroll
       for (int e = 0; e < BE; ++e) {
         int k, n;
@@ -194,8 +217,8 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
 #pragma unroll
   for (int j = 0; j < NBW; ++j) {
     if (j >= nbw) break;
-    double2* cb = p.C + (int64_t)(b0 + j) * p.sCb;
-    const double2* db = p.Dadd ? p.Dadd + (int64_t)(b0 + j) * p.sCb : nullptr;
+    double2* cb = p.C + (int64_t)z * p.sCz + (int64_t)j * p.sCb;
+    const double2* db = p.Dadd ? p.Dadd + (int64_t)z * p.sCz + (int64_t)j * p.sCb : nullptr;
 #pragma unroll
     for (int i = 0; i < BM; ++i)
 #pragma unroll
@@ -219,37 +242,61 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
   }
 }
 
-template <int TM, int TN, int NBW, bool ASH, bool BSH>
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
 hipError_t launch_cfg(const ZgemmDesc& p, hipStream_t s) {
-  constexpr int TK = 16;
-  dim3 grid((p.M + TM - 1) / TM, (p.N + TN - 1) / TN, (p.NB + NBW - 1) / NBW);
-  hipLaunchKernelGGL((zbrgemm_f64_kernel<TM, TN, TK, NBW, ASH, BSH>), grid, dim3(kThreads), 0, s, p);
+  const int gm = (p.M + TM - 1) / TM, gn = (p.N + TN - 1) / TN, gz = (p.NB + NBW - 1) / NBW;
+  const int64_t total = (int64_t)gm * gn * gz;
+  if (total > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((zbrgemm_f64_kernel<TM, TN, TK, NBW, ASH, BSH>), dim3((unsigned)total), dim3(kThreads), 0, s,
+                     p, gm, gn);
   return hipGetLastError();
 }
 
-template <int TM, int TN>
-hipError_t launch_tile(const ZgemmDesc& p, hipStream_t s) {
-  if (p.NB >= 2 && p.sAb == 0) return launch_cfg<TM, TN, 2, true, false>(p, s);
-  if (p.NB >= 2 && p.sBb == 0) return launch_cfg<TM, TN, 2, false, true>(p, s);
-  return launch_cfg<TM, TN, 1, false, false>(p, s);
+template <int TM, int TN, int TK>
+hipError_t launch_tile(const ZgemmDesc& p, int nbw, hipStream_t s) {
+  if (nbw == 2 && p.sAb == 0) return launch_cfg<TM, TN, TK, 2, true, false>(p, s);
+  if (nbw == 2 && p.sBb == 0) return launch_cfg<TM, TN, TK, 2, false, true>(p, s);
+  if (nbw == 2) return launch_cfg<TM, TN, TK, 2, false, false>(p, s);
+  return launch_cfg<TM, TN, TK, 1, false, false>(p, s);
+}
+
+template <int TK>
+hipError_t launch_tk(const ZgemmDesc& p, TileCfg cfg, int nbw, hipStream_t s) {
+  if (cfg.tm == 64 && cfg.tn == 64) return launch_tile<64, 64, TK>(p, nbw, s);
+  if (cfg.tm == 64 && cfg.tn == 32) return launch_tile<64, 32, TK>(p, nbw, s);
+  if (cfg.tm == 32 && cfg.tn == 64) return launch_tile<32, 64, TK>(p, nbw, s);
+  return launch_tile<32, 32, TK>(p, nbw, s);
 }
 
 }  // namespace
 
-hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s) {
-  if (p.M <= 0 || p.N <= 0 || p.NB <= 0) return hipSuccess;
+hipError_t launch_zbrgemm_f64(const ZgemmDesc& pin, TileCfg cfg, hipStream_t s) {
+  if (pin.M <= 0 || pin.N <= 0 || pin.NB <= 0) return hipSuccess;
+  ZgemmDesc p = pin;
+  // batches per workgroup: an explicit group, else two when one operand is shared by all
+  int nbw;
+  if (p.group > 0) {
+    nbw = p.group;
+    if (nbw > 2 || p.NB % nbw != 0) return hipErrorInvalidValue;
+  } else {
+    nbw = (p.NB >= 2 && (p.sAb == 0 || p.sBb == 0)) ? 2 : 1;
+    p.sAz = nbw * p.sAb;
+    p.sBz = nbw * p.sBb;
+    p.sCz = nbw * p.sCb;
+  }
   if (cfg.tm == 0 || cfg.tn == 0) {
     // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.
-    const int64_t zb = (p.NB >= 2 && (p.sAb == 0 || p.sBb == 0)) ? (p.NB + 1) / 2 : p.NB;
+    const int64_t zb = (p.NB + nbw - 1) / nbw;
     auto wgs = [&](int tm, int tn) { return (int64_t)((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn) * zb; };
-    if (wgs(64, 64) >= 512) cfg = {64, 64};
-    else if (wgs(64, 32) >= 512) cfg = {64, 32};
-    else cfg = {32, 32};
+    if (wgs(64, 64) >= 512)
+      cfg.tm = 64, cfg.tn = 64;
+    else if (wgs(64, 32) >= 512)
+      cfg.tm = 64, cfg.tn = 32;
+    else
+      cfg.tm = 32, cfg.tn = 32;
   }
-  if (cfg.tm == 64 && cfg.tn == 64) return launch_tile<64, 64>(p, s);
-  if (cfg.tm == 64 && cfg.tn == 32) return launch_tile<64, 32>(p, s);
-  if (cfg.tm == 32 && cfg.tn == 64) return launch_tile<32, 64>(p, s);
-  return launch_tile<32, 32>(p, s);
+  if (cfg.tk == 32) return launch_tk<32>(p, cfg, nbw, s);
+  return launch_tk<16>(p, cfg, nbw, s);
 }
 
 // ------------------------------------------------------------------------------ MPO mix
@@ -317,6 +364,36 @@ __global__ void __launch_bounds__(kThreads) heff_full_f64_kernel(
 }
 
 }  // namespace
+
+namespace {
+// y[i] = sum_{p < np} P[p * stride + i] (+ beta * x[i]), summed in a fixed order.
+__global__ void __launch_bounds__(kThreads) zsum_parts_kernel(const double2* __restrict__ P, int np, int64_t stride,
+                                                              const double2* __restrict__ x, double beta,
+                                                              double2* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = P[i];
+  for (int q = 1; q < np; ++q) {
+    const double2 v = P[(int64_t)q * stride + i];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  if (x) {
+    const double2 v = x[i];
+    acc.x += beta * v.x;
+    acc.y += beta * v.y;
+  }
+  y[i] = acc;
+}
+}  // namespace
+
+hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const double2* x, double beta, double2* y,
+                                 int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zsum_parts_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, P, np,
+                     stride, x, beta, y, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, int Bx, int nin,
                            int nout, int64_t Ax, hipStream_t s) {

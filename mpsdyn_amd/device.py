@@ -23,6 +23,9 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X3 = 2
 
+OPT_TILE_K = 1
+OPT_SPLIT_STAGE3 = 2
+
 _lib = None
 
 
@@ -97,6 +100,7 @@ def _optional(lib):
         "mpsd_ctx_set_timing": (I, [P, I]),
         "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
         "mpsd_ctx_set_tile": (I, [P, I, I]),
+        "mpsd_ctx_set_option": (I, [P, I, I]),
     }
     for name, (res, args) in extra.items():
         if hasattr(lib, name):
@@ -230,6 +234,9 @@ class Context:
 
     def set_tile(self, tm, tn):
         self.check(self.lib.mpsd_ctx_set_tile(self._h, tm, tn))
+
+    def set_option(self, option, value):
+        self.check(self.lib.mpsd_ctx_set_option(self._h, option, value))
 
 
 class Tensor:
