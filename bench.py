@@ -9,7 +9,13 @@ dominant kernel measured with HIP events on the library's stream during the time
 the environment-update time at the same shape, and a CPU baseline (the numpy oracle with
 the reference's contraction order, OpenBLAS ZGEMM on the host cores) on a bounded sample.
 
+Ground-state part of the metric: --gs-solves N independent solves of the config-1 chain
+(TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20, convtol 1e-12, seeds 1..N) run back to back;
+reported as solves/s, p50/p99 latency and the worst energy error against the reference's
+own value for that chain.
+
     python bench.py [--steps K] [--warmup W] [--D 512] [--w 3] [--cpu-seconds 15] [--no-cpu]
+                    [--gs-solves N]
 """
 from __future__ import annotations
 
@@ -71,6 +77,26 @@ def cpu_baseline(L, W, R, x, flops, seconds):
     }
 
 
+def ground_states(ctx, dev, n):
+    """n config-1 solves back to back: solves/s, latency percentiles, energy error."""
+    with open(os.path.join(ROOT, "tests", "golden", "reference_values.json")) as f:
+        ref = json.load(f)["config1_tfim_L20_D20"]["reference_energy"]
+    lat, errs, mv = [], [], 0
+    dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12, seed=99))
+    t0 = time.perf_counter()
+    for k in range(n):
+        t = time.perf_counter()
+        r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12, seed=k + 1))
+        lat.append(time.perf_counter() - t)
+        errs.append(abs(r.energy - ref) / abs(ref))
+        mv += r.matvecs
+    wall = time.perf_counter() - t0
+    return {"workload": "tfim_L20_D20_convtol1e-12_fp64", "solves": n, "solves_per_s": round(n / wall, 3),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "max_rel_energy_error_vs_reference": float(max(errs)), "matvecs_per_solve": mv / n}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +106,7 @@ def main():
     ap.add_argument("--w", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gs-solves", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
@@ -153,6 +180,8 @@ def main():
         env[f"env_{side}_ms"] = 1e3 * (time.perf_counter() - te0) / n_env
     env_tflops = F / (env["env_left_ms"] * 1e-3) / 1e12
 
+    gs = ground_states(ctx, dev, args.gs_solves) if args.gs_solves > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, W, R, x, F, args.cpu_seconds)
@@ -179,6 +208,7 @@ def main():
                      "traffic": traffic.get("bytes_per_launch") if traffic else None},
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
         "env_update": {k: round(v, 4) for k, v in env.items()} | {"env_left_tflops": round(env_tflops, 3)},
+        "gs": gs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
