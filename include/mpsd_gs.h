@@ -45,6 +45,8 @@ typedef struct mpsd_gs_result {
   double seconds;         /* wall time of findGroundState */
   double residual;        /* last local eigen-residual */
   int64_t device_bytes;   /* HBM held by the context after the solve */
+  int64_t matvecs_lowp;   /* matvecs done in the low-precision form (precision != FP64) */
+  int refine_from;        /* first half-sweep refined in FP64 (-1: FP64 throughout) */
   int n_history;
   double history[MPSD_GS_MAX_HISTORY]; /* energy at the end of each half-sweep */
 } mpsd_gs_result;

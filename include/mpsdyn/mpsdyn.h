@@ -270,7 +270,9 @@ enum EigSolver { primme, arpack, fulleig, lanczos };
 
 struct SolveStats {
   int halfsweeps = 0;
-  int64_t matvecs = 0;
+  int64_t matvecs = 0;        // all matvecs
+  int64_t matvecs_lowp = 0;   // of which in the low-precision form (precision != FP64)
+  int refine_from = -1;       // first half-sweep with FP64 refinement (-1: FP64 throughout)
   int64_t env_updates = 0;
   double final_residual = 0.0;
   std::vector<double> energies;  // energy at the end of each half-sweep

@@ -108,8 +108,12 @@ static bool rank_ok(const mpsd_tensor* t, int r) {
 // Stage 1 + 2 of a left-type contraction into T2 = scratch[1]:
 //   T1[b, al, s, ar] = sum_a E[b, al, a] X[s, a, ar]           (BRGEMM, batch s)
 //   T2[b, s', be, ar] = sum_{al,s} W[s', al, s, be] T1[b, al, s, ar]   (mix)
+static int arith_of(int precision) {
+  return precision == MPSD_PREC_FP32 ? 1 : precision == MPSD_PREC_BF16X3 ? 2 : 0;
+}
+
 static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, const double2* X,
-                       int d, int Dkr, const double2* mix, int wr) {
+                       int d, int Dkr, const double2* mix, int wr, int arith = 0) {
   const size_t t1 = sizeof(double2) * (size_t)Db * wl * d * Dkr;
   const size_t t2 = sizeof(double2) * (size_t)Db * d * wr * Dkr;
   int rc = ensure_scratch(ctx, 0, t1);
@@ -135,6 +139,7 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
   g.sCb = (int64_t)Db * wl;
   g.a_mfast = 1;
   g.b_kfast = 1;
+  g.arith = arith;
   hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
                        [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
   if (e != hipSuccess) return hip_err(ctx, e, "stage1 launch");
@@ -561,8 +566,8 @@ int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* 
   *out = nullptr;
   if (!rank_ok(envL, 3) || !rank_ok(envR, 3) || !rank_ok(W, 4))
     return set_err(ctx, MPSD_ERR_SHAPE, "heff_create: rank too high");
-  if (precision != MPSD_PREC_FP64)
-    return set_err(ctx, MPSD_ERR_UNSUPPORTED, "heff_create: precision not built");
+  if (precision < MPSD_PREC_FP64 || precision > MPSD_PREC_BF16X3)
+    return set_err(ctx, MPSD_ERR_ARG, "heff_create: unknown precision");
   mpsd_heff* h = new mpsd_heff();
   h->ctx = ctx;
   h->L = envL;
@@ -609,8 +614,8 @@ int64_t mpsd_heff_size(const mpsd_heff* h) {
 
 int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
   if (!h) return MPSD_ERR_ARG;
-  if (precision != MPSD_PREC_FP64)
-    return set_err(h->ctx, MPSD_ERR_UNSUPPORTED, "heff_set_precision: precision not built");
+  if (precision < MPSD_PREC_FP64 || precision > MPSD_PREC_BF16X3)
+    return set_err(h->ctx, MPSD_ERR_ARG, "heff_set_precision: unknown precision");
   h->precision = precision;
   return MPSD_OK;
 }
@@ -620,7 +625,8 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   mpsd_ctx* ctx = h->ctx;
   int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
   if (rc) return rc;
-  rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr);
+  const int arith = arith_of(h->precision);
+  rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr, arith);
   if (rc) return rc;
   const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
   // Split the last stage over the MPO bond when its plain grid would leave CUs idle: each
@@ -662,6 +668,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
     g.sCz = ny;
     g.a_mfast = 1;
     g.b_kfast = 0;
+    g.arith = arith;
     hipError_t e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
                          [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
     if (e != hipSuccess) return hip_err(ctx, e, "heff stage3 split");
@@ -690,6 +697,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   g.sCb = 1;
   g.a_mfast = 1;
   g.b_kfast = 0;
+  g.arith = arith;
   if (h->offset != 0.0) {
     g.Dadd = x;
     g.beta = h->offset;

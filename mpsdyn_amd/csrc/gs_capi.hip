@@ -57,6 +57,8 @@ int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mps
     out->residual = st.final_residual;
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
     out->device_bytes = mpsd_ctx_device_bytes(ctx);
+    out->matvecs_lowp = st.matvecs_lowp;
+    out->refine_from = st.refine_from;
     out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
     for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
     if (export_path) mps.exportMPS(export_path);

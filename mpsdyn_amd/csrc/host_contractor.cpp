@@ -78,25 +78,50 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   int pos = 0, round = 0;
   bool right = true, done = false;
   const double tol = eigtol_ > 0 ? eigtol_ : 1e-10;
+  // Precision form: local solves start with the low-precision matvec; once a half-sweep
+  // changes the energy by less than the low-precision floor, every site is refined with the
+  // FP64 matvec from that vector, and only FP64 energies enter the stopping rule and the
+  // energy-increase guard.
+  const bool lowp = precision_ != MPSD_PREC_FP64;
+  const double low_tol = precision_ == MPSD_PREC_FP32 ? 1e-6 : 1e-4;  // residual of the low solve
+  const double switch_tol = precision_ == MPSD_PREC_FP32 ? 1e-7 : 3e-5;  // relative dE per half-sweep
+  bool refine = !lowp;
+  bool sweep_all_fp64 = refine;
+  bool have_fp64 = false;
   while (!done) {
     calculateL(pos, mgr, ops, init, init);
     calculateR(pos, mgr, ops, init, init);
     Site& site = init.site(pos);
+    double* xv = (double*)mpsd_tensor_devptr(site.device());
     {
       TensorMultiplierOffset multi(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset,
-                                   precision_);
-      EigStats es = eig_->lowest(multi, (double*)mpsd_tensor_devptr(site.device()), tol);
-      stats_.matvecs += es.matvecs;
-      stats_.final_residual = es.residual;
-      val = es.value - offset;
+                                   lowp ? precision_ : MPSD_PREC_FP64);
+      if (lowp) {
+        EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol));
+        stats_.matvecs += es.matvecs;
+        stats_.matvecs_lowp += es.matvecs;
+        stats_.final_residual = es.residual;
+        val = es.value - offset;
+      }
+      if (refine) {
+        if (lowp) multi.setPrecision(MPSD_PREC_FP64);
+        EigStats es = eig_->lowest(multi, xv, tol);
+        stats_.matvecs += es.matvecs;
+        stats_.final_residual = es.residual;
+        val = es.value - offset;
+      }
     }
-    // energy-increase guard (Contractor.cpp:1417-1434)
-    if ((val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
+    // energy-increase guard (Contractor.cpp:1417-1434), between FP64 energies only
+    if (refine && have_fp64 && (val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
         (val - energy > std::fabs(energy + offset) * convtol_)) {
       char buf[256];
       snprintf(buf, sizeof buf, "energy increasing in this step: old %.15g new %.15g (round %d, pos %d)", energy,
                val, round, pos);
       throw Error(MPSD_ERR_NUMERIC, buf);
+    }
+    if (refine && !have_fp64) {
+      energy = val;  // first FP64 value: the guard's baseline from here on
+      have_fp64 = true;
     }
     init.gaugeCond(pos, right ? 'R' : 'L', true);
     for (int l = pos + 1; l < L; ++l) mgr.outofdateL(l);
@@ -104,11 +129,18 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     const bool endRight = right && pos == L - 1;
     const bool endLeft = !right && pos == 0;
     if (endRight || endLeft) {
-      done = std::fabs(energy - val) < convtol_ * std::fabs(energy + offset);
-      if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
+      const double dE = std::fabs(energy - val);
+      if (sweep_all_fp64) {
+        done = dE < convtol_ * std::fabs(energy + offset);
+        if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
+      } else if (!refine && dE < switch_tol * std::fabs(val + offset)) {
+        refine = true;  // next half-sweep refines every site in FP64
+        stats_.refine_from = round + 1;
+      }
       energy = val;
       stats_.energies.push_back(val);
       ++round;
+      sweep_all_fp64 = refine;
       if (max_halfsweeps_ > 0 && round >= max_halfsweeps_) done = true;
       if (!done) {
         if (endRight) {

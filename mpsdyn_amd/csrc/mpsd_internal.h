@@ -32,6 +32,7 @@ struct ZgemmDesc {
   int group = 0;
   int64_t sAz = 0, sBz = 0, sCz = 0;
   int conjA = 0;
+  int arith = 0;    // MFMA arithmetic: 0 FP64, 1 FP32 (FP64 in and out), 2 split-BF16
   int a_mfast = 1;  // load order of the A tile: m fastest (1) or k fastest (0)
   int b_kfast = 1;  // load order of the B tile: k fastest (1) or n fastest (0)
 };
@@ -42,6 +43,7 @@ struct TileCfg {
 };
 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
+hipError_t launch_zbrgemm_bf16x3(const ZgemmDesc& p, int nbw, hipStream_t s);
 
 // y[i] = sum_{p < np} P[p * stride + i] (+ beta * x[i]) in a fixed order (x may be null).
 hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const double2* x, double beta, double2* y,

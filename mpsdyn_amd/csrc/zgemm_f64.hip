@@ -18,10 +18,38 @@
 namespace mpsd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int kThreads = 256;
+
+// Arithmetic of the BRGEMM's MFMAs.  Operands always arrive as FP64 from HBM and leave as
+// FP64; the FP32 form converts when staging into LDS and accumulates in FP32.
+template <int ARITH>
+struct Pol;
+template <>
+struct Pol<0> {  // FP64: v_mfma_f64_16x16x4f64, element (m, n) in lane n + 16*(m%4), reg m/4
+  typedef double T;
+  typedef double2 T2;
+  typedef d4 Acc;
+  static __device__ __forceinline__ Acc mfma(T a, T b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  static __device__ __forceinline__ T2 cvt(double2 v, double s) { return T2{v.x, s * v.y}; }
+};
+template <>
+struct Pol<1> {  // FP32: v_mfma_f32_16x16x4f32, element (m, n) in lane n + 16*(m/4), reg m%4
+  typedef float T;
+  typedef float2 T2;
+  typedef f4 Acc;
+  static __device__ __forceinline__ Acc mfma(T a, T b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  static __device__ __forceinline__ T2 cvt(double2 v, double s) { return T2{(float)v.x, (float)(s * v.y)}; }
+};
 
 __device__ __forceinline__ double2 cload(const double2* p) { return *p; }
 
@@ -47,12 +75,16 @@ __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, in
   tn_i = rr / gsize;
 }
 
-template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p, int gm, int gn) {
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
+__global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
+  typedef Pol<ARITH> P;
+  typedef typename P::T T;
+  typedef typename P::T2 T2;
+  typedef typename P::Acc Acc;
   constexpr int NA = ASH ? 1 : NBW;
   constexpr int NB_ = BSH ? 1 : NBW;
-  __shared__ double2 As[NA][TK][TM];
-  __shared__ double2 Bs[NB_][TK][TN];
+  __shared__ T2 As[NA][TK][TM];
+  __shared__ T2 Bs[NB_][TK][TN];
 
   constexpr int WTM = TM / 2, WTN = TN / 2;  // 2x2 waves
   constexpr int BM = WTM / 16, BN = WTN / 16;
@@ -70,16 +102,16 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
   const int n0 = tn_i * TN;
   const int nbw = min(NBW, p.NB - z * NBW);
 
-  d4 accr[NBW][BM][BN];
-  d4 acci[NBW][BM][BN];
+  Acc accr[NBW][BM][BN];
+  Acc acci[NBW][BM][BN];
 #pragma unroll
   for (int j = 0; j < NBW; ++j)
 #pragma unroll
     for (int i = 0; i < BM; ++i)
 #pragma unroll
       for (int q = 0; q < BN; ++q) {
-        accr[j][i][q] = d4{0.0, 0.0, 0.0, 0.0};
-        acci[j][i][q] = d4{0.0, 0.0, 0.0, 0.0};
+        accr[j][i][q] = Acc{0, 0, 0, 0};
+        acci[j][i][q] = Acc{0, 0, 0, 0};
       }
 
   double2 ra[NA][AE];
@@ -148,7 +180,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
       for (int e = 0; e < AE; ++e) {
         int m, k;
         a_coord(e, m, k);
-        As[j][k][m] = double2{ra[j][e].x, csign * ra[j][e].y};
+        As[j][k][m] = P::cvt(ra[j][e], csign);
       }
 #pragma unroll
     for (int j = 0; j < NB_; ++j)
@@ -156,7 +188,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
       for (int e = 0; e < BE; ++e) {
         int k, n;
         b_coord(e, k, n);
-        Bs[j][k][n] = rb[j][e];
+        Bs[j][k][n] = P::cvt(rb[j][e], 1.0);
       }
   };
 
@@ -169,13 +201,13 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
     for (int kk = 0; kk < TK / 4; ++kk) {
       const int krow = kk * 4 + (lane >> 4);
       const int lc = lane & 15;
-      double ar[NA][BM], ai[NA][BM], an[NA][BM];
-      double br[NB_][BN], bi[NB_][BN];
+      T ar[NA][BM], ai[NA][BM], an[NA][BM];
+      T br[NB_][BN], bi[NB_][BN];
 #pragma unroll
       for (int j = 0; j < NA; ++j)
 #pragma unroll
         for (int i = 0; i < BM; ++i) {
-          const double2 v = As[j][krow][wm * WTM + i * 16 + lc];
+          const T2 v = As[j][krow][wm * WTM + i * 16 + lc];
           ar[j][i] = v.x;
           ai[j][i] = v.y;
           an[j][i] = -v.y;
@@ -184,7 +216,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
       for (int j = 0; j < NB_; ++j)
 #pragma unroll
         for (int q = 0; q < BN; ++q) {
-          const double2 v = Bs[j][krow][wn * WTN + q * 16 + lc];
+          const T2 v = Bs[j][krow][wn * WTN + q * 16 + lc];
           br[j][q] = v.x;
           bi[j][q] = v.y;
         }
@@ -196,15 +228,15 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
         for (int i = 0; i < BM; ++i)
 #pragma unroll
           for (int q = 0; q < BN; ++q) {
-            accr[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ja][i], br[jb][q], accr[j][i][q], 0, 0, 0);
-            acci[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ja][i], bi[jb][q], acci[j][i][q], 0, 0, 0);
+            accr[j][i][q] = P::mfma(ar[ja][i], br[jb][q], accr[j][i][q]);
+            acci[j][i][q] = P::mfma(ar[ja][i], bi[jb][q], acci[j][i][q]);
           }
 #pragma unroll
         for (int i = 0; i < BM; ++i)
 #pragma unroll
           for (int q = 0; q < BN; ++q) {
-            accr[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[ja][i], bi[jb][q], accr[j][i][q], 0, 0, 0);
-            acci[j][i][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[ja][i], br[jb][q], acci[j][i][q], 0, 0, 0);
+            accr[j][i][q] = P::mfma(an[ja][i], bi[jb][q], accr[j][i][q]);
+            acci[j][i][q] = P::mfma(ai[ja][i], br[jb][q], acci[j][i][q]);
           }
       }
     }
@@ -213,7 +245,6 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
 
   // Epilogue: C = acc (+ beta * Dadd).
   const int lc = lane & 15;
-  const int lr = lane >> 4;
 #pragma unroll
   for (int j = 0; j < NBW; ++j) {
     if (j >= nbw) break;
@@ -226,10 +257,10 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
         const int gn = n0 + wn * WTN + q * 16 + lc;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int gm = m0 + wm * WTM + i * 16 + lr + 4 * rr;
+          const int gm = m0 + wm * WTM + i * 16 + P::row(lane, rr);
           if (gm < p.M && gn < p.N) {
             const int64_t off = gm * p.sCm + gn * p.sCn;
-            double2 v{accr[j][i][q][rr], acci[j][i][q][rr]};
+            double2 v{(double)accr[j][i][q][rr], (double)acci[j][i][q][rr]};
             if (db) {
               const double2 x = db[off];
               v.x += p.beta * x.x;
@@ -242,14 +273,20 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_f64_kernel(const ZgemmDesc p
   }
 }
 
-template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
-hipError_t launch_cfg(const ZgemmDesc& p, hipStream_t s) {
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
+hipError_t launch_cfg_a(const ZgemmDesc& p, hipStream_t s) {
   const int gm = (p.M + TM - 1) / TM, gn = (p.N + TN - 1) / TN, gz = (p.NB + NBW - 1) / NBW;
   const int64_t total = (int64_t)gm * gn * gz;
   if (total > 0x7fffffff) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((zbrgemm_f64_kernel<TM, TN, TK, NBW, ASH, BSH>), dim3((unsigned)total), dim3(kThreads), 0, s,
-                     p, gm, gn);
+  hipLaunchKernelGGL((zbrgemm_kernel<TM, TN, TK, NBW, ASH, BSH, ARITH>), dim3((unsigned)total), dim3(kThreads), 0,
+                     s, p, gm, gn);
   return hipGetLastError();
+}
+
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
+hipError_t launch_cfg(const ZgemmDesc& p, hipStream_t s) {
+  if (p.arith == 1) return launch_cfg_a<TM, TN, TK, NBW, ASH, BSH, 1>(p, s);
+  return launch_cfg_a<TM, TN, TK, NBW, ASH, BSH, 0>(p, s);
 }
 
 template <int TM, int TN, int TK>
@@ -284,6 +321,7 @@ hipError_t launch_zbrgemm_f64(const ZgemmDesc& pin, TileCfg cfg, hipStream_t s) 
     p.sBz = nbw * p.sBb;
     p.sCz = nbw * p.sCb;
   }
+  if (p.arith == 2) return launch_zbrgemm_bf16x3(p, nbw, s);  // fixed 32x32 tiles
   if (cfg.tm == 0 || cfg.tn == 0) {
     // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.
     const int64_t zb = (p.NB + nbw - 1) / nbw;

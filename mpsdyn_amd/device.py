@@ -123,7 +123,8 @@ class GsResult(C.Structure):
     """mpsd_gs_result (include/mpsd_gs.h)."""
     _fields_ = [("energy", C.c_double), ("initial_energy", C.c_double), ("halfsweeps", C.c_int),
                 ("matvecs", C.c_int64), ("env_updates", C.c_int64), ("seconds", C.c_double),
-                ("residual", C.c_double), ("device_bytes", C.c_int64), ("n_history", C.c_int),
+                ("residual", C.c_double), ("device_bytes", C.c_int64), ("matvecs_lowp", C.c_int64),
+                ("refine_from", C.c_int), ("n_history", C.c_int),
                 ("history", C.c_double * 256)]
 
 

@@ -1,0 +1,69 @@
+"""Precision form: complex-FP32 and split-BF16 MFMA matvecs against the FP64 oracle, and
+ground-state energies of the mixed-precision sweep against the FP64 device sweep and exact
+diagonalisation.  Tolerances: matvec relative error 1e-6 (FP32) / 1e-4 (split BF16);
+energies 1e-10 relative to the FP64 result."""
+import numpy as np
+import pytest
+
+from oracle import mpsd_oracle as orc
+from tests import _physics as ph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from mpsdyn_amd import device as dev
+    c = dev.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 1e-4)])
+@pytest.mark.parametrize("Db,Dbr,wl,wr", [(20, 20, 3, 3), (37, 50, 5, 5), (1, 4, 1, 3), (128, 128, 5, 5),
+                                          (512, 512, 3, 3)])
+def test_lowp_matvec_accuracy(ctx, prec, tol, Db, Dbr, wl, wr):
+    from mpsdyn_amd import device as dev
+    print(f"prec={prec} Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
+    L = orc.complex_gaussian(1, (Db, wl, Db))
+    W = orc.complex_gaussian(2, (2, wl, 2, wr))
+    R = orc.complex_gaussian(3, (Dbr, wr, Dbr))
+    x = orc.complex_gaussian(4, (2 * Db * Dbr,))
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R), offset=0.3, precision=prec)
+    err = _rel(h.apply(x), orc.heff_product(L, W, R, x, offset=0.3))
+    assert err <= tol, err
+    assert err > 1e-15  # really the reduced-precision path
+    h.set_precision(0)
+    assert _rel(h.apply(x), orc.heff_product(L, W, R, x, offset=0.3)) <= 1e-12
+    h.close()
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_precision_schwinger_energy(ctx, prec):
+    """Schwinger N=12, x=100, m/g=0.125: mixed-precision energy within 1e-10 of the FP64
+    sweep and of exact diagonalisation, with most matvecs in the low-precision form."""
+    from mpsdyn_amd import device as dev
+    N, x, mg = 12, 100.0, 0.125
+    mu = 2 * mg * np.sqrt(x)
+    e_ed = ph.ground_energy(ph.schwinger_hamiltonian(N, mu, x))
+    p = [mu, x, 0.0, 0.0]
+    r64 = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, N, 48, p, convtol=1e-12))
+    rlp = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, N, 48, p, convtol=1e-12, precision=prec))
+    print(f"prec={prec} E64={r64.energy:.15f} Elp={rlp.energy:.15f} ED={e_ed:.15f} "
+          f"mv={rlp.matvecs} lowp={rlp.matvecs_lowp} refine_from={rlp.refine_from}", flush=True)
+    assert abs(rlp.energy - r64.energy) <= 1e-10 * abs(r64.energy)
+    assert abs(rlp.energy - e_ed) <= 1e-10 * abs(e_ed)
+    assert rlp.matvecs_lowp >= 0.5 * rlp.matvecs
+    assert rlp.refine_from >= 1
+
+
+def test_mixed_precision_ising_closed_form(ctx):
+    from mpsdyn_amd import device as dev
+    L = 40
+    r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, L, 32, [-1.0, -1.0, 0.0], convtol=1e-12, precision=1))
+    e = ph.tfim_free_fermion(L, 1.0, 1.0)
+    assert abs(r.energy - e) <= 1e-10 * abs(e)
