@@ -78,13 +78,17 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   int pos = 0, round = 0;
   bool right = true, done = false;
   const double tol = eigtol_ > 0 ? eigtol_ : 1e-10;
-  // Precision form: local solves start with the low-precision matvec; once a half-sweep
-  // changes the energy by less than the low-precision floor, every site is refined with the
-  // FP64 matvec from that vector, and only FP64 energies enter the stopping rule and the
+  // Precision form, a ladder of phases: split-BF16 -> complex FP32 -> FP64.  A phase ends
+  // when a half-sweep changes the energy by less than that arithmetic's floor; in the last
+  // phase each site is first solved with the FP32 matvec and then refined with the FP64
+  // one from that vector, and only FP64 energies enter the stopping rule and the
   // energy-increase guard.
   const bool lowp = precision_ != MPSD_PREC_FP64;
-  const double low_tol = precision_ == MPSD_PREC_FP32 ? 1e-6 : 1e-4;  // residual of the low solve
-  const double switch_tol = precision_ == MPSD_PREC_FP32 ? 1e-7 : 3e-5;  // relative dE per half-sweep
+  int phase = precision_;  // MPSD_PREC_BF16X3 (2) -> MPSD_PREC_FP32 (1) -> FP64 refinement (0)
+  auto low_tol = [](int p) { return p == MPSD_PREC_FP32 ? 1e-6 : 1e-4; };      // local residual
+  // relative dE per half-sweep below which a phase ends (or when its energy stops decreasing:
+  // the arithmetic's noise floor is reached)
+  auto switch_tol = [](int p) { return p == MPSD_PREC_FP32 ? 1e-9 : 3e-6; };
   bool refine = !lowp;
   bool sweep_all_fp64 = refine;
   bool have_fp64 = false;
@@ -94,10 +98,11 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     Site& site = init.site(pos);
     double* xv = (double*)mpsd_tensor_devptr(site.device());
     {
+      const int first = lowp ? (refine ? MPSD_PREC_FP32 : phase) : MPSD_PREC_FP64;
       TensorMultiplierOffset multi(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset,
-                                   lowp ? precision_ : MPSD_PREC_FP64);
+                                   first);
       if (lowp) {
-        EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol));
+        EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol(first)));
         stats_.matvecs += es.matvecs;
         stats_.matvecs_lowp += es.matvecs;
         stats_.final_residual = es.residual;
@@ -133,9 +138,13 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
       if (sweep_all_fp64) {
         done = dE < convtol_ * std::fabs(energy + offset);
         if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
-      } else if (!refine && dE < switch_tol * std::fabs(val + offset)) {
-        refine = true;  // next half-sweep refines every site in FP64
-        stats_.refine_from = round + 1;
+      } else if (!refine && (dE < switch_tol(phase) * std::fabs(val + offset) || (round > 0 && val > energy))) {
+        if (phase == MPSD_PREC_BF16X3) {
+          phase = MPSD_PREC_FP32;  // next half-sweep in FP32
+        } else {
+          refine = true;  // next half-sweep refines every site in FP64
+          stats_.refine_from = round + 1;
+        }
       }
       energy = val;
       stats_.energies.push_back(val);

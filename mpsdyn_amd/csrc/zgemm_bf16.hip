@@ -55,7 +55,7 @@ __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, in
 }
 
 // One operand tile in LDS: [plane][row][k], planes = re hi, re lo, im hi, im lo.
-struct Planes {
+struct alignas(16) Planes {
   unsigned short v[4][32][PITCH];
 };
 
