@@ -128,6 +128,12 @@ int64_t mpsd_heff_size(const mpsd_heff* h);
  * and, for host buffers, returns after y has been written. */
 int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx,
                     int64_t ldy, int on_device);
+/* Grouped form: y_i = Heff_i x_i for n effective Hamiltonians of any shapes (independent
+ * solves), in one sequence of grouped persistent launches on ctx's stream.  x_i / y_i are
+ * device vectors.  Each result is bit-identical to mpsd_heff_apply of the same handle
+ * alone.  No counterpart in the reference (one solve at a time there). */
+int mpsd_heff_apply_grouped(mpsd_ctx* ctx, int n, mpsd_heff* const* h, const double* const* x,
+                            double* const* y);
 /* Dense Heff (size x size, column-major) into a host buffer.  Replaces
  * TensorMultiplier::getFullTensor (TensorMultiplier.cpp:102-117) / the Offset variant
  * (TensorMultiplierOffset.cpp:21-25), used by Contractor::solveEigs below 801

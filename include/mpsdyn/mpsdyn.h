@@ -17,6 +17,7 @@
 #include <complex>
 #include <cstdint>
 #include <fstream>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -214,6 +215,7 @@ class Multiplier {
   virtual void product(const mwArray& x, mwArray& y);
   virtual void getFullTensor(mwArray& result) = 0;
   virtual mpsd_ctx* ctx() const = 0;
+  virtual void setPrecision(int) {}
 };
 
 // TensorMultiplierOffset (TensorMultiplierOffset.cpp:8-25): Heff + offset, on the device.
@@ -226,7 +228,8 @@ class TensorMultiplierOffset : public Multiplier {
   void product_device(const double* x, double* y) override;
   void getFullTensor(mwArray& result) override;
   mpsd_ctx* ctx() const override { return ctx_; }
-  void setPrecision(int p);
+  void setPrecision(int p) override;
+  mpsd_heff* handle() const { return h_; }
 
  private:
   mpsd_ctx* ctx_;
@@ -278,6 +281,13 @@ struct SolveStats {
   std::vector<double> energies;  // energy at the end of each half-sweep
 };
 
+// Builds the Multiplier a sweep applies at one site (default: TensorMultiplierOffset on the
+// Contractor's context).  A serving runtime substitutes one that batches matvecs of
+// concurrent solves.
+typedef std::function<std::unique_ptr<Multiplier>(mpsd_ctx*, const mpsd_tensor* L, const mpsd_tensor* W,
+                                                  const mpsd_tensor* R, double offset, int precision)>
+    MultiplierFactory;
+
 class Contractor {
  public:
   explicit Contractor(mpsd_ctx* ctx = nullptr);
@@ -290,6 +300,7 @@ class Contractor {
   // Limit on half-sweeps (the reference has none); <= 0 means unlimited.
   void setMaxHalfSweeps(int n) { max_halfsweeps_ = n; }
   void setPrecision(int p) { precision_ = p; }
+  void setMultiplierFactory(MultiplierFactory f) { factory_ = std::move(f); }
   // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).
   void findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset = 0.,
                        int knr = 1, const std::string& tmpfile = "", int freqSv = 200, bool useLM = false);
@@ -307,6 +318,7 @@ class Contractor {
   int precision_ = MPSD_PREC_FP64;
   SolveStats stats_;
   std::unique_ptr<LanczosSolver> eig_;
+  MultiplierFactory factory_;
 };
 
 // ------------------------------------------------------------------------------ models

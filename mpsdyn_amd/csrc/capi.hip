@@ -2,6 +2,7 @@
 // effective-Hamiltonian matvec.  Shapes and their chaining are checked on the host before
 // any launch; every kernel below only ever sees operands whose sizes match its grid.
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -251,6 +252,9 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   for (int i = 0; i < 2; ++i)
     if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
   if (ctx->dots) (void)hipFree(ctx->dots);
+  if (ctx->gtab) (void)hipFree(ctx->gtab);
+  if (ctx->gtab_host) (void)hipHostFree(ctx->gtab_host);
+  if (ctx->gtab_ev) (void)hipEventDestroy(ctx->gtab_ev);
   if (ctx->coef) (void)hipFree(ctx->coef);
   if (ctx->coef_host) (void)hipHostFree(ctx->coef_host);
   for (hipEvent_t e : ctx->coef_ev)
@@ -736,6 +740,207 @@ int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t 
     if (e != hipSuccess) return hip_err(ctx, e, "heff_apply D2H");
   }
   return hip_err(ctx, hipStreamSynchronize(ctx->stream), "heff_apply sync");
+}
+
+}  // extern "C"
+
+// Decide the last-stage split from the problem shape only (the same rule as heff_apply_dev),
+// so a problem takes the same arithmetic path alone and inside a group.
+static bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx) {
+  if (h->d != 2 || h->wr < 2) return false;
+  if (ctx->split3 == 1) return true;
+  if (ctx->split3 == 0) return false;
+  const int64_t tiles = (int64_t)((h->Db + 31) / 32) * ((h->Dbr + 31) / 32);
+  return tiles < 512;
+}
+
+static int upload_tables(mpsd_ctx* ctx, const std::vector<char>& host, void** dev) {
+  if (ctx->gtab_bytes < host.size()) {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->gtab) (void)hipFree(ctx->gtab);
+    if (ctx->gtab_host) (void)hipHostFree(ctx->gtab_host);
+    ctx->gtab = ctx->gtab_host = nullptr;
+    ctx->gtab_bytes = 0;
+    const size_t want = host.size() * 2 + 4096;
+    hipError_t e = hipMalloc(&ctx->gtab, want);
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->gtab_host, want, hipHostMallocDefault);
+    if (e == hipSuccess && !ctx->gtab_ev) e = hipEventCreateWithFlags(&ctx->gtab_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_err(ctx, e, "group table alloc");
+    ctx->gtab_bytes = want;
+  }
+  hipError_t e = hipEventSynchronize(ctx->gtab_ev);  // previous upload done with the staging buffer
+  if (e != hipSuccess) return hip_err(ctx, e, "group table wait");
+  memcpy(ctx->gtab_host, host.data(), host.size());
+  e = hipMemcpyAsync(ctx->gtab, ctx->gtab_host, host.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->gtab_ev, ctx->stream);
+  *dev = ctx->gtab;
+  return hip_err(ctx, e, "group table upload");
+}
+
+template <class T>
+static size_t append(std::vector<char>& buf, const std::vector<T>& v) {
+  const size_t off = (buf.size() + 255) & ~(size_t)255;
+  buf.resize(off + sizeof(T) * v.size());
+  if (!v.empty()) memcpy(buf.data() + off, v.data(), sizeof(T) * v.size());
+  return off;
+}
+
+static int grouped_apply(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const double* const* xs, double* const* ys,
+                         int arith) {
+  // per-problem scratch: T1 (+ partials) in slot 4, T2 in slot 5
+  std::vector<int64_t> o1(n), o2(n), op(n);
+  int64_t t1 = 0, t2 = 0;
+  for (int i = 0; i < n; ++i) {
+    const mpsd_heff* h = hs[i];
+    const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
+    o1[i] = t1;
+    t1 += (int64_t)h->Db * h->wl * h->d * h->Dkr;
+    op[i] = t1;
+    if (stage3_split(h, ctx)) t1 += ny * h->wr;
+    o2[i] = t2;
+    t2 += (int64_t)h->Db * h->d * h->wr * h->Dkr;
+  }
+  int rc = ensure_scratch(ctx, 4, sizeof(double2) * (size_t)std::max<int64_t>(t1, 1));
+  if (!rc) rc = ensure_scratch(ctx, 5, sizeof(double2) * (size_t)std::max<int64_t>(t2, 1));
+  if (rc) return rc;
+  double2* S1 = (double2*)ctx->scratch[4];
+  double2* S2 = (double2*)ctx->scratch[5];
+  const int TM = 32, TN = 32;
+  auto tiles = [&](GroupedProblem& g, int& begin) {
+    g.gm = (g.desc.M + TM - 1) / TM;
+    g.gn = (g.desc.N + TN - 1) / TN;
+    g.gz = (g.desc.NB + g.desc.group - 1) / g.desc.group;
+    g.ntiles = (g.desc.M > 0 && g.desc.N > 0) ? g.gm * g.gn * g.gz : 0;
+    g.tile_begin = begin;
+    begin += g.ntiles;
+  };
+  std::vector<GroupedProblem> st1, st3s, st3n;
+  std::vector<GroupedElem> mx, sm;
+  int b1 = 0, b3s = 0, b3n = 0;
+  int64_t bm = 0, bs = 0;
+  for (int i = 0; i < n; ++i) {
+    mpsd_heff* h = hs[i];
+    const double2* x = (const double2*)xs[i];
+    double2* y = (double2*)ys[i];
+    const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
+    // stage 1, exactly as left_stages
+    GroupedProblem g;
+    ZgemmDesc& d = g.desc;
+    d.A = h->L->ptr; d.B = x; d.C = S1 + o1[i];
+    d.M = h->Db * h->wl; d.N = h->Dkr; d.K = h->Dk; d.NB = h->d;
+    d.sAm = 1; d.sAk = (int64_t)h->Db * h->wl; d.sAb = 0;
+    d.sBk = h->d; d.sBn = (int64_t)h->d * h->Dk; d.sBb = 1;
+    d.sCm = 1; d.sCn = (int64_t)h->Db * h->wl * h->d; d.sCb = (int64_t)h->Db * h->wl;
+    d.a_mfast = 1; d.b_kfast = 1; d.arith = arith;
+    if (resolve_group(d) != (h->d >= 2 ? 2 : 1)) return set_err(ctx, MPSD_ERR_UNSUPPORTED, "grouped: batch layout");
+    tiles(g, b1);
+    st1.push_back(g);
+    // mix
+    GroupedElem e;
+    e.in = S1 + o1[i]; e.out = S2 + o2[i]; e.aux = h->mix;
+    e.Bx = h->Db; e.nin = h->wl * h->d; e.nout = h->d * h->wr; e.Ax = h->Dkr; e.begin = bm;
+    bm += (int64_t)e.Bx * e.Ax;
+    mx.push_back(e);
+    // stage 3
+    GroupedProblem g3;
+    ZgemmDesc& t = g3.desc;
+    t.A = S2 + o2[i]; t.B = h->R->ptr; t.a_mfast = 1; t.b_kfast = 0; t.arith = arith;
+    t.M = h->Db; t.N = h->Dbr; t.sAm = 1; t.sBn = 1; t.sBb = 0;
+    if (stage3_split(h, ctx)) {
+      t.C = S1 + op[i];
+      t.K = h->Dkr; t.NB = h->d * h->wr; t.group = h->d;
+      t.sAk = (int64_t)h->Db * h->d * h->wr; t.sAb = h->Db; t.sAz = (int64_t)h->Db * h->d;
+      t.sBk = (int64_t)h->Dbr * h->wr; t.sBz = h->Dbr;
+      t.sCm = h->d; t.sCn = (int64_t)h->d * h->Db; t.sCb = 1; t.sCz = ny;
+      resolve_group(t);
+      tiles(g3, b3s);
+      st3s.push_back(g3);
+      GroupedElem su;
+      su.in = S1 + op[i]; su.out = y; su.aux = h->offset != 0.0 ? x : nullptr; su.beta = h->offset;
+      su.np = h->wr; su.stride = ny; su.n = ny; su.begin = bs;
+      bs += ny;
+      sm.push_back(su);
+    } else {
+      t.C = y;
+      t.K = h->wr * h->Dkr; t.NB = h->d;
+      t.sAk = (int64_t)h->Db * h->d; t.sAb = h->Db;
+      t.sBk = h->Dbr;
+      t.sCm = h->d; t.sCn = (int64_t)h->d * h->Db; t.sCb = 1;
+      if (h->offset != 0.0) {
+        t.Dadd = x;
+        t.beta = h->offset;
+      }
+      resolve_group(t);
+      tiles(g3, b3n);
+      st3n.push_back(g3);
+    }
+  }
+  std::vector<char> buf;
+  const size_t f1 = append(buf, st1), fm = append(buf, mx), f3s = append(buf, st3s), f3n = append(buf, st3n),
+               fs = append(buf, sm);
+  void* dev = nullptr;
+  rc = upload_tables(ctx, buf, &dev);
+  if (rc) return rc;
+  char* D = (char*)dev;
+  const int persist = 256 * 3;
+  auto grid_of = [&](int64_t work, int per) { return (int)std::max<int64_t>(1, std::min<int64_t>(persist, (work + per - 1) / per)); };
+  const int kind1 = st1.empty() ? 0 : (st1[0].desc.group == 2 ? 1 : 0);
+  hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {
+    return launch_zbrgemm_grouped((const GroupedProblem*)(D + f1), (int)st1.size(), grid_of(b1, 1), TM, TN, 16,
+                                  st1.empty() ? 1 : st1[0].desc.group, kind1, arith, ctx->stream);
+  });
+  if (e == hipSuccess)
+    e = timed(ctx, MPSD_KERNEL_MIX,
+              [&] { return launch_zmix_grouped((const GroupedElem*)(D + fm), (int)mx.size(), grid_of(bm, 256), ctx->stream); });
+  if (e == hipSuccess && !st3s.empty())
+    e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3, [&] {
+      return launch_zbrgemm_grouped((const GroupedProblem*)(D + f3s), (int)st3s.size(), grid_of(b3s, 1), TM, TN, 16,
+                                    2, 2, arith, ctx->stream);
+    });
+  if (e == hipSuccess && !st3n.empty())
+    e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3, [&] {
+      return launch_zbrgemm_grouped((const GroupedProblem*)(D + f3n), (int)st3n.size(), grid_of(b3n, 1), TM, TN, 16,
+                                    st3n[0].desc.group, st3n[0].desc.group == 2 ? 2 : 0, arith, ctx->stream);
+    });
+  if (e == hipSuccess && !sm.empty())
+    e = timed(ctx, MPSD_KERNEL_MIX,
+              [&] { return launch_zsum_grouped((const GroupedElem*)(D + fs), (int)sm.size(), grid_of(bs, 256), ctx->stream); });
+  return hip_err(ctx, e, "grouped matvec");
+}
+
+extern "C" {
+
+int mpsd_heff_apply_grouped(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const double* const* x, double* const* y) {
+  if (!ctx || n < 0 || (n > 0 && (!hs || !x || !y))) return set_err(ctx, MPSD_ERR_ARG, "apply_grouped: bad arguments");
+  if (n == 0) return MPSD_OK;
+  std::vector<mpsd_heff*> g[2];
+  std::vector<const double*> gx[2];
+  std::vector<double*> gy[2];
+  for (int i = 0; i < n; ++i) {
+    mpsd_heff* h = hs[i];
+    if (!h || !x[i] || !y[i]) return set_err(ctx, MPSD_ERR_ARG, "apply_grouped: null handle or vector");
+    int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);
+    if (rc) return rc;
+    if (h->precision == MPSD_PREC_BF16X3 || h->d != 2) {
+      // split-BF16 and d != 2 take the single-problem path on this stream
+      mpsd_ctx* own = h->ctx;
+      h->ctx = ctx;
+      rc = heff_apply_dev(h, (const double2*)x[i], (double2*)y[i]);
+      h->ctx = own;
+      if (rc) return rc;
+      continue;
+    }
+    const int a = arith_of(h->precision);
+    g[a].push_back(h);
+    gx[a].push_back(x[i]);
+    gy[a].push_back(y[i]);
+  }
+  for (int a = 0; a < 2; ++a) {
+    if (g[a].empty()) continue;
+    int rc = grouped_apply(ctx, (int)g[a].size(), g[a].data(), gx[a].data(), gy[a].data(), a);
+    if (rc) return rc;
+  }
+  return MPSD_OK;
 }
 
 int mpsd_heff_full(mpsd_heff* h, double* host_out) {

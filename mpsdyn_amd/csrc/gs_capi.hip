@@ -30,7 +30,12 @@ int fail(mpsd_ctx* ctx, const Error& e) {
   return e.code ? e.code : MPSD_ERR_ARG;
 }
 
-int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mpsd_gs_result* out) {
+}  // namespace
+
+namespace mpsdyn {
+// One solve on ctx with an optional multiplier factory (used by the serving loop).
+int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, const MultiplierFactory* factory,
+                  mpsd_gs_result* out) {
   if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
   if (cfg->L < 2 || cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need L >= 2 and D >= 1");
   memset(out, 0, sizeof(*out));
@@ -39,6 +44,7 @@ int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mps
     MPS mps(cfg->L, cfg->D, 2, ctx);
     mps.setRandomState(cfg->seed);
     Contractor c(ctx);
+    if (factory) c.setMultiplierFactory(*factory);
     c.setConvTol(cfg->convtol > 0 ? cfg->convtol : 1e-5);
     c.setEigTol(cfg->eigtol);
     c.setMaxHalfSweeps(cfg->max_halfsweeps);
@@ -69,6 +75,12 @@ int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mps
     ctx->err = e.what();
     return MPSD_ERR_ARG;
   }
+}
+}  // namespace mpsdyn
+
+namespace {
+int solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mpsd_gs_result* out) {
+  return mpsdyn::gs_solve_with(ctx, cfg, export_path, nullptr, out);
 }
 
 // Non-owning Multiplier over a C-ABI Heff handle.

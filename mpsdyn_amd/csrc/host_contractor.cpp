@@ -99,8 +99,11 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     double* xv = (double*)mpsd_tensor_devptr(site.device());
     {
       const int first = lowp ? (refine ? MPSD_PREC_FP32 : phase) : MPSD_PREC_FP64;
-      TensorMultiplierOffset multi(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset,
-                                   first);
+      std::unique_ptr<Multiplier> mp =
+          factory_ ? factory_(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first)
+                   : std::unique_ptr<Multiplier>(new TensorMultiplierOffset(
+                         ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first));
+      Multiplier& multi = *mp;
       if (lowp) {
         EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol(first)));
         stats_.matvecs += es.matvecs;

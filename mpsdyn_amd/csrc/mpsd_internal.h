@@ -37,6 +37,26 @@ struct ZgemmDesc {
   int b_kfast = 1;  // load order of the B tile: k fastest (1) or n fastest (0)
 };
 
+// One problem of a grouped BRGEMM launch: a resolved descriptor (group strides set) and its
+// tile range in the launch's concatenated tile list.
+struct GroupedProblem {
+  ZgemmDesc desc;
+  int gm = 0, gn = 0, gz = 0;
+  int ntiles = 0, tile_begin = 0;
+  int pad_[3] = {0, 0, 0};
+};
+
+// One problem of a grouped elementwise launch (MPO mix or partial sum).
+struct GroupedElem {
+  const double2* in = nullptr;  // mix: T1; sum: partials P
+  double2* out = nullptr;       // mix: T2; sum: y
+  const double2* aux = nullptr; // mix: mix matrix; sum: x (for beta * x), may be null
+  double beta = 0.0;
+  int Bx = 0, nin = 0, nout = 0, np = 0;
+  int64_t Ax = 0, stride = 0, n = 0;
+  int64_t begin = 0;  // first element index of this problem in the launch
+};
+
 // Tile configuration of a BRGEMM launch (chosen by the tuner or by heuristics).
 struct TileCfg {
   int tm = 0, tn = 0, tk = 16;  // tm = tn = 0: built-in choice
@@ -44,6 +64,19 @@ struct TileCfg {
 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
 hipError_t launch_zbrgemm_bf16x3(const ZgemmDesc& p, int nbw, hipStream_t s);
+
+// Resolve a descriptor's batch group (sets group strides as launch_zbrgemm_f64 would) and
+// return the batches per workgroup it uses.
+int resolve_group(ZgemmDesc& p);
+
+// Grouped persistent BRGEMM over nprob problems whose table lives in device memory; all
+// problems share (tm, tn, tk, nbw, ASH/BSH kind, arith).  kind: 0 = none shared,
+// 1 = A shared by the batch group, 2 = B shared.
+hipError_t launch_zbrgemm_grouped(const GroupedProblem* dev_probs, int nprob, int grid, int tm, int tn, int tk,
+                                  int nbw, int kind, int arith, hipStream_t s);
+// Grouped MPO mix / partial sums over problem tables in device memory.
+hipError_t launch_zmix_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s);
+hipError_t launch_zsum_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s);
 
 // y[i] = sum_{p < np} P[p * stride + i] (+ beta * x[i]) in a fixed order (x may be null).
 hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const double2* x, double beta, double2* y,
@@ -66,10 +99,17 @@ struct mpsd_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
-  // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step.
-  static constexpr int kScratch = 4;
-  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr};
-  size_t scratch_bytes[kScratch] = {0, 0, 0, 0};
+  // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step;
+  // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec.
+  static constexpr int kScratch = 6;
+  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0};
+  // Problem tables of grouped launches: device buffer + pinned host staging, reused after
+  // the event of the previous upload has completed.
+  void* gtab = nullptr;
+  void* gtab_host = nullptr;
+  size_t gtab_bytes = 0;
+  hipEvent_t gtab_ev = nullptr;
   // Eigensolver support: rocBLAS handle (for rocSOLVER), reduction partials, coefficient ring.
   void* blas = nullptr;
   void* dots = nullptr;

@@ -75,16 +75,27 @@ __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, in
   tn_i = rr / gsize;
 }
 
+// LDS operand tiles of one workgroup.
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
+struct Smem {
+  typename Pol<ARITH>::T2 As[ASH ? 1 : NBW][TK][TM];
+  typename Pol<ARITH>::T2 Bs[BSH ? 1 : NBW][TK][TN];
+};
+
+// One TM x TN output tile (for NBW batches of group z) of the BRGEMM described by p.  The
+// order of every accumulation depends only on p and the tile, never on which workgroup or
+// launch computes it, so a problem gives bit-identical results alone and inside a group.
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
+__device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, int z,
+                                            Smem<TM, TN, TK, NBW, ASH, BSH, ARITH>& sm) {
   typedef Pol<ARITH> P;
   typedef typename P::T T;
   typedef typename P::T2 T2;
   typedef typename P::Acc Acc;
   constexpr int NA = ASH ? 1 : NBW;
   constexpr int NB_ = BSH ? 1 : NBW;
-  __shared__ T2 As[NA][TK][TM];
-  __shared__ T2 Bs[NB_][TK][TN];
+  auto& As = sm.As;
+  auto& Bs = sm.Bs;
 
   constexpr int WTM = TM / 2, WTN = TN / 2;  // 2x2 waves
   constexpr int BM = WTM / 16, BN = WTN / 16;
@@ -96,10 +107,6 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, in
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  int tm_i, tn_i, z;
-  tile_of(gm, gn, tm_i, tn_i, z);
-  const int m0 = tm_i * TM;
-  const int n0 = tn_i * TN;
   const int nbw = min(NBW, p.NB - z * NBW);
 
   Acc accr[NBW][BM][BN];
@@ -263,13 +270,39 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, in
             double2 v{(double)accr[j][i][q][rr], (double)acci[j][i][q][rr]};
             if (db) {
               const double2 x = db[off];
-              v.x += p.beta * x.x;
-              v.y += p.beta * x.y;
+              v.x = fma(p.beta, x.x, v.x);
+              v.y = fma(p.beta, x.y, v.y);
             }
             cb[off] = v;
           }
         }
       }
+  }
+}
+
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
+__global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
+  __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
+  int tm_i, tn_i, z;
+  tile_of(gm, gn, tm_i, tn_i, z);
+  brgemm_tile<TM, TN, TK, NBW, ASH, BSH, ARITH>(p, tm_i * TM, tn_i * TN, z, sm);
+}
+
+// Grouped persistent form: the workgroups of one launch stride over the concatenated tile
+// lists of all problems (problem q owns tiles [tile_begin[q], tile_begin[q+1]) ).
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
+__global__ void __launch_bounds__(kThreads) zbrgemm_grouped_kernel(const GroupedProblem* __restrict__ probs, int nprob) {
+  __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
+  const int total = probs[nprob - 1].tile_begin + probs[nprob - 1].ntiles;
+  int q = 0;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    while (t >= probs[q].tile_begin + probs[q].ntiles) ++q;  // tiles are visited in increasing order
+    const GroupedProblem& gp = probs[q];
+    const int local = t - gp.tile_begin;
+    const int per_z = gp.gm * gp.gn;
+    const int z = local / per_z;
+    const int r = local - z * per_z;
+    brgemm_tile<TM, TN, TK, NBW, ASH, BSH, ARITH>(gp.desc, (r % gp.gm) * TM, (r / gp.gm) * TN, z, sm);
   }
 }
 
@@ -307,20 +340,51 @@ hipError_t launch_tk(const ZgemmDesc& p, TileCfg cfg, int nbw, hipStream_t s) {
 
 }  // namespace
 
+int resolve_group(ZgemmDesc& p) {
+  // batches per workgroup: an explicit group, else two when one operand is shared by all
+  if (p.group > 0) return (p.group > 2 || p.NB % p.group != 0) ? -1 : p.group;
+  const int nbw = (p.NB >= 2 && (p.sAb == 0 || p.sBb == 0)) ? 2 : 1;
+  p.sAz = nbw * p.sAb;
+  p.sBz = nbw * p.sBb;
+  p.sCz = nbw * p.sCb;
+  p.group = nbw;
+  return nbw;
+}
+
+namespace {
+template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
+hipError_t grouped_a(const GroupedProblem* d, int n, int grid, int arith, hipStream_t s) {
+  if (arith == 1)
+    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 1>), dim3(grid), dim3(kThreads), 0, s, d, n);
+  else
+    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 0>), dim3(grid), dim3(kThreads), 0, s, d, n);
+  return hipGetLastError();
+}
+template <int TM, int TN, int TK>
+hipError_t grouped_k(const GroupedProblem* d, int n, int grid, int nbw, int kind, int arith, hipStream_t s) {
+  if (nbw == 2 && kind == 1) return grouped_a<TM, TN, TK, 2, true, false>(d, n, grid, arith, s);
+  if (nbw == 2 && kind == 2) return grouped_a<TM, TN, TK, 2, false, true>(d, n, grid, arith, s);
+  if (nbw == 2) return grouped_a<TM, TN, TK, 2, false, false>(d, n, grid, arith, s);
+  return grouped_a<TM, TN, TK, 1, false, false>(d, n, grid, arith, s);
+}
+}  // namespace
+
+hipError_t launch_zbrgemm_grouped(const GroupedProblem* d, int n, int grid, int tm, int tn, int tk, int nbw, int kind,
+                                  int arith, hipStream_t s) {
+  if (n <= 0 || grid <= 0) return hipSuccess;
+  if (arith != 0 && arith != 1) return hipErrorInvalidValue;
+  if (tm == 32 && tn == 32) return tk == 32 ? grouped_k<32, 32, 32>(d, n, grid, nbw, kind, arith, s)
+                                            : grouped_k<32, 32, 16>(d, n, grid, nbw, kind, arith, s);
+  if (tm == 64 && tn == 32) return grouped_k<64, 32, 16>(d, n, grid, nbw, kind, arith, s);
+  if (tm == 32 && tn == 64) return grouped_k<32, 64, 16>(d, n, grid, nbw, kind, arith, s);
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& pin, TileCfg cfg, hipStream_t s) {
   if (pin.M <= 0 || pin.N <= 0 || pin.NB <= 0) return hipSuccess;
   ZgemmDesc p = pin;
-  // batches per workgroup: an explicit group, else two when one operand is shared by all
-  int nbw;
-  if (p.group > 0) {
-    nbw = p.group;
-    if (nbw > 2 || p.NB % nbw != 0) return hipErrorInvalidValue;
-  } else {
-    nbw = (p.NB >= 2 && (p.sAb == 0 || p.sBb == 0)) ? 2 : 1;
-    p.sAz = nbw * p.sAb;
-    p.sBz = nbw * p.sBb;
-    p.sCz = nbw * p.sCb;
-  }
+  const int nbw = resolve_group(p);
+  if (nbw < 0) return hipErrorInvalidValue;
   if (p.arith == 2) return launch_zbrgemm_bf16x3(p, nbw, s);  // fixed 32x32 tiles
   if (cfg.tm == 0 || cfg.tn == 0) {
     // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.
@@ -362,8 +426,8 @@ __global__ void __launch_bounds__(kThreads) zmix_f64_kernel(const double2* __res
     for (int o = 0; o < kMixMax; ++o) {
       if (o < nout) {
         const double2 w = mix[o + ldmix * i];
-        acc[o].x += w.x * v.x - w.y * v.y;
-        acc[o].y += w.x * v.y + w.y * v.x;
+        acc[o].x = fma(w.x, v.x, fma(-w.y, v.y, acc[o].x));
+        acc[o].y = fma(w.x, v.y, fma(w.y, v.x, acc[o].y));
       }
     }
   }
@@ -418,8 +482,8 @@ __global__ void __launch_bounds__(kThreads) zsum_parts_kernel(const double2* __r
   }
   if (x) {
     const double2 v = x[i];
-    acc.x += beta * v.x;
-    acc.y += beta * v.y;
+    acc.x = fma(beta, v.x, acc.x);
+    acc.y = fma(beta, v.y, acc.y);
   }
   y[i] = acc;
 }
@@ -430,6 +494,83 @@ hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(zsum_parts_kernel, dim3((unsigned)((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, P, np,
                      stride, x, beta, y, n);
+  return hipGetLastError();
+}
+
+namespace {
+__device__ __forceinline__ int find_elem(const GroupedElem* probs, int nprob, int64_t i, int q) {
+  while (q + 1 < nprob && i >= probs[q + 1].begin) ++q;
+  return q;
+}
+
+// Persistent grid-stride form of zmix_f64 over several problems (element = one (bx, ax)).
+__global__ void __launch_bounds__(kThreads) zmix_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob) {
+  const GroupedElem& last = probs[nprob - 1];
+  const int64_t total = last.begin + (int64_t)last.Bx * last.Ax;
+  int q = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads) {
+    q = find_elem(probs, nprob, i, q);
+    const GroupedElem& g = probs[q];
+    const int64_t li = i - g.begin;
+    const int bx = (int)(li % g.Bx);
+    const int64_t ax = li / g.Bx;
+    const double2* src = g.in + bx + (int64_t)g.Bx * g.nin * ax;
+    double2* dst = g.out + bx + (int64_t)g.Bx * g.nout * ax;
+    for (int o0 = 0; o0 < g.nout; o0 += kMixMax) {
+      double2 acc[kMixMax];
+#pragma unroll
+      for (int o = 0; o < kMixMax; ++o) acc[o] = double2{0.0, 0.0};
+      for (int k = 0; k < g.nin; ++k) {
+        const double2 v = src[(int64_t)g.Bx * k];
+#pragma unroll
+        for (int o = 0; o < kMixMax; ++o) {
+          if (o0 + o < g.nout) {
+            const double2 w = g.aux[o0 + o + g.nout * k];
+            acc[o].x = fma(w.x, v.x, fma(-w.y, v.y, acc[o].x));
+            acc[o].y = fma(w.x, v.y, fma(w.y, v.x, acc[o].y));
+          }
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kMixMax; ++o)
+        if (o0 + o < g.nout) dst[(int64_t)g.Bx * (o0 + o)] = acc[o];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) zsum_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob) {
+  const GroupedElem& last = probs[nprob - 1];
+  const int64_t total = last.begin + last.n;
+  int q = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < total; i += (int64_t)gridDim.x * kThreads) {
+    q = find_elem(probs, nprob, i, q);
+    const GroupedElem& g = probs[q];
+    const int64_t li = i - g.begin;
+    double2 acc = g.in[li];
+    for (int k = 1; k < g.np; ++k) {
+      const double2 v = g.in[(int64_t)k * g.stride + li];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    if (g.aux) {
+      const double2 v = g.aux[li];
+      acc.x = fma(g.beta, v.x, acc.x);
+      acc.y = fma(g.beta, v.y, acc.y);
+    }
+    g.out[li] = acc;
+  }
+}
+}  // namespace
+
+hipError_t launch_zmix_grouped(const GroupedElem* d, int n, int grid, hipStream_t s) {
+  if (n <= 0 || grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zmix_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zsum_grouped(const GroupedElem* d, int n, int grid, hipStream_t s) {
+  if (n <= 0 || grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zsum_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n);
   return hipGetLastError();
 }
 

@@ -97,6 +97,15 @@ def _optional(lib):
         "mpsd_gs_solve_export": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
+        "mpsd_heff_apply_grouped": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "mpsd_gs_footprint": (I64, [C.POINTER(GsConfig)]),
+        "mpsd_server_create": (I, [C.POINTER(ServeConfig), C.POINTER(P)]),
+        "mpsd_server_destroy": (I, [P]),
+        "mpsd_server_last_error": (C.c_char_p, [P]),
+        "mpsd_server_submit": (I, [P, C.POINTER(GsConfig), C.POINTER(I)]),
+        "mpsd_server_wait": (I, [P, I, C.POINTER(ServeResult)]),
+        "mpsd_server_run_trace": (I, [P, I, C.POINTER(GsConfig), C.POINTER(D), C.POINTER(ServeResult)]),
+        "mpsd_server_stats": (I, [P, C.POINTER(ServeStats)]),
         "mpsd_ctx_set_timing": (I, [P, I]),
         "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
         "mpsd_ctx_set_tile": (I, [P, I, I]),
@@ -126,6 +135,69 @@ class GsResult(C.Structure):
                 ("residual", C.c_double), ("device_bytes", C.c_int64), ("matvecs_lowp", C.c_int64),
                 ("refine_from", C.c_int), ("n_history", C.c_int),
                 ("history", C.c_double * 256)]
+
+
+class ServeConfig(C.Structure):
+    """mpsd_serve_config (include/mpsd_serve.h)."""
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("hbm_budget", C.c_int64), ("batching", C.c_int),
+                ("window_us", C.c_int), ("graph", C.c_int)]
+
+
+class ServeResult(C.Structure):
+    """mpsd_serve_result (include/mpsd_serve.h)."""
+    _fields_ = [("id", C.c_int), ("status", C.c_int), ("energy", C.c_double), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("finish_s", C.c_double), ("halfsweeps", C.c_int),
+                ("matvecs", C.c_int64), ("matvecs_lowp", C.c_int64), ("footprint_bytes", C.c_int64)]
+
+
+class ServeStats(C.Structure):
+    """mpsd_serve_stats (include/mpsd_serve.h)."""
+    _fields_ = [("solves", C.c_int64), ("batches", C.c_int64), ("batched_matvecs", C.c_int64),
+                ("max_batch", C.c_int64), ("peak_admitted_bytes", C.c_int64), ("peak_running", C.c_int),
+                ("graph_replays", C.c_int64)]
+
+
+class Server:
+    """The serving loop (mpsd_server): concurrent solves admitted by HBM footprint."""
+
+    def __init__(self, slots=8, hbm_budget=0, batching=True, window_us=200, graph=False, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        cfg = ServeConfig(device, slots, hbm_budget, 1 if batching else 0, window_us, 1 if graph else 0)
+        rc = self.lib.mpsd_server_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise MpsdError(rc, "mpsd_server_create failed")
+        self._h = h
+
+    def run_trace(self, cfgs, arrivals):
+        n = len(cfgs)
+        arr = (GsConfig * n)(*cfgs)
+        t = (C.c_double * n)(*arrivals)
+        out = (ServeResult * n)()
+        rc = self.lib.mpsd_server_run_trace(self._h, n, arr, t, out)
+        if rc != 0:
+            raise MpsdError(rc, self.lib.mpsd_server_last_error(self._h).decode())
+        return list(out)
+
+    def stats(self):
+        st = ServeStats()
+        self.lib.mpsd_server_stats(self._h, C.byref(st))
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.mpsd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gs_footprint(cfg):
+    return int(load().mpsd_gs_footprint(C.byref(cfg)))
 
 
 def gs_config(model, L, D, p, convtol=1e-10, eigtol=0.0, max_halfsweeps=0, precision=PREC_FP64,
@@ -305,6 +377,15 @@ def env_left(ctx, envL, ket, W, bra, out):
 def env_right(ctx, envR, ket, W, bra, out):
     ctx.check(ctx.lib.mpsd_env_right(ctx.handle, envR.handle, ket.handle, W.handle, bra.handle, out.handle))
     return out
+
+
+def heff_apply_grouped(ctx, heffs, xs, ys):
+    """y_i = Heff_i x_i for device tensors xs/ys, in one grouped launch sequence."""
+    n = len(heffs)
+    H = (C.c_void_p * n)(*[h._h.value for h in heffs])
+    X = (C.c_void_p * n)(*[x.devptr() for x in xs])
+    Y = (C.c_void_p * n)(*[y.devptr() for y in ys])
+    ctx.check(ctx.lib.mpsd_heff_apply_grouped(ctx.handle, n, H, X, Y))
 
 
 class Heff:
