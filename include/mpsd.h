@@ -153,11 +153,25 @@ int mpsd_vec_dots(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const doub
 /* w[i] += sum_j V[j*ldv + i] * c[j], j < m <= 1024, c a host array of m complex. */
 int mpsd_vec_gemv(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_host, double* w,
                   int64_t n);
+/* Device-resident forms (no host synchronisation): out_dev = V^H w as m complex numbers in
+ * device memory; w += sign * V c_dev. */
+int mpsd_vec_dots_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* w, int64_t n,
+                      double* out_dev);
+int mpsd_vec_gemv_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_dev, double sign,
+                      double* w, int64_t n);
+/* Lanczos step j on the device: from h1 = V_{0..j}^H w (first Gram-Schmidt pass) and
+ * h2 = [V_{0..j}, w]^H w (after it) store alpha_j = Re(h1[j] + h2[j]) and
+ * beta_j = sqrt(|w|^2 - sum |h2[i]|^2) at ab_dev[2j], ab_dev[2j+1], and scale w by 1/beta_j
+ * (w = 0 when beta_j underflows). */
+int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev, int j, double* ab_dev,
+                      double* w, int64_t n);
 /* y = a*x + b*y with complex scalars a, b given as {re, im}. */
 int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n);
 /* y = x ; x = 0 (plain stores: a NaN in the old y does not survive). */
 int mpsd_vec_copy(mpsd_ctx* ctx, const double* x, double* y, int64_t n);
 int mpsd_vec_zero(mpsd_ctx* ctx, double* x, int64_t n);
+/* Copy `bytes` from device memory to host after the context's queued work (synchronises). */
+int mpsd_copy_to_host(mpsd_ctx* ctx, const void* dev, void* host, int64_t bytes);
 /* out = sum |x_i|^2 (synchronises the stream). */
 int mpsd_vec_norm2(mpsd_ctx* ctx, const double* x, int64_t n, double* out);
 

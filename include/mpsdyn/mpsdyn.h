@@ -259,8 +259,9 @@ class LanczosSolver {
 
  private:
   mpsd_ctx* ctx_;
-  DevTensor V_;
+  DevTensor V_, coef_;
   int64_t cap_ = 0;
+  int coef_cap_ = 0;
 };
 
 // Lowest eigenpair of a symmetric tridiagonal matrix (diag a[0..m), offdiag b[0..m-1)):

@@ -137,19 +137,31 @@ double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b
 LanczosSolver::~LanczosSolver() {}
 
 EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_krylov, int max_restarts) {
+  // Restarted Lanczos with two-pass full reorthogonalisation.  The iterations run on the
+  // device in chunks (first 6, then 4): Gram-Schmidt coefficients, alpha and beta stay in
+  // device memory, and the host reads the chunk's alphas/betas once to solve the tridiagonal
+  // problem.  The stopping step is the first converged one inside the chunk, so the result
+  // does not depend on the chunking (later iterations of that chunk are discarded).
   EigStats st;
   const int64_t n = H.getSize();
-  const int m = (int)std::max<int64_t>(2, std::min<int64_t>(max_krylov, n));
-  if ((m + 1) * n > cap_) {
+  const int m = (int)std::max<int64_t>(1, std::min<int64_t>(max_krylov, n));
+  if ((int64_t)(m + 1) * n > cap_) {
     V_ = DevTensor(ctx_, std::vector<int>{(int)n, m + 1});
     cap_ = (int64_t)(m + 1) * n;
+  }
+  if (coef_cap_ < m) {
+    coef_ = DevTensor(ctx_, std::vector<int>{4 * (m + 2)});
+    coef_cap_ = m;
   }
   double* V = V_.ptr();
   const int64_t ldv = n;
   auto vec = [&](int j) { return V + 2 * (int64_t)j * ldv; };
-  const double zero[2] = {0.0, 0.0}, one[2] = {1.0, 0.0};
+  double* h1 = coef_.ptr();               // m + 1 complex
+  double* h2 = h1 + 2 * (m + 2);          // m + 2 complex
+  double* ab = h2 + 2 * (m + 2);          // 2 m reals (+ 2 for the start norm)
+  const double zero[2] = {0.0, 0.0};
+  std::vector<double> abh(2 * m + 2), s;
 
-  // start vector
   double nx = 0.0;
   check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
   if (!(nx > 0.0) || !std::isfinite(nx)) {
@@ -157,58 +169,50 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
     check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
     check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
   }
-  std::vector<double> alpha, beta, s, h;
   for (int restart = 0; restart <= max_restarts; ++restart) {
     st.restarts = restart;
     const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
     check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, vec(0), n), "normalise");
-    alpha.clear();
-    beta.clear();
-    double theta = 0.0, res = 0.0;
-    int j = 0;
+    int K = 0;  // size of the Krylov basis used for the Ritz vector
     bool stop = false;
-    for (; j < m; ++j) {
-      double* w = vec(j + 1);
-      H.product_device(vec(j), w);
-      ++st.matvecs;
-      // first Gram-Schmidt pass: h = V^H w, w -= V h
-      h.assign(2 * (j + 1), 0.0);
-      check(ctx_, mpsd_vec_dots(ctx_, V, ldv, j + 1, w, n, h.data()), "dots");
-      double a = h[2 * j];
-      for (double& v : h) v = -v;
-      check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, j + 1, h.data(), w, n), "gemv");
-      // second pass (DGKS) fused with the norm of w: dots over [V_0..V_j, w]
-      std::vector<double> h2(2 * (j + 2), 0.0);
-      check(ctx_, mpsd_vec_dots(ctx_, V, ldv, j + 2, w, n, h2.data()), "dots2");
-      double nw2 = h2[2 * (j + 1)];
-      double corr2 = 0.0;
-      for (int i = 0; i <= j; ++i) corr2 += h2[2 * i] * h2[2 * i] + h2[2 * i + 1] * h2[2 * i + 1];
-      a += h2[2 * j];
-      h2.resize(2 * (j + 1));
-      for (double& v : h2) v = -v;
-      check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, j + 1, h2.data(), w, n), "gemv2");
-      const double b = std::sqrt(std::max(nw2 - corr2, 0.0));
-      alpha.push_back(a);
-      theta = tridiag_lowest(alpha, beta, s);
-      res = b * std::fabs(s[j]);
-      const double scale = std::max(1.0, std::fabs(theta));
-      if (!std::isfinite(theta) || !std::isfinite(b))
-        throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
-      if (res <= tol * scale || b <= 1e-14 * scale || j + 1 >= n) {
-        stop = res <= tol * scale || b <= 1e-14 * scale || j + 1 >= n;
-        ++j;
-        break;
+    double theta = 0.0, res = 0.0;
+    for (int j = 0; j < m && !stop;) {
+      const int jend = std::min(m, j + (j == 0 ? 6 : 4));
+      for (int jj = j; jj < jend; ++jj) {
+        double* w = vec(jj + 1);
+        H.product_device(vec(jj), w);
+        ++st.matvecs;
+        check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 1, w, n, h1), "dots");
+        check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h1, -1.0, w, n), "gemv");
+        check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 2, w, n, h2), "dots2");
+        check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h2, -1.0, w, n), "gemv2");
+        check(ctx_, mpsd_lanczos_coef(ctx_, h1, h2, jj, ab, w, n), "coef");
       }
-      beta.push_back(b);
-      const double ib[2] = {1.0 / b, 0.0};
-      check(ctx_, mpsd_vec_axpby(ctx_, ib, w, zero, w, n), "scale");
+      check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * 2 * jend), "coef download");
+      for (int k = j; k < jend; ++k) {
+        const double a = abh[2 * k], b = abh[2 * k + 1];
+        if (!std::isfinite(a) || !std::isfinite(b))
+          throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
+        std::vector<double> al(k + 1), be(k);
+        for (int i = 0; i <= k; ++i) al[i] = abh[2 * i];
+        for (int i = 0; i < k; ++i) be[i] = abh[2 * i + 1];
+        theta = tridiag_lowest(al, be, s);
+        res = b * std::fabs(s[k]);
+        const double scale = std::max(1.0, std::fabs(theta));
+        if (res <= tol * scale || b <= 1e-14 * scale || k + 1 >= n) {
+          stop = true;
+          K = k + 1;
+          break;
+        }
+      }
+      j = jend;
+      if (!stop && j >= m) K = m;
     }
     // Ritz vector x = V s
-    const int k = (int)alpha.size();
-    std::vector<double> c(2 * k, 0.0);
-    for (int i = 0; i < k; ++i) c[2 * i] = s[i];
+    std::vector<double> c(2 * K, 0.0);
+    for (int i = 0; i < K; ++i) c[2 * i] = s[i];
     check(ctx_, mpsd_vec_zero(ctx_, x, n), "zero");
-    check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, k, c.data(), x, n), "ritz");
+    check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, K, c.data(), x, n), "ritz");
     st.value = theta;
     st.residual = res;
     check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
@@ -217,7 +221,6 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
       break;
     }
   }
-  // unit norm on exit
   const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
   check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, x, n), "normalise");
   return st;

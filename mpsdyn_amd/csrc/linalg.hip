@@ -11,6 +11,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -72,20 +73,45 @@ __global__ void __launch_bounds__(kDotBlocks) dots_final_kernel(const double2* _
   if (threadIdx.x == 0) out[j] = red[0];
 }
 
-// w[i] += sum_j V_j[i] * c[j]   (c on the device, m <= 1024)
+// w[i] += sign * sum_j V_j[i] * c[j]   (c on the device, m <= 1024)
 __global__ void __launch_bounds__(kT) gemv_update_kernel(const double2* __restrict__ V, int64_t ldv, int m,
-                                                         const double2* __restrict__ c,
+                                                         const double2* __restrict__ c, double sign,
                                                          double2* __restrict__ w, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
-  double2 acc = w[i];
+  double2 acc{0.0, 0.0};
   for (int j = 0; j < m; ++j) {
     const double2 v = V[(int64_t)j * ldv + i];
     const double2 cj = c[j];
-    acc.x += v.x * cj.x - v.y * cj.y;
-    acc.y += v.x * cj.y + v.y * cj.x;
+    acc.x = fma(v.x, cj.x, fma(-v.y, cj.y, acc.x));
+    acc.y = fma(v.x, cj.y, fma(v.y, cj.x, acc.y));
   }
-  w[i] = acc;
+  const double2 wi = w[i];
+  w[i] = double2{fma(sign, acc.x, wi.x), fma(sign, acc.y, wi.y)};
+}
+
+// Lanczos coefficients of step j from the two Gram-Schmidt passes (h1 = V^H w before the
+// first pass, h2 = [V_0..V_j, w]^H w after it): alpha_j = Re(h1[j] + h2[j]),
+// beta_j = sqrt(|w|^2 - sum_i |h2[i]|^2) (the norm after the second pass), then
+// w /= beta_j (w = 0 when beta_j underflows: the Krylov space is exhausted).  Every block
+// recomputes the scalars in the same order; block 0 stores them.
+__global__ void __launch_bounds__(kT) lanczos_coef_kernel(const double2* __restrict__ h1,
+                                                          const double2* __restrict__ h2, int j,
+                                                          double* __restrict__ ab, double2* __restrict__ w,
+                                                          int64_t n) {
+  const double alpha = h1[j].x + h2[j].x;
+  double corr = 0.0;
+  for (int i = 0; i <= j; ++i) corr = fma(h2[i].x, h2[i].x, fma(h2[i].y, h2[i].y, corr));
+  const double beta = sqrt(fmax(h2[j + 1].x - corr, 0.0));
+  const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double2 v = w[i];
+    w[i] = double2{v.x * inv, v.y * inv};
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ab[2 * j] = alpha;
+    ab[2 * j + 1] = beta;
+  }
 }
 
 // y = a x + b y
@@ -192,6 +218,25 @@ static int solver_err(mpsd_ctx* ctx, rocblas_status st, const char* what) {
 
 }  // namespace mpsd
 
+namespace mpsd {
+static int ensure_dots(mpsd_ctx* ctx, int m) {
+  const size_t need = sizeof(double2) * ((size_t)m * kDotBlocks + m);
+  if (ctx->dots_bytes >= need) return MPSD_OK;
+  if (ctx->dots) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->dots);
+  }
+  ctx->dots = nullptr;
+  ctx->dots_bytes = 0;
+  const size_t want = std::max(need, sizeof(double2) * (64 * kDotBlocks + 64));
+  hipError_t e = hipMalloc(&ctx->dots, want);
+  if (e != hipSuccess) return hip_err(ctx, e, "dots alloc");
+  ctx->dots_bytes = want;
+  return MPSD_OK;
+}
+
+}  // namespace mpsd
+
 using namespace mpsd;
 
 extern "C" {
@@ -200,15 +245,8 @@ int mpsd_vec_dots(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const doub
                   double* out_host) {
   if (!ctx || !V || !w || !out_host || m < 0 || n < 0 || (m > 1 && ldv < n)) return MPSD_ERR_ARG;
   if (m == 0) return MPSD_OK;
-  const size_t need = sizeof(double2) * ((size_t)m * kDotBlocks + m);
-  if (ctx->dots_bytes < need) {
-    if (ctx->dots) (void)hipFree(ctx->dots);
-    ctx->dots = nullptr;
-    ctx->dots_bytes = 0;
-    hipError_t e = hipMalloc(&ctx->dots, need);
-    if (e != hipSuccess) return hip_err(ctx, e, "dots alloc");
-    ctx->dots_bytes = need;
-  }
+  int rc = ensure_dots(ctx, m);
+  if (rc) return rc;
   double2* partial = (double2*)ctx->dots;
   double2* out = partial + (size_t)m * kDotBlocks;
   hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks, m), dim3(kT), 0, ctx->stream,
@@ -240,9 +278,40 @@ int mpsd_vec_gemv(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const doub
   e = hipMemcpyAsync(c, ch, sizeof(double2) * m, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) return hip_err(ctx, e, "vec_gemv H2D");
   hipLaunchKernelGGL(gemv_update_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)V, ldv, m,
-                     c, (double2*)w, n);
+                     c, 1.0, (double2*)w, n);
   e = hipEventRecord(ctx->coef_ev[slot], ctx->stream);
   return hip_err(ctx, e, "vec_gemv");
+}
+
+int mpsd_vec_dots_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* w, int64_t n,
+                      double* out_dev) {
+  if (!ctx || !V || !w || !out_dev || m < 0 || n < 0 || (m > 1 && ldv < n)) return MPSD_ERR_ARG;
+  if (m == 0) return MPSD_OK;
+  int rc = ensure_dots(ctx, m);
+  if (rc) return rc;
+  double2* partial = (double2*)ctx->dots;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks, m), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)w, n, partial);
+  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, (double2*)out_dev);
+  return hip_err(ctx, hipGetLastError(), "vec_dots_dev");
+}
+
+int mpsd_vec_gemv_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_dev, double sign,
+                      double* w, int64_t n) {
+  if (!ctx || !V || !w || !c_dev || m < 0 || n < 0) return MPSD_ERR_ARG;
+  if (m == 0 || n == 0) return MPSD_OK;
+  hipLaunchKernelGGL(gemv_update_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)V, ldv, m,
+                     (const double2*)c_dev, sign, (double2*)w, n);
+  return hip_err(ctx, hipGetLastError(), "vec_gemv_dev");
+}
+
+int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev, int j, double* ab_dev, double* w,
+                      int64_t n) {
+  if (!ctx || !h1_dev || !h2_dev || !ab_dev || !w || j < 0 || n < 0) return MPSD_ERR_ARG;
+  const unsigned blocks = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kT - 1) / kT), 1024);
+  hipLaunchKernelGGL(lanczos_coef_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)h1_dev,
+                     (const double2*)h2_dev, j, ab_dev, (double2*)w, n);
+  return hip_err(ctx, hipGetLastError(), "lanczos_coef");
 }
 
 int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n) {
@@ -251,6 +320,14 @@ int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double
   hipLaunchKernelGGL(axpby_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, double2{a[0], a[1]},
                      (const double2*)x, double2{b[0], b[1]}, (double2*)y, n);
   return hip_err(ctx, hipGetLastError(), "vec_axpby");
+}
+
+int mpsd_copy_to_host(mpsd_ctx* ctx, const void* dev, void* host, int64_t bytes) {
+  if (!ctx || !dev || !host || bytes < 0) return MPSD_ERR_ARG;
+  if (bytes == 0) return MPSD_OK;
+  hipError_t e = hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return hip_err(ctx, e, "copy_to_host");
 }
 
 int mpsd_vec_copy(mpsd_ctx* ctx, const double* x, double* y, int64_t n) {
