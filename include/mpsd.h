@@ -73,7 +73,9 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
 /* Other launch options of the context (used by the tile tuner). */
 #define MPSD_OPT_TILE_K 1       /* K depth of a BRGEMM LDS stage: 16 (default) or 32 */
 #define MPSD_OPT_SPLIT_STAGE3 2 /* matvec last stage split over the MPO bond: -1 auto, 0 off, 1 on */
+#define MPSD_OPT_GRAPH 3        /* eigensolver steps as replayed hipGraphs: 1 (default) or 0 */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
+int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option);
 
 /* ---- device tensors -------------------------------------------------------------------
  * HBM-resident counterpart of an mwArray (src/wrapper/mwArray.h:23-498): rank <= 8,
@@ -188,6 +190,26 @@ int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out);
 int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R);
 /* site[s, a, :] := site[s, a, :] . L  -- Site::reduceDimL (Site.cpp:263-283). */
 int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L);
+
+/* ---- device-driven Lanczos steps (replayed hipGraph) ----------------------------------------
+ * The Lanczos loop of the eigensolver with every step's kernels driven from a device
+ * control block, so k steps are one captured hipGraph replayed for every chunk
+ * of every site with the same launch shape.  Requires an Heff with d = 2 in the FP64 or
+ * FP32 form.  Stands in for the
+ * per-iteration host work of PRIMME/ARPACK (src/wrapper/eigs.cpp:71-422). */
+typedef struct mpsd_lanczos mpsd_lanczos;
+int mpsd_lanczos_create(mpsd_ctx* ctx, int max_krylov, mpsd_lanczos** out);
+int mpsd_lanczos_destroy(mpsd_lanczos* lz);
+/* Bind Heff h and start the basis with V_0 = x / xnorm (x a device vector of the Heff size). */
+int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x, double xnorm);
+/* Run k (1..8) more steps as one graph replay (graphs are captured once per launch shape and
+ * cached); ab_host receives alpha_i, beta_i (interleaved) of all steps so far, *j_end their
+ * number. */
+int mpsd_lanczos_steps(mpsd_lanczos* lz, int k, double* ab_host, int* j_end);
+/* x = sum_{i<K} s_host[i] V_i (Ritz vector from real coefficients). */
+int mpsd_lanczos_ritz(mpsd_lanczos* lz, const double* s_host, int K, double* x);
+/* Number of graph replays so far. */
+int64_t mpsd_lanczos_replays(const mpsd_lanczos* lz);
 
 #ifdef __cplusplus
 }

@@ -216,6 +216,9 @@ class Multiplier {
   virtual void getFullTensor(mwArray& result) = 0;
   virtual mpsd_ctx* ctx() const = 0;
   virtual void setPrecision(int) {}
+  // The C-ABI handle when the product is a plain device Heff (enables the graph-driven
+  // eigensolver path), else null.
+  virtual mpsd_heff* heff() const { return nullptr; }
 };
 
 // TensorMultiplierOffset (TensorMultiplierOffset.cpp:8-25): Heff + offset, on the device.
@@ -230,6 +233,7 @@ class TensorMultiplierOffset : public Multiplier {
   mpsd_ctx* ctx() const override { return ctx_; }
   void setPrecision(int p) override;
   mpsd_heff* handle() const { return h_; }
+  mpsd_heff* heff() const override { return h_; }
 
  private:
   mpsd_ctx* ctx_;
@@ -243,6 +247,7 @@ class TensorMultiplierOffset : public Multiplier {
 // host.  Stands in for PRIMME/ARPACK (src/wrapper/eigs.cpp:71-422), whose sources are not
 // part of the reference.
 struct EigStats {
+  int64_t graph_replays = 0;
   double value = 0.0;
   double residual = 0.0;  // ||H x - value x|| (Lanczos estimate)
   int matvecs = 0;
@@ -258,6 +263,9 @@ class LanczosSolver {
   EigStats lowest(Multiplier& H, double* x, double tol, int max_krylov = 30, int max_restarts = 40);
 
  private:
+  EigStats lowest_graph(Multiplier& H, mpsd_heff* h, double* x, double tol, int m, int max_restarts);
+  mpsd_lanczos* lz_ = nullptr;
+  int lz_m_ = 0;
   mpsd_ctx* ctx_;
   DevTensor V_, coef_;
   int64_t cap_ = 0;

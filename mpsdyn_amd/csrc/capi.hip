@@ -298,6 +298,16 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn) {
   return MPSD_OK;
 }
 
+int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
+  if (!ctx) return MPSD_ERR_ARG;
+  switch (option) {
+    case MPSD_OPT_TILE_K: return ctx->tile_override.tk;
+    case MPSD_OPT_SPLIT_STAGE3: return ctx->split3;
+    case MPSD_OPT_GRAPH: return ctx->graph ? 1 : 0;
+    default: return MPSD_ERR_ARG;
+  }
+}
+
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
   if (!ctx) return MPSD_ERR_ARG;
   switch (option) {
@@ -308,6 +318,9 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_SPLIT_STAGE3:
       if (value < -1 || value > 1) return set_err(ctx, MPSD_ERR_ARG, "set_option: split must be -1, 0 or 1");
       ctx->split3 = value;
+      return MPSD_OK;
+    case MPSD_OPT_GRAPH:
+      ctx->graph = value != 0;
       return MPSD_OK;
     default:
       return set_err(ctx, MPSD_ERR_ARG, "set_option: unknown option");
@@ -635,15 +648,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
   // Split the last stage over the MPO bond when its plain grid would leave CUs idle: each
   // beta gives a partial product of the same shape as stage 1, summed in a fixed order.
-  bool split = false;
-  if (h->d == 2 && h->wr >= 2) {
-    if (ctx->split3 == 1)
-      split = true;
-    else if (ctx->split3 == -1) {
-      const int64_t tiles = (int64_t)((h->Db + 31) / 32) * ((h->Dbr + 31) / 32);
-      split = tiles < 512;
-    }
-  }
+  const bool split = stage3_split(h, ctx);
   if (split) {
     rc = ensure_scratch(ctx, 0, sizeof(double2) * (size_t)(ny * h->wr));
     if (rc) return rc;
@@ -746,13 +751,15 @@ int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t 
 
 // Decide the last-stage split from the problem shape only (the same rule as heff_apply_dev),
 // so a problem takes the same arithmetic path alone and inside a group.
-static bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx) {
+namespace mpsd {
+bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx) {
   if (h->d != 2 || h->wr < 2) return false;
   if (ctx->split3 == 1) return true;
   if (ctx->split3 == 0) return false;
   const int64_t tiles = (int64_t)((h->Db + 31) / 32) * ((h->Dbr + 31) / 32);
   return tiles < 512;
 }
+}  // namespace mpsd
 
 static int upload_tables(mpsd_ctx* ctx, const std::vector<char>& host, void** dev) {
   if (ctx->gtab_bytes < host.size()) {

@@ -98,6 +98,7 @@ class HeffRef : public Multiplier {
     check(h_->ctx, mpsd_heff_full(h_, (double*)r.data()), "heff_full");
   }
   mpsd_ctx* ctx() const override { return h_->ctx; }
+  mpsd_heff* heff() const override { return h_; }
 
  private:
   mpsd_heff* h_;

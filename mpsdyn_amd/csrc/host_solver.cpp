@@ -134,7 +134,85 @@ double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b
 }
 
 // ------------------------------------------------------------------------------ Lanczos
-LanczosSolver::~LanczosSolver() {}
+// Steps per chunk between two host checks: short chunks first (late sweeps converge within a
+// few steps), then 4.  Shared by both launch paths so they build the same basis.
+static int lanczos_chunk(int j) { return j < 6 ? 3 : 4; }
+
+LanczosSolver::~LanczosSolver() {
+  if (lz_) mpsd_lanczos_destroy(lz_);
+}
+
+EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, double tol, int m, int max_restarts) {
+  EigStats st;
+  const int64_t n = H.getSize();
+  if (!lz_ || lz_m_ != m) {
+    if (lz_) mpsd_lanczos_destroy(lz_);
+    lz_ = nullptr;
+    check(ctx_, mpsd_lanczos_create(ctx_, m, &lz_), "lanczos_create");
+    lz_m_ = m;
+  }
+  const double zero[2] = {0.0, 0.0};
+  std::vector<double> abh(2 * m + 2), s;
+  double nx = 0.0;
+  check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+  if (!(nx > 0.0) || !std::isfinite(nx)) {
+    DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));
+    check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
+    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+  }
+  const int64_t rep0 = mpsd_lanczos_replays(lz_);
+  for (int restart = 0; restart <= max_restarts; ++restart) {
+    st.restarts = restart;
+    check(ctx_, mpsd_lanczos_begin(lz_, h, x, std::sqrt(nx)), "lanczos_begin");
+    int K = 0;
+    bool stop = false;
+    double theta = 0.0, res = 0.0;
+    for (int j = 0; j < m && !stop;) {
+      const int k = lanczos_chunk(j);
+      if (j + k > m) break;  // the tail shorter than a chunk ends this restart
+      int jend = 0;
+      check(ctx_, mpsd_lanczos_steps(lz_, k, abh.data(), &jend), "lanczos_steps");
+      st.matvecs += k;
+      for (int kk = j; kk < jend; ++kk) {
+        const double a = abh[2 * kk], b = abh[2 * kk + 1];  // graph path
+        if (!std::isfinite(a) || !std::isfinite(b))
+          throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
+        std::vector<double> al(kk + 1), be(kk);
+        for (int i = 0; i <= kk; ++i) al[i] = abh[2 * i];
+        for (int i = 0; i < kk; ++i) be[i] = abh[2 * i + 1];
+        theta = tridiag_lowest(al, be, s);
+        res = b * std::fabs(s[kk]);
+        const double scale = std::max(1.0, std::fabs(theta));
+        if (res <= tol * scale || b <= 1e-14 * scale || kk + 1 >= n) {
+          stop = true;
+          K = kk + 1;
+          break;
+        }
+      }
+      j = jend;
+      if (!stop) K = j;
+    }
+    if (K == 0) throw Error(MPSD_ERR_NUMERIC, "Lanczos: empty basis");
+    if ((int)s.size() != K) {  // Ritz pair of the basis actually used
+      std::vector<double> al(K), be(K - 1);
+      for (int i = 0; i < K; ++i) al[i] = abh[2 * i];
+      for (int i = 0; i + 1 < K; ++i) be[i] = abh[2 * i + 1];
+      theta = tridiag_lowest(al, be, s);
+    }
+    check(ctx_, mpsd_lanczos_ritz(lz_, s.data(), K, x), "ritz");
+    st.value = theta;
+    st.residual = res;
+    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
+    if (stop) {
+      st.converged = true;
+      break;
+    }
+  }
+  const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
+  check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, x, n), "normalise");
+  st.graph_replays = mpsd_lanczos_replays(lz_) - rep0;
+  return st;
+}
 
 EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_krylov, int max_restarts) {
   // Restarted Lanczos with two-pass full reorthogonalisation.  The iterations run on the
@@ -145,6 +223,17 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
   EigStats st;
   const int64_t n = H.getSize();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>(max_krylov, n));
+  mpsd_heff* hh = H.heff();
+  if (hh && m >= 3 && mpsd_ctx_get_option(ctx_, MPSD_OPT_GRAPH) == 1) {
+    // the graph path needs d = 2 and the FP64 / FP32 forms; anything else runs below
+    int64_t np = mpsd_heff_size(hh);
+    (void)np;
+    try {
+      return lowest_graph(H, hh, x, tol, m, max_restarts);
+    } catch (const Error& e) {
+      if (e.code != MPSD_ERR_UNSUPPORTED) throw;
+    }
+  }
   if ((int64_t)(m + 1) * n > cap_) {
     V_ = DevTensor(ctx_, std::vector<int>{(int)n, m + 1});
     cap_ = (int64_t)(m + 1) * n;
@@ -177,7 +266,15 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
     bool stop = false;
     double theta = 0.0, res = 0.0;
     for (int j = 0; j < m && !stop;) {
-      const int jend = std::min(m, j + (j == 0 ? 6 : 4));
+      const int k = lanczos_chunk(j);
+      if (j + k > m) {
+        if (j == 0) {
+          // basis smaller than the first chunk: run it whole
+        } else {
+          break;
+        }
+      }
+      const int jend = std::min(m, j + k);
       for (int jj = j; jj < jend; ++jj) {
         double* w = vec(jj + 1);
         H.product_device(vec(jj), w);
@@ -206,7 +303,7 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
         }
       }
       j = jend;
-      if (!stop && j >= m) K = m;
+      if (!stop) K = j;
     }
     // Ritz vector x = V s
     std::vector<double> c(2 * K, 0.0);

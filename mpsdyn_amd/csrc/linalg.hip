@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "mpsd_internal.h"
+#include "vec_kernels.h"
 
 namespace mpsd {
 namespace {
@@ -24,53 +25,18 @@ namespace {
 constexpr int kT = 256;
 constexpr int kDotBlocks = 128;  // partial sums per vector (fixed => fixed order)
 
-__device__ __forceinline__ double2 cmul_conj(double2 a, double2 b) {  // conj(a) * b
-  return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
-}
-
-// partial[j * kDotBlocks + blk] = sum over this block's fixed slice of conj(V_j) * w
+// partial[j * kDotBlocks + blk] = block blk of nb partial sums of conj(V_j) . w
 __global__ void __launch_bounds__(kT) dots_partial_kernel(const double2* __restrict__ V, int64_t ldv,
-                                                          const double2* __restrict__ w, int64_t n,
+                                                          const double2* __restrict__ w, int64_t n, int nb,
                                                           double2* __restrict__ partial) {
-  const int j = blockIdx.y;
-  const int blk = blockIdx.x;
-  const double2* v = V + (int64_t)j * ldv;
-  const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
-  const int64_t lo = blk * chunk;
-  const int64_t hi = min(n, lo + chunk);
-  double2 acc{0.0, 0.0};
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kT) {
-    const double2 p = cmul_conj(v[i], w[i]);
-    acc.x += p.x;
-    acc.y += p.y;
-  }
-  __shared__ double2 red[kT];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = kT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[threadIdx.x].x += red[threadIdx.x + s].x;
-      red[threadIdx.x].y += red[threadIdx.x + s].y;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[(int64_t)j * kDotBlocks + blk] = red[0];
+  const double2 r = dot_block(V + (int64_t)blockIdx.y * ldv, w, n, blockIdx.x, nb);
+  if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * kDotBlocks + blockIdx.x] = r;
 }
 
-__global__ void __launch_bounds__(kDotBlocks) dots_final_kernel(const double2* __restrict__ partial,
+__global__ void __launch_bounds__(kDotBlocks) dots_final_kernel(const double2* __restrict__ partial, int nb,
                                                                double2* __restrict__ out) {
-  const int j = blockIdx.x;
-  __shared__ double2 red[kDotBlocks];
-  red[threadIdx.x] = partial[(int64_t)j * kDotBlocks + threadIdx.x];
-  __syncthreads();
-  for (int s = kDotBlocks / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[threadIdx.x].x += red[threadIdx.x + s].x;
-      red[threadIdx.x].y += red[threadIdx.x + s].y;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[j] = red[0];
+  const double2 r = dot_final(partial + (int64_t)blockIdx.x * kDotBlocks, nb);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
 // w[i] += sign * sum_j V_j[i] * c[j]   (c on the device, m <= 1024)
@@ -78,16 +44,7 @@ __global__ void __launch_bounds__(kT) gemv_update_kernel(const double2* __restri
                                                          const double2* __restrict__ c, double sign,
                                                          double2* __restrict__ w, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  double2 acc{0.0, 0.0};
-  for (int j = 0; j < m; ++j) {
-    const double2 v = V[(int64_t)j * ldv + i];
-    const double2 cj = c[j];
-    acc.x = fma(v.x, cj.x, fma(-v.y, cj.y, acc.x));
-    acc.y = fma(v.x, cj.y, fma(v.y, cj.x, acc.y));
-  }
-  const double2 wi = w[i];
-  w[i] = double2{fma(sign, acc.x, wi.x), fma(sign, acc.y, wi.y)};
+  if (i < n) gemv_elem(V, ldv, m, c, sign, w, i);
 }
 
 // Lanczos coefficients of step j from the two Gram-Schmidt passes (h1 = V^H w before the
@@ -99,11 +56,8 @@ __global__ void __launch_bounds__(kT) lanczos_coef_kernel(const double2* __restr
                                                           const double2* __restrict__ h2, int j,
                                                           double* __restrict__ ab, double2* __restrict__ w,
                                                           int64_t n) {
-  const double alpha = h1[j].x + h2[j].x;
-  double corr = 0.0;
-  for (int i = 0; i <= j; ++i) corr = fma(h2[i].x, h2[i].x, fma(h2[i].y, h2[i].y, corr));
-  const double beta = sqrt(fmax(h2[j + 1].x - corr, 0.0));
-  const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+  double alpha, beta, inv;
+  lanczos_scalars(h1, h2, j, alpha, beta, inv);
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
     const double2 v = w[i];
     w[i] = double2{v.x * inv, v.y * inv};
@@ -113,6 +67,7 @@ __global__ void __launch_bounds__(kT) lanczos_coef_kernel(const double2* __restr
     ab[2 * j + 1] = beta;
   }
 }
+
 
 // y = a x + b y
 __global__ void __launch_bounds__(kT) axpby_kernel(double2 a, const double2* __restrict__ x, double2 b,
@@ -249,9 +204,10 @@ int mpsd_vec_dots(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const doub
   if (rc) return rc;
   double2* partial = (double2*)ctx->dots;
   double2* out = partial + (size_t)m * kDotBlocks;
-  hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks, m), dim3(kT), 0, ctx->stream,
-                     (const double2*)V, ldv, (const double2*)w, n, partial);
-  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, out);
+  const int nb = dot_blocks(n);
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, m), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)w, n, nb, partial);
+  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, nb, out);
   hipError_t e = hipMemcpyAsync(out_host, out, sizeof(double2) * m, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   return hip_err(ctx, e, "vec_dots");
@@ -290,9 +246,11 @@ int mpsd_vec_dots_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const 
   int rc = ensure_dots(ctx, m);
   if (rc) return rc;
   double2* partial = (double2*)ctx->dots;
-  hipLaunchKernelGGL(dots_partial_kernel, dim3(kDotBlocks, m), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
-                     (const double2*)w, n, partial);
-  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, (double2*)out_dev);
+  const int nb = dot_blocks(n);
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, m), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)w, n, nb, partial);
+  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, nb,
+                     (double2*)out_dev);
   return hip_err(ctx, hipGetLastError(), "vec_dots_dev");
 }
 

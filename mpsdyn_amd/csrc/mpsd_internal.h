@@ -72,11 +72,14 @@ int resolve_group(ZgemmDesc& p);
 // Grouped persistent BRGEMM over nprob problems whose table lives in device memory; all
 // problems share (tm, tn, tk, nbw, ASH/BSH kind, arith).  kind: 0 = none shared,
 // 1 = A shared by the batch group, 2 = B shared.
+// nprob_dev (optional): the problem count is read from device memory at run time instead.
 hipError_t launch_zbrgemm_grouped(const GroupedProblem* dev_probs, int nprob, int grid, int tm, int tn, int tk,
-                                  int nbw, int kind, int arith, hipStream_t s);
+                                  int nbw, int kind, int arith, hipStream_t s, const int* nprob_dev = nullptr);
 // Grouped MPO mix / partial sums over problem tables in device memory.
-hipError_t launch_zmix_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s);
-hipError_t launch_zsum_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s);
+hipError_t launch_zmix_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s,
+                               const int* nprob_dev = nullptr);
+hipError_t launch_zsum_grouped(const GroupedElem* dev_probs, int nprob, int grid, hipStream_t s,
+                               const int* nprob_dev = nullptr);
 
 // y[i] = sum_{p < np} P[p * stride + i] (+ beta * x[i]) in a fixed order (x may be null).
 hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const double2* x, double beta, double2* y,
@@ -124,6 +127,7 @@ struct mpsd_ctx {
   int64_t tensor_bytes = 0;
   mpsd::TileCfg tile_override{0, 0, 16};
   int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
+  bool graph = true;  // eigensolver steps as replayed hipGraphs where the Heff allows it
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   struct Pending {
@@ -165,6 +169,9 @@ namespace mpsd {
 int set_err(mpsd_ctx* ctx, int code, const std::string& msg);
 int hip_err(mpsd_ctx* ctx, hipError_t e, const char* where);
 int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes);
+// Whether the matvec's last stage is split over the MPO bond (a function of the shape and
+// the context option only, so every launch path takes the same arithmetic).
+bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx);
 // Run `launch` on the context's stream, bracketed by HIP events when timing is on.
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch);
 int flush_timing(mpsd_ctx* ctx);

@@ -291,8 +291,11 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, in
 // Grouped persistent form: the workgroups of one launch stride over the concatenated tile
 // lists of all problems (problem q owns tiles [tile_begin[q], tile_begin[q+1]) ).
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_grouped_kernel(const GroupedProblem* __restrict__ probs, int nprob) {
+__global__ void __launch_bounds__(kThreads) zbrgemm_grouped_kernel(const GroupedProblem* __restrict__ probs, int nprob,
+                                                                   const int* __restrict__ nprob_dev) {
   __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
+  if (nprob_dev) nprob = *nprob_dev;
+  if (nprob <= 0) return;
   const int total = probs[nprob - 1].tile_begin + probs[nprob - 1].ntiles;
   int q = 0;
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -353,30 +356,33 @@ int resolve_group(ZgemmDesc& p) {
 
 namespace {
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH>
-hipError_t grouped_a(const GroupedProblem* d, int n, int grid, int arith, hipStream_t s) {
+hipError_t grouped_a(const GroupedProblem* d, int n, const int* nd, int grid, int arith, hipStream_t s) {
   if (arith == 1)
-    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 1>), dim3(grid), dim3(kThreads), 0, s, d, n);
+    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 1>), dim3(grid), dim3(kThreads), 0, s, d, n,
+                       nd);
   else
-    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 0>), dim3(grid), dim3(kThreads), 0, s, d, n);
+    hipLaunchKernelGGL((zbrgemm_grouped_kernel<TM, TN, TK, NBW, ASH, BSH, 0>), dim3(grid), dim3(kThreads), 0, s, d, n,
+                       nd);
   return hipGetLastError();
 }
 template <int TM, int TN, int TK>
-hipError_t grouped_k(const GroupedProblem* d, int n, int grid, int nbw, int kind, int arith, hipStream_t s) {
-  if (nbw == 2 && kind == 1) return grouped_a<TM, TN, TK, 2, true, false>(d, n, grid, arith, s);
-  if (nbw == 2 && kind == 2) return grouped_a<TM, TN, TK, 2, false, true>(d, n, grid, arith, s);
-  if (nbw == 2) return grouped_a<TM, TN, TK, 2, false, false>(d, n, grid, arith, s);
-  return grouped_a<TM, TN, TK, 1, false, false>(d, n, grid, arith, s);
+hipError_t grouped_k(const GroupedProblem* d, int n, const int* nd, int grid, int nbw, int kind, int arith,
+                     hipStream_t s) {
+  if (nbw == 2 && kind == 1) return grouped_a<TM, TN, TK, 2, true, false>(d, n, nd, grid, arith, s);
+  if (nbw == 2 && kind == 2) return grouped_a<TM, TN, TK, 2, false, true>(d, n, nd, grid, arith, s);
+  if (nbw == 2) return grouped_a<TM, TN, TK, 2, false, false>(d, n, nd, grid, arith, s);
+  return grouped_a<TM, TN, TK, 1, false, false>(d, n, nd, grid, arith, s);
 }
 }  // namespace
 
 hipError_t launch_zbrgemm_grouped(const GroupedProblem* d, int n, int grid, int tm, int tn, int tk, int nbw, int kind,
-                                  int arith, hipStream_t s) {
-  if (n <= 0 || grid <= 0) return hipSuccess;
+                                  int arith, hipStream_t s, const int* nprob_dev) {
+  if ((n <= 0 && !nprob_dev) || grid <= 0) return hipSuccess;
   if (arith != 0 && arith != 1) return hipErrorInvalidValue;
-  if (tm == 32 && tn == 32) return tk == 32 ? grouped_k<32, 32, 32>(d, n, grid, nbw, kind, arith, s)
-                                            : grouped_k<32, 32, 16>(d, n, grid, nbw, kind, arith, s);
-  if (tm == 64 && tn == 32) return grouped_k<64, 32, 16>(d, n, grid, nbw, kind, arith, s);
-  if (tm == 32 && tn == 64) return grouped_k<32, 64, 16>(d, n, grid, nbw, kind, arith, s);
+  if (tm == 32 && tn == 32) return tk == 32 ? grouped_k<32, 32, 32>(d, n, nprob_dev, grid, nbw, kind, arith, s)
+                                            : grouped_k<32, 32, 16>(d, n, nprob_dev, grid, nbw, kind, arith, s);
+  if (tm == 64 && tn == 32) return grouped_k<64, 32, 16>(d, n, nprob_dev, grid, nbw, kind, arith, s);
+  if (tm == 32 && tn == 64) return grouped_k<32, 64, 16>(d, n, nprob_dev, grid, nbw, kind, arith, s);
   return hipErrorInvalidValue;
 }
 
@@ -504,7 +510,10 @@ __device__ __forceinline__ int find_elem(const GroupedElem* probs, int nprob, in
 }
 
 // Persistent grid-stride form of zmix_f64 over several problems (element = one (bx, ax)).
-__global__ void __launch_bounds__(kThreads) zmix_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob) {
+__global__ void __launch_bounds__(kThreads) zmix_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob,
+                                                               const int* __restrict__ nprob_dev) {
+  if (nprob_dev) nprob = *nprob_dev;
+  if (nprob <= 0) return;
   const GroupedElem& last = probs[nprob - 1];
   const int64_t total = last.begin + (int64_t)last.Bx * last.Ax;
   int q = 0;
@@ -538,7 +547,10 @@ __global__ void __launch_bounds__(kThreads) zmix_grouped_kernel(const GroupedEle
   }
 }
 
-__global__ void __launch_bounds__(kThreads) zsum_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob) {
+__global__ void __launch_bounds__(kThreads) zsum_grouped_kernel(const GroupedElem* __restrict__ probs, int nprob,
+                                                               const int* __restrict__ nprob_dev) {
+  if (nprob_dev) nprob = *nprob_dev;
+  if (nprob <= 0) return;
   const GroupedElem& last = probs[nprob - 1];
   const int64_t total = last.begin + last.n;
   int q = 0;
@@ -562,15 +574,15 @@ __global__ void __launch_bounds__(kThreads) zsum_grouped_kernel(const GroupedEle
 }
 }  // namespace
 
-hipError_t launch_zmix_grouped(const GroupedElem* d, int n, int grid, hipStream_t s) {
-  if (n <= 0 || grid <= 0) return hipSuccess;
-  hipLaunchKernelGGL(zmix_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n);
+hipError_t launch_zmix_grouped(const GroupedElem* d, int n, int grid, hipStream_t s, const int* nprob_dev) {
+  if ((n <= 0 && !nprob_dev) || grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zmix_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n, nprob_dev);
   return hipGetLastError();
 }
 
-hipError_t launch_zsum_grouped(const GroupedElem* d, int n, int grid, hipStream_t s) {
-  if (n <= 0 || grid <= 0) return hipSuccess;
-  hipLaunchKernelGGL(zsum_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n);
+hipError_t launch_zsum_grouped(const GroupedElem* d, int n, int grid, hipStream_t s, const int* nprob_dev) {
+  if ((n <= 0 && !nprob_dev) || grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zsum_grouped_kernel, dim3(grid), dim3(kThreads), 0, s, d, n, nprob_dev);
   return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ PREC_BF16X3 = 2
 
 OPT_TILE_K = 1
 OPT_SPLIT_STAGE3 = 2
+OPT_GRAPH = 3
 
 _lib = None
 
@@ -110,6 +111,7 @@ def _optional(lib):
         "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
         "mpsd_ctx_set_tile": (I, [P, I, I]),
         "mpsd_ctx_set_option": (I, [P, I, I]),
+        "mpsd_ctx_get_option": (I, [P, I]),
     }
     for name, (res, args) in extra.items():
         if hasattr(lib, name):
