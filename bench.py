@@ -14,8 +14,19 @@ Ground-state part of the metric: --gs-solves N independent solves of the config-
 reported as solves/s, p50/p99 latency and the worst energy error against the reference's
 own value for that chain.
 
+Batched form (--batched N, default 64): N independent TFIM solves with seeded uniform
+L in [16, 64], g in [0.5, 1.5], h in [0, 0.5], D in {16, 32, 64, 128}, all submitted at once to
+the serving loop (16 slots); aggregate solves/s and matvecs/s.
+
+Served form (--served N, default 32): a seeded Poisson stream (--rate per second) of the
+model mix 50% TFIM / 30% XXZ (Delta in [0.5, 1.5]) / 20% Schwinger (x in [25, 100], m/g in
+[0, 0.5]) with energy tolerance 1e-10, run once in FP64 and once in the mixed-precision form;
+solves/s and p50/p99 latency (arrival to result).  By default L is drawn from [20, 60]
+(Schwinger [40, 60]) and D from {32, 64, 128} so the run stays within minutes;
+--full-serving uses the full ranges L in [20, 200] (Schwinger [40, 120]) and D up to 512.
+
     python bench.py [--steps K] [--warmup W] [--D 512] [--w 3] [--cpu-seconds 15] [--no-cpu]
-                    [--gs-solves N]
+                    [--gs-solves N] [--batched N] [--served N] [--rate R] [--full-serving]
 """
 from __future__ import annotations
 
@@ -97,6 +108,92 @@ def ground_states(ctx, dev, n):
             "max_rel_energy_error_vs_reference": float(max(errs)), "matvecs_per_solve": mv / n}
 
 
+def traffic_probe(dev, gaussian, D, w):
+    """Workload of the HBM-traffic counter runs: a device copy of known size (the scale of
+    FETCH_SIZE / WRITE_SIZE) followed by 5 matvecs at the benchmark shape."""
+    ctx = dev.Context(0)
+    nbytes = 256 << 20
+    a = ctx.tensor(dims=(nbytes // 16,))
+    b = ctx.tensor(dims=(nbytes // 16,))
+    for _ in range(3):
+        ctx.check(ctx.lib.mpsd_tensor_copy(b.handle, a.handle))
+    ctx.sync()
+    d = 2
+    tL, tW, tR = (ctx.tensor(gaussian(101, (D, w, D))), ctx.tensor(gaussian(102, (d, w, d, w))),
+                  ctx.tensor(gaussian(103, (D, w, D))))
+    tx, ty = ctx.tensor(gaussian(104, (d, D, D))), ctx.tensor(dims=(d, D, D))
+    h = dev.Heff(ctx, tL, tW, tR)
+    for _ in range(5):
+        h.apply_device(tx.devptr(), ty.devptr())
+    ctx.sync()
+    h.close()
+    ctx.close()
+    print(json.dumps({"traffic_probe": {"copy_bytes": nbytes, "copies": 3, "matvecs": 5, "D": D, "w": w}}))
+
+
+def batched_form(dev, n):
+    """Config 4: n TFIM instances of uneven L, g, h, D submitted at once."""
+    rng = np.random.default_rng(4)
+    cfgs = []
+    for k in range(n):
+        L = int(rng.integers(16, 65))
+        g, h = float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.0, 0.5))
+        D = int(rng.choice([16, 32, 64, 128]))
+        cfgs.append(dev.gs_config(dev.MODEL_ISING, L, D, [-1.0, -g, -h], convtol=1e-10, seed=k + 1))
+    srv = dev.Server(slots=16, batching=False)
+    t0 = time.perf_counter()
+    res = srv.run_trace(cfgs, [0.0] * n)
+    wall = time.perf_counter() - t0
+    srv.close()
+    mv = sum(r.matvecs for r in res)
+    return {"workload": "tfim_x64_L16-64_D16-128_fp64", "solves": n, "solves_per_s": round(n / wall, 3),
+            "matvecs_per_s": round(mv / wall, 1), "wall_s": round(wall, 3),
+            "failed": sum(1 for r in res if r.status != 0)}
+
+
+def served_form(dev, n, rate, full):
+    """Config 5: Poisson arrivals of a model mix; FP64 and mixed precision."""
+    rng = np.random.default_rng(5)
+    Lmax, Smax, Ds = (200, 120, [32, 64, 128, 256, 512]) if full else (60, 60, [32, 64, 128])
+    cfgs, t = [], 0.0
+    arrivals = []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        arrivals.append(t)
+        u = rng.uniform()
+        D = int(rng.choice(Ds))
+        if u < 0.5:
+            L = int(rng.integers(20, Lmax + 1))
+            cfgs.append((dev.MODEL_ISING, L, D, [-1.0, -float(rng.uniform(0.5, 1.5)), -float(rng.uniform(0, 0.5))]))
+        elif u < 0.8:
+            L = int(rng.integers(20, Lmax + 1))
+            cfgs.append((dev.MODEL_HEISENBERG, L, D, [1.0, 1.0, float(rng.uniform(0.5, 1.5)), 0.0]))
+        else:
+            L = int(rng.integers(40, Smax + 1))
+            x = float(rng.uniform(25, 100))
+            mg = float(rng.uniform(0.0, 0.5))
+            cfgs.append((dev.MODEL_SCHWINGER, L, D, [2 * mg * np.sqrt(x), x, 0.0, 0.0]))
+    out = {"workload": ("poisson_mix_full" if full else "poisson_mix_L20-60_D32-128") + f"_rate{rate}",
+           "requests": n}
+    for name, prec in (("fp64", 0), ("mixed_fp32", 1)):
+        gcfgs = [dev.gs_config(m, L, D, p, convtol=1e-10, seed=i + 1, precision=prec)
+                 for i, (m, L, D, p) in enumerate(cfgs)]
+        srv = dev.Server(slots=16, batching=False)
+        t0 = time.perf_counter()
+        res = srv.run_trace(gcfgs, arrivals)
+        wall = time.perf_counter() - t0
+        srv.close()
+        lat = [r.finish_s - r.arrival_s for r in res]
+        out[name] = {"solves_per_s": round(n / wall, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                     "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+                     "mean_queue_s": round(float(np.mean([r.admit_s - r.arrival_s for r in res])), 4),
+                     "failed": sum(1 for r in res if r.status != 0)}
+        out[name + "_energies"] = [r.energy for r in res]
+    e64, e32 = out.pop("fp64_energies"), out.pop("mixed_fp32_energies")
+    out["mixed_vs_fp64_max_rel_energy_diff"] = float(max(abs(a - b) / abs(a) for a, b in zip(e64, e32)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +204,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gs-solves", type=int, default=8)
+    ap.add_argument("--batched", type=int, default=64)
+    ap.add_argument("--served", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--full-serving", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only a 256 MiB device copy (3x) and 5 matvecs, for rocprofv3 --pmc counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a design "
@@ -116,6 +219,10 @@ def main():
 
     from mpsdyn_amd import device as dev
     from oracle.mpsd_oracle import complex_gaussian  # input generator only
+
+    if args.traffic_probe:
+        traffic_probe(dev, complex_gaussian, args.D, args.w)
+        return
 
     D, w, d = args.D, args.w, 2
     L = complex_gaussian(101, (D, w, D))
@@ -181,6 +288,8 @@ def main():
     env_tflops = F / (env["env_left_ms"] * 1e-3) / 1e12
 
     gs = ground_states(ctx, dev, args.gs_solves) if args.gs_solves > 0 else None
+    batched = batched_form(dev, args.batched) if args.batched > 0 else None
+    served = served_form(dev, args.served, args.rate, args.full_serving) if args.served > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +318,8 @@ def main():
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
         "env_update": {k: round(v, 4) for k, v in env.items()} | {"env_left_tflops": round(env_tflops, 3)},
         "gs": gs,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -161,6 +161,9 @@ int mpsd_vec_dots_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const 
                       double* out_dev);
 int mpsd_vec_gemv_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const double* c_dev, double sign,
                       double* w, int64_t n);
+/* y = x / |x| on the device (y may be x); nrm2_dev receives |x|^2 as a complex (re, 0).
+ * A zero or non-finite norm leaves y = x. */
+int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n, double* nrm2_dev);
 /* Lanczos step j on the device: from h1 = V_{0..j}^H w (first Gram-Schmidt pass) and
  * h2 = [V_{0..j}, w]^H w (after it) store alpha_j = Re(h1[j] + h2[j]) and
  * beta_j = sqrt(|w|^2 - sum |h2[i]|^2) at ab_dev[2j], ab_dev[2j+1], and scale w by 1/beta_j
@@ -200,13 +203,15 @@ int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* 
 typedef struct mpsd_lanczos mpsd_lanczos;
 int mpsd_lanczos_create(mpsd_ctx* ctx, int max_krylov, mpsd_lanczos** out);
 int mpsd_lanczos_destroy(mpsd_lanczos* lz);
-/* Bind Heff h and start the basis with V_0 = x / xnorm (x a device vector of the Heff size). */
-int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x, double xnorm);
+/* Bind Heff h and start the basis with V_0 = x / |x| (x a device vector of the Heff size;
+ * normalised on the device). */
+int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x);
 /* Run k (1..8) more steps as one graph replay (graphs are captured once per launch shape and
- * cached); ab_host receives alpha_i, beta_i (interleaved) of all steps so far, *j_end their
- * number. */
+ * cached); ab_host receives |x|^2 of the start vector, 0, then alpha_i, beta_i (interleaved)
+ * of all steps so far: 2 + 2 * (*j_end) doubles. */
 int mpsd_lanczos_steps(mpsd_lanczos* lz, int k, double* ab_host, int* j_end);
-/* x = sum_{i<K} s_host[i] V_i (Ritz vector from real coefficients). */
+/* x = sum_{i<K} s_host[i] V_i / |.| (Ritz vector from real coefficients, normalised on the
+ * device). */
 int mpsd_lanczos_ritz(mpsd_lanczos* lz, const double* s_host, int K, double* x);
 /* Number of graph replays so far. */
 int64_t mpsd_lanczos_replays(const mpsd_lanczos* lz);

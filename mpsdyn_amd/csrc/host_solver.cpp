@@ -143,6 +143,8 @@ LanczosSolver::~LanczosSolver() {
 }
 
 EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, double tol, int m, int max_restarts) {
+  // Device-driven steps replayed as graphs; the start and Ritz vectors are normalised on the
+  // device, so a site costs one host round trip per chunk of steps.
   EigStats st;
   const int64_t n = H.getSize();
   if (!lz_ || lz_m_ != m) {
@@ -151,19 +153,12 @@ EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, dou
     check(ctx_, mpsd_lanczos_create(ctx_, m, &lz_), "lanczos_create");
     lz_m_ = m;
   }
-  const double zero[2] = {0.0, 0.0};
-  std::vector<double> abh(2 * m + 2), s;
-  double nx = 0.0;
-  check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
-  if (!(nx > 0.0) || !std::isfinite(nx)) {
-    DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));
-    check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
-    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
-  }
+  std::vector<double> abh(2 * m + 4), s;
   const int64_t rep0 = mpsd_lanczos_replays(lz_);
+  bool seeded = false;
   for (int restart = 0; restart <= max_restarts; ++restart) {
     st.restarts = restart;
-    check(ctx_, mpsd_lanczos_begin(lz_, h, x, std::sqrt(nx)), "lanczos_begin");
+    check(ctx_, mpsd_lanczos_begin(lz_, h, x), "lanczos_begin");
     int K = 0;
     bool stop = false;
     double theta = 0.0, res = 0.0;
@@ -173,13 +168,23 @@ EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, dou
       int jend = 0;
       check(ctx_, mpsd_lanczos_steps(lz_, k, abh.data(), &jend), "lanczos_steps");
       st.matvecs += k;
+      if (j == 0) {
+        const double nx = abh[0];
+        if (!(nx > 0.0) || !std::isfinite(nx)) {
+          if (seeded) throw Error(MPSD_ERR_NUMERIC, "Lanczos: start vector is zero or not finite");
+          DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));  // reseed and start over
+          check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
+          seeded = true;
+          break;
+        }
+      }
       for (int kk = j; kk < jend; ++kk) {
-        const double a = abh[2 * kk], b = abh[2 * kk + 1];  // graph path
+        const double a = abh[2 + 2 * kk], b = abh[3 + 2 * kk];
         if (!std::isfinite(a) || !std::isfinite(b))
           throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
         std::vector<double> al(kk + 1), be(kk);
-        for (int i = 0; i <= kk; ++i) al[i] = abh[2 * i];
-        for (int i = 0; i < kk; ++i) be[i] = abh[2 * i + 1];
+        for (int i = 0; i <= kk; ++i) al[i] = abh[2 + 2 * i];
+        for (int i = 0; i < kk; ++i) be[i] = abh[3 + 2 * i];
         theta = tridiag_lowest(al, be, s);
         res = b * std::fabs(s[kk]);
         const double scale = std::max(1.0, std::fabs(theta));
@@ -192,42 +197,35 @@ EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, dou
       j = jend;
       if (!stop) K = j;
     }
-    if (K == 0) throw Error(MPSD_ERR_NUMERIC, "Lanczos: empty basis");
-    if ((int)s.size() != K) {  // Ritz pair of the basis actually used
-      std::vector<double> al(K), be(K - 1);
-      for (int i = 0; i < K; ++i) al[i] = abh[2 * i];
-      for (int i = 0; i + 1 < K; ++i) be[i] = abh[2 * i + 1];
-      theta = tridiag_lowest(al, be, s);
+    if (K == 0) {
+      if (seeded && restart < max_restarts) continue;
+      throw Error(MPSD_ERR_NUMERIC, "Lanczos: empty basis");
     }
     check(ctx_, mpsd_lanczos_ritz(lz_, s.data(), K, x), "ritz");
     st.value = theta;
     st.residual = res;
-    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
     if (stop) {
       st.converged = true;
       break;
     }
   }
-  const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
-  check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, x, n), "normalise");
   st.graph_replays = mpsd_lanczos_replays(lz_) - rep0;
   return st;
 }
 
 EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_krylov, int max_restarts) {
   // Restarted Lanczos with two-pass full reorthogonalisation.  The iterations run on the
-  // device in chunks (first 6, then 4): Gram-Schmidt coefficients, alpha and beta stay in
-  // device memory, and the host reads the chunk's alphas/betas once to solve the tridiagonal
-  // problem.  The stopping step is the first converged one inside the chunk, so the result
-  // does not depend on the chunking (later iterations of that chunk are discarded).
+  // device in chunks (lanczos_chunk): Gram-Schmidt coefficients, alpha and beta stay in device
+  // memory, and the host reads the chunk's alphas/betas once to solve the tridiagonal
+  // problem.  The stopping step is the first converged one inside the chunk.  This
+  // stream-launched form and the graph-replayed form (lowest_graph) run the same operations
+  // in the same order, so they return the same bits.
   EigStats st;
   const int64_t n = H.getSize();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>(max_krylov, n));
   mpsd_heff* hh = H.heff();
   if (hh && m >= 3 && mpsd_ctx_get_option(ctx_, MPSD_OPT_GRAPH) == 1) {
     // the graph path needs d = 2 and the FP64 / FP32 forms; anything else runs below
-    int64_t np = mpsd_heff_size(hh);
-    (void)np;
     try {
       return lowest_graph(H, hh, x, tol, m, max_restarts);
     } catch (const Error& e) {
@@ -245,35 +243,20 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
   double* V = V_.ptr();
   const int64_t ldv = n;
   auto vec = [&](int j) { return V + 2 * (int64_t)j * ldv; };
-  double* h1 = coef_.ptr();               // m + 1 complex
-  double* h2 = h1 + 2 * (m + 2);          // m + 2 complex
-  double* ab = h2 + 2 * (m + 2);          // 2 m reals (+ 2 for the start norm)
-  const double zero[2] = {0.0, 0.0};
-  std::vector<double> abh(2 * m + 2), s;
-
-  double nx = 0.0;
-  check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
-  if (!(nx > 0.0) || !std::isfinite(nx)) {
-    DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));
-    check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
-    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
-  }
+  double* h1 = coef_.ptr();       // m + 2 complex
+  double* h2 = h1 + 2 * (m + 2);  // m + 2 complex
+  double* ab = h2 + 2 * (m + 2);  // |x|^2, 0, then alpha_j, beta_j
+  std::vector<double> abh(2 * m + 4), s;
+  bool seeded = false;
   for (int restart = 0; restart <= max_restarts; ++restart) {
     st.restarts = restart;
-    const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
-    check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, vec(0), n), "normalise");
+    check(ctx_, mpsd_vec_normalize_dev(ctx_, x, vec(0), n, ab), "normalise");
     int K = 0;  // size of the Krylov basis used for the Ritz vector
     bool stop = false;
     double theta = 0.0, res = 0.0;
     for (int j = 0; j < m && !stop;) {
       const int k = lanczos_chunk(j);
-      if (j + k > m) {
-        if (j == 0) {
-          // basis smaller than the first chunk: run it whole
-        } else {
-          break;
-        }
-      }
+      if (j + k > m && j > 0) break;  // the tail shorter than a chunk ends this restart
       const int jend = std::min(m, j + k);
       for (int jj = j; jj < jend; ++jj) {
         double* w = vec(jj + 1);
@@ -283,43 +266,55 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
         check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h1, -1.0, w, n), "gemv");
         check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 2, w, n, h2), "dots2");
         check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h2, -1.0, w, n), "gemv2");
-        check(ctx_, mpsd_lanczos_coef(ctx_, h1, h2, jj, ab, w, n), "coef");
+        check(ctx_, mpsd_lanczos_coef(ctx_, h1, h2, jj, ab + 2, w, n), "coef");
       }
-      check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * 2 * jend), "coef download");
-      for (int k = j; k < jend; ++k) {
-        const double a = abh[2 * k], b = abh[2 * k + 1];
+      check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef download");
+      if (j == 0) {
+        const double nx = abh[0];
+        if (!(nx > 0.0) || !std::isfinite(nx)) {
+          if (seeded) throw Error(MPSD_ERR_NUMERIC, "Lanczos: start vector is zero or not finite");
+          DevTensor g(ctx_, complex_gaussian(12345, {(int)n}));  // reseed and start over
+          check(ctx_, mpsd_vec_copy(ctx_, g.ptr(), x, n), "seed");
+          seeded = true;
+          break;
+        }
+      }
+      for (int kk = j; kk < jend; ++kk) {
+        const double a = abh[2 + 2 * kk], b = abh[3 + 2 * kk];
         if (!std::isfinite(a) || !std::isfinite(b))
           throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
-        std::vector<double> al(k + 1), be(k);
-        for (int i = 0; i <= k; ++i) al[i] = abh[2 * i];
-        for (int i = 0; i < k; ++i) be[i] = abh[2 * i + 1];
+        std::vector<double> al(kk + 1), be(kk);
+        for (int i = 0; i <= kk; ++i) al[i] = abh[2 + 2 * i];
+        for (int i = 0; i < kk; ++i) be[i] = abh[3 + 2 * i];
         theta = tridiag_lowest(al, be, s);
-        res = b * std::fabs(s[k]);
+        res = b * std::fabs(s[kk]);
         const double scale = std::max(1.0, std::fabs(theta));
-        if (res <= tol * scale || b <= 1e-14 * scale || k + 1 >= n) {
+        if (res <= tol * scale || b <= 1e-14 * scale || kk + 1 >= n) {
           stop = true;
-          K = k + 1;
+          K = kk + 1;
           break;
         }
       }
       j = jend;
       if (!stop) K = j;
     }
-    // Ritz vector x = V s
+    if (K == 0) {
+      if (seeded && restart < max_restarts) continue;
+      throw Error(MPSD_ERR_NUMERIC, "Lanczos: empty basis");
+    }
+    // Ritz vector x = V s / |V s|
     std::vector<double> c(2 * K, 0.0);
     for (int i = 0; i < K; ++i) c[2 * i] = s[i];
     check(ctx_, mpsd_vec_zero(ctx_, x, n), "zero");
     check(ctx_, mpsd_vec_gemv(ctx_, V, ldv, K, c.data(), x, n), "ritz");
+    check(ctx_, mpsd_vec_normalize_dev(ctx_, x, x, n, h1), "normalise");
     st.value = theta;
     st.residual = res;
-    check(ctx_, mpsd_vec_norm2(ctx_, x, n, &nx), "norm");
     if (stop) {
       st.converged = true;
       break;
     }
   }
-  const double inv[2] = {1.0 / std::sqrt(nx), 0.0};
-  check(ctx_, mpsd_vec_axpby(ctx_, inv, x, zero, x, n), "normalise");
   return st;
 }
 

@@ -103,8 +103,8 @@ __global__ void __launch_bounds__(kT) coef_ctl(const LanczosCtl* __restrict__ c,
     w[i] = double2{v.x * inv, v.y * inv};
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ab[2 * j] = alpha;
-    ab[2 * j + 1] = beta;
+    ab[2 + 2 * j] = alpha;  // ab[0] holds |x|^2 of the start vector
+    ab[3 + 2 * j] = beta;
   }
 }
 
@@ -225,7 +225,7 @@ int mpsd_lanczos_create(mpsd_ctx* ctx, int max_krylov, mpsd_lanczos** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&lz->partial, sizeof(double2) * kDotBlocks * ma);
   if (e == hipSuccess) e = hipMalloc((void**)&lz->h1, sizeof(double2) * ma);
   if (e == hipSuccess) e = hipMalloc((void**)&lz->h2, sizeof(double2) * ma);
-  if (e == hipSuccess) e = hipMalloc((void**)&lz->ab, sizeof(double) * 2 * ma);
+  if (e == hipSuccess) e = hipMalloc((void**)&lz->ab, sizeof(double) * (2 * ma + 2));
   if (e != hipSuccess) {
     mpsd_lanczos_destroy(lz);
     return hip_err(ctx, e, "lanczos_create");
@@ -249,8 +249,8 @@ int mpsd_lanczos_destroy(mpsd_lanczos* lz) {
   return MPSD_OK;
 }
 
-int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x, double xnorm) {
-  if (!lz || !h || !x || !(xnorm > 0)) return MPSD_ERR_ARG;
+int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x) {
+  if (!lz || !h || !x) return MPSD_ERR_ARG;
   mpsd_ctx* ctx = lz->ctx;
   if (h->d != 2 || h->precision == MPSD_PREC_BF16X3)
     return set_err(ctx, MPSD_ERR_UNSUPPORTED, "lanczos graph: needs d = 2 and the FP64 or FP32 form");
@@ -330,9 +330,8 @@ int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x, double x
                       cap((n + kT - 1) / kT, kVecBlocks), dot_blocks(n)};
   rc = upload(lz);
   if (rc) return rc;
-  // V_0 = x / |x|
-  const double inv[2] = {1.0 / xnorm, 0.0}, zero[2] = {0.0, 0.0};
-  return mpsd_vec_axpby(ctx, inv, x, zero, (double*)lz->V, n);
+  // V_0 = x / |x|, |x|^2 into ab[0..1]
+  return mpsd_vec_normalize_dev(ctx, x, (double*)lz->V, n, lz->ab);
 }
 
 int mpsd_lanczos_steps(mpsd_lanczos* lz, int k, double* ab_host, int* j_end) {
@@ -346,7 +345,7 @@ int mpsd_lanczos_steps(mpsd_lanczos* lz, int k, double* ab_host, int* j_end) {
   ++lz->replays;
   lz->jn += k;
   *j_end = lz->jn;
-  return mpsd_copy_to_host(lz->ctx, lz->ab, ab_host, (int64_t)sizeof(double) * 2 * (*j_end));
+  return mpsd_copy_to_host(lz->ctx, lz->ab, ab_host, (int64_t)sizeof(double) * (2 + 2 * (*j_end)));
 }
 
 int mpsd_lanczos_ritz(mpsd_lanczos* lz, const double* s_host, int K, double* x) {
@@ -355,6 +354,7 @@ int mpsd_lanczos_ritz(mpsd_lanczos* lz, const double* s_host, int K, double* x) 
   for (int i = 0; i < K; ++i) c[2 * i] = s_host[i];
   int rc = mpsd_vec_zero(lz->ctx, x, lz->n);
   if (!rc) rc = mpsd_vec_gemv(lz->ctx, (const double*)lz->V, lz->n, K, c.data(), x, lz->n);
+  if (!rc) rc = mpsd_vec_normalize_dev(lz->ctx, x, x, lz->n, (double*)lz->h1);  // h1 is free between chunks
   return rc;
 }
 

@@ -84,6 +84,17 @@ __global__ void __launch_bounds__(kT) fill_kernel(double2* __restrict__ x, int64
   if (i < n) x[i] = double2{0.0, 0.0};
 }
 
+// y = x / sqrt(nrm2->x) (y = x when the norm is zero or not finite: the caller checks it)
+__global__ void __launch_bounds__(kT) scale_invsqrt_kernel(const double2* __restrict__ x, double2* __restrict__ y,
+                                                           int64_t n, const double2* __restrict__ nrm2) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double q = nrm2->x;
+  const double inv = (q > 0.0 && q < 1e300) ? 1.0 / sqrt(q) : 1.0;
+  const double2 v = x[i];
+  y[i] = double2{v.x * inv, v.y * inv};
+}
+
 // Upper-trapezoidal R (k x n) out of a geqrf result (lda = m), zeros below the diagonal.
 __global__ void __launch_bounds__(kT) extract_r_kernel(const double2* __restrict__ A, int m, int k, int n,
                                                        double2* __restrict__ R) {
@@ -270,6 +281,15 @@ int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev,
   hipLaunchKernelGGL(lanczos_coef_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)h1_dev,
                      (const double2*)h2_dev, j, ab_dev, (double2*)w, n);
   return hip_err(ctx, hipGetLastError(), "lanczos_coef");
+}
+
+int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n, double* nrm2_dev) {
+  if (!ctx || !x || !y || !nrm2_dev || n <= 0) return MPSD_ERR_ARG;
+  int rc = mpsd_vec_dots_dev(ctx, x, n, 1, x, n, nrm2_dev);  // nrm2_dev[0..1] = (|x|^2, 0)
+  if (rc) return rc;
+  hipLaunchKernelGGL(scale_invsqrt_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)x, (double2*)y,
+                     n, (const double2*)nrm2_dev);
+  return hip_err(ctx, hipGetLastError(), "vec_normalize_dev");
 }
 
 int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n) {
