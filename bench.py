@@ -267,11 +267,14 @@ def main():
     st1 = kern["stage1_env_x_site"]["avg_us"]
     st3 = kern["stage3_heff"]["avg_us"]
     if st1 >= st3:
-        dom, dom_us, dom_flops = "zbrgemm_f64 (stage 1: env x site)", st1, s1
+        dom, dom_us, dom_flops = "zbrgemm_f64 stage1 (env x site)", st1, s1
     else:
-        dom, dom_us, dom_flops = "zbrgemm_f64 (stage 3: x R)", st3, s3
+        dom, dom_us, dom_flops = "zbrgemm_f64 stage3 (x R, beta-split parts)", st3, s3
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
     traffic = load_traffic()
+    dom_traffic = None
+    if traffic and traffic.get("workload", "").find(f"D={D}, w={w},") >= 0:
+        dom_traffic = traffic["kernels"].get(dom, {}).get("bytes_per_launch")
 
     # environment updates at the same shape (one of each per site in a sweep)
     env = {}
@@ -314,7 +317,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None},
+                     "traffic": dom_traffic},
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
         "env_update": {k: round(v, 4) for k, v in env.items()} | {"env_left_tflops": round(env_tflops, 3)},
         "gs": gs,
