@@ -22,7 +22,11 @@ LIB = os.path.join(HERE, "libmpsd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: no v_pk_*_f32 in generated code (see the library notes on packed FP32).
-DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs.  Measured on gfx950, a wave issues
+# v_mfma_f64_16x16x4_f64 every 64 cycles with VGPR accumulators but only every ~140 cycles
+# with AGPR ones, so one wave per SIMD can keep the FP64 matrix core busy only in VGPR form.
+DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                "-mllvm", "-amdgpu-mfma-vgpr-form"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

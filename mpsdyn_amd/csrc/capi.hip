@@ -304,6 +304,7 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_TILE_K: return ctx->tile_override.tk;
     case MPSD_OPT_SPLIT_STAGE3: return ctx->split3;
     case MPSD_OPT_GRAPH: return ctx->graph ? 1 : 0;
+    case MPSD_OPT_BATCH_GROUP: return ctx->tile_override.nbw;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -321,6 +322,10 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
       return MPSD_OK;
     case MPSD_OPT_GRAPH:
       ctx->graph = value != 0;
+      return MPSD_OK;
+    case MPSD_OPT_BATCH_GROUP:
+      if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");
+      ctx->tile_override.nbw = value;
       return MPSD_OK;
     default:
       return set_err(ctx, MPSD_ERR_ARG, "set_option: unknown option");
@@ -838,7 +843,7 @@ static int grouped_apply(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const doubl
     d.sBk = h->d; d.sBn = (int64_t)h->d * h->Dk; d.sBb = 1;
     d.sCm = 1; d.sCn = (int64_t)h->Db * h->wl * h->d; d.sCb = (int64_t)h->Db * h->wl;
     d.a_mfast = 1; d.b_kfast = 1; d.arith = arith;
-    if (resolve_group(d) != (h->d >= 2 ? 2 : 1)) return set_err(ctx, MPSD_ERR_UNSUPPORTED, "grouped: batch layout");
+    choose_group(d, ctx->tile_override.nbw);
     tiles(g, b1);
     st1.push_back(g);
     // mix
@@ -876,7 +881,7 @@ static int grouped_apply(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const doubl
         t.Dadd = x;
         t.beta = h->offset;
       }
-      resolve_group(t);
+      choose_group(t, ctx->tile_override.nbw);
       tiles(g3, b3n);
       st3n.push_back(g3);
     }
@@ -888,7 +893,8 @@ static int grouped_apply(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const doubl
   rc = upload_tables(ctx, buf, &dev);
   if (rc) return rc;
   char* D = (char*)dev;
-  const int persist = 256 * 3;
+  // persistent grids: as many workgroups as fit at once (one-batch FP64 tiles: six per CU)
+  const int persist = 256 * ((!st1.empty() && st1[0].desc.group == 1) ? 6 : 3);
   auto grid_of = [&](int64_t work, int per) { return (int)std::max<int64_t>(1, std::min<int64_t>(persist, (work + per - 1) / per)); };
   const int kind1 = st1.empty() ? 0 : (st1[0].desc.group == 2 ? 1 : 0);
   hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {

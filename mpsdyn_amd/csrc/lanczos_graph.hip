@@ -24,11 +24,11 @@ namespace {
 
 constexpr int kT = kVecThreads;
 constexpr int kDotBlocks = kDotMaxBlocks;
-constexpr int kPersist = 512;    // most workgroups of a persistent matvec launch
+constexpr int kPersist = 512;    // most workgroups of a persistent matvec launch (x3 for one-batch FP64 tiles)
 constexpr int kVecBlocks = 512;  // most blocks of a grid-stride vector kernel
 
 // Launch shape of one captured step: grids sized to the bound site's shapes.
-typedef std::array<int, 8> GridKey;  // arith, k, st1, mix, st3, sum, vec, dot blocks
+typedef std::array<int, 10> GridKey;  // arith, k, st1, mix, st3, sum, vec, dot blocks, st1 group, st3 group
 
 struct LanczosCtl {
   int j, j_next, split, has_offset;
@@ -163,9 +163,9 @@ hipError_t enqueue_step(mpsd_lanczos* lz, const GridKey& g) {
   const int ma = lz->mmax + 2;
   const int arith = g[0], nb = g[7];
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, b);
-  hipError_t e = launch_zbrgemm_grouped(&b->st1, 1, g[2], 32, 32, 16, 2, 1, arith, s);
+  hipError_t e = launch_zbrgemm_grouped(&b->st1, 1, g[2], 32, 32, 16, g[8], g[8] == 2 ? 1 : 0, arith, s);
   if (e == hipSuccess) e = launch_zmix_grouped(&b->mix, 1, g[3], s);
-  if (e == hipSuccess) e = launch_zbrgemm_grouped(&b->st3, 1, g[4], 32, 32, 16, 2, 2, arith, s);
+  if (e == hipSuccess) e = launch_zbrgemm_grouped(&b->st3, 1, g[4], 32, 32, 16, g[9], g[9] == 2 ? 2 : 0, arith, s);
   if (e == hipSuccess) e = launch_zsum_grouped(&b->sum, 0, g[5], s, &b->ctl.n_sum);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(dots_ctl_partial, dim3(nb, ma), dim3(kT), 0, s, &b->ctl, 1, nb, lz->partial);
@@ -287,7 +287,7 @@ int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x) {
     d.sBk = h->d; d.sBn = (int64_t)h->d * h->Dk; d.sBb = 1;
     d.sCm = 1; d.sCn = (int64_t)h->Db * h->wl * h->d; d.sCb = (int64_t)h->Db * h->wl;
     d.a_mfast = 1; d.b_kfast = 1; d.arith = arith;
-    resolve_group(d);
+    choose_group(d, lz->ctx->tile_override.nbw);
     b.st1.gm = (d.M + 31) / 32; b.st1.gn = (d.N + 31) / 32; b.st1.gz = (d.NB + d.group - 1) / d.group;
     b.st1.ntiles = b.st1.gm * b.st1.gn * b.st1.gz; b.st1.tile_begin = 0;
   }
@@ -317,7 +317,7 @@ int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x) {
       t.sCm = h->d; t.sCn = (int64_t)h->d * h->Db; t.sCb = 1;
       t.beta = h->offset;
     }
-    resolve_group(t);
+    choose_group(t, lz->ctx->tile_override.nbw);
     b.st3.gm = (t.M + 31) / 32; b.st3.gn = (t.N + 31) / 32; b.st3.gz = (t.NB + t.group - 1) / t.group;
     b.st3.ntiles = b.st3.gm * b.st3.gn * b.st3.gz; b.st3.tile_begin = 0;
   }
@@ -325,9 +325,10 @@ int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x) {
   lz->n = n;
   lz->jn = 0;
   auto cap = [](int64_t v, int hi) { return (int)std::max<int64_t>(1, std::min<int64_t>(v, hi)); };
-  lz->grids = GridKey{arith, 0, cap(b.st1.ntiles, kPersist), cap(((int64_t)b.mix.Bx * b.mix.Ax + kT - 1) / kT, kPersist),
-                      cap(b.st3.ntiles, kPersist), split ? cap((ny + kT - 1) / kT, kPersist) : 1,
-                      cap((n + kT - 1) / kT, kVecBlocks), dot_blocks(n)};
+  const int p1 = b.st1.desc.group == 1 ? 3 * kPersist : kPersist, p3 = b.st3.desc.group == 1 ? 3 * kPersist : kPersist;
+  lz->grids = GridKey{arith, 0, cap(b.st1.ntiles, p1), cap(((int64_t)b.mix.Bx * b.mix.Ax + kT - 1) / kT, kPersist),
+                      cap(b.st3.ntiles, p3), split ? cap((ny + kT - 1) / kT, kPersist) : 1,
+                      cap((n + kT - 1) / kT, kVecBlocks), dot_blocks(n), b.st1.desc.group, b.st3.desc.group};
   rc = upload(lz);
   if (rc) return rc;
   // V_0 = x / |x|, |x|^2 into ab[0..1]

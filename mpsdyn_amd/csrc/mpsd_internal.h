@@ -60,6 +60,7 @@ struct GroupedElem {
 // Tile configuration of a BRGEMM launch (chosen by the tuner or by heuristics).
 struct TileCfg {
   int tm = 0, tn = 0, tk = 16;  // tm = tn = 0: built-in choice
+  int nbw = 0;                  // batches per workgroup: 0 built-in (choose_group), 1, 2
 };
 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
@@ -68,6 +69,9 @@ hipError_t launch_zbrgemm_bf16x3(const ZgemmDesc& p, int nbw, hipStream_t s);
 // Resolve a descriptor's batch group (sets group strides as launch_zbrgemm_f64 would) and
 // return the batches per workgroup it uses.
 int resolve_group(ZgemmDesc& p);
+// The batch group every launch path uses for a descriptor with no explicit group (cfg_nbw:
+// 0 built-in policy, 1 one batch per workgroup), then resolve_group.
+int choose_group(ZgemmDesc& p, int cfg_nbw);
 
 // Grouped persistent BRGEMM over nprob problems whose table lives in device memory; all
 // problems share (tm, tn, tk, nbw, ASH/BSH kind, arith).  kind: 0 = none shared,

@@ -2,8 +2,10 @@
 //
 // zbrgemm_f64: batch-reduce complex GEMM on v_mfma_f64_16x16x4f64.  A workgroup of four
 // 64-wide waves owns a TM x TN output tile for NBW batches; operand tiles are staged
-// through LDS as interleaved (re, im) pairs and every complex product is four real MFMAs
-// (re*re, -im*im into the real accumulator; re*im, im*re into the imaginary one).
+// through LDS as interleaved (re, im) pairs.  A complex 16x16x4 block costs three real
+// MFMAs in FP64 (the three-product form: re*re, im*im and (re+im)*(re+im), combined in the
+// epilogue) and four in FP32 (re*re, -im*im into the real accumulator; re*im, im*re into
+// the imaginary one).
 // The global->register load of tile t+1 is issued before the MFMAs of tile t, so HBM/L2
 // latency hides behind the matrix cores.  When one operand is shared by all batches of
 // the workgroup (the environment in stage 1, the right environment in stage 3) it is
@@ -75,11 +77,18 @@ __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, in
   tn_i = rr / gsize;
 }
 
-// LDS operand tiles of one workgroup.
+// LDS operand tiles of one workgroup, [k][m] and [k][n].  Each row carries one element of
+// padding: when the operand is k-fast in HBM, consecutive threads store consecutive k, and
+// an unpadded 32-element row (512 B) would put all of them in the same banks.
+// Tiles whose single stage fits in 40 KB are double-buffered (one barrier per K stage: the
+// next stage is stored into the other buffer while this one is read).
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
 struct Smem {
-  typename Pol<ARITH>::T2 As[ASH ? 1 : NBW][TK][TM];
-  typename Pol<ARITH>::T2 Bs[BSH ? 1 : NBW][TK][TN];
+  typedef typename Pol<ARITH>::T2 T2;
+  static constexpr int kStage = ((ASH ? 1 : NBW) * TM + (BSH ? 1 : NBW) * TN + 2) * TK * (int)sizeof(T2);
+  static constexpr int NBUF = kStage <= 40960 ? 2 : 1;
+  T2 As[NBUF][ASH ? 1 : NBW][TK][TM + 1];
+  T2 Bs[NBUF][BSH ? 1 : NBW][TK][TN + 1];
 };
 
 // One TM x TN output tile (for NBW batches of group z) of the BRGEMM described by p.  The
@@ -94,8 +103,7 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
   typedef typename P::Acc Acc;
   constexpr int NA = ASH ? 1 : NBW;
   constexpr int NB_ = BSH ? 1 : NBW;
-  auto& As = sm.As;
-  auto& Bs = sm.Bs;
+  constexpr int NBUF = Smem<TM, TN, TK, NBW, ASH, BSH, ARITH>::NBUF;
 
   constexpr int WTM = TM / 2, WTN = TN / 2;  // 2x2 waves
   constexpr int BM = WTM / 16, BN = WTN / 16;
@@ -109,17 +117,20 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
   const int wm = wave & 1, wn = wave >> 1;
   const int nbw = min(NBW, p.NB - z * NBW);
 
-  Acc accr[NBW][BM][BN];
-  Acc acci[NBW][BM][BN];
+  // FP64 takes the three-product form of the complex product (P1 = ar*br, P2 = ai*bi,
+  // P3 = (ar+ai)*(br+bi); re = P1 - P2, im = P3 - P1 - P2): three MFMA chains per block
+  // instead of four.  FP32 keeps the four-product form (no cancellation in im).
+  constexpr bool G3 = (ARITH == 0);
+  constexpr int NP = G3 ? 3 : 2;
+  Acc acc[NP][NBW][BM][BN];
 #pragma unroll
-  for (int j = 0; j < NBW; ++j)
+  for (int c = 0; c < NP; ++c)
 #pragma unroll
-    for (int i = 0; i < BM; ++i)
+    for (int j = 0; j < NBW; ++j)
 #pragma unroll
-      for (int q = 0; q < BN; ++q) {
-        accr[j][i][q] = Acc{0, 0, 0, 0};
-        acci[j][i][q] = Acc{0, 0, 0, 0};
-      }
+      for (int i = 0; i < BM; ++i)
+#pragma unroll
+        for (int q = 0; q < BN; ++q) acc[c][j][i][q] = Acc{0, 0, 0, 0};
 
   double2 ra[NA][AE];
   double2 rb[NB_][BE];
@@ -179,7 +190,9 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
     }
   };
 
-  auto store_tile = [&]() {
+  auto store_tile = [&](int buf) {
+    auto& As = sm.As[buf];
+    auto& Bs = sm.Bs[buf];
     const double csign = p.conjA ? -1.0 : 1.0;
 #pragma unroll
     for (int j = 0; j < NA; ++j)
@@ -199,17 +212,29 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
       }
   };
 
-  if (ntiles > 0) load_tile(0);
-  for (int t = 0; t < ntiles; ++t) {
-    store_tile();
+  if (NBUF == 2 && ntiles > 0) {
+    load_tile(0);
+    store_tile(0);
     __syncthreads();
+  } else if (ntiles > 0) {
+    load_tile(0);
+  }
+  for (int t = 0; t < ntiles; ++t) {
+    const int cur = NBUF == 2 ? (t & 1) : 0;
+    if constexpr (NBUF == 1) {
+      store_tile(0);
+      __syncthreads();
+    }
     if (t + 1 < ntiles) load_tile(t + 1);
+    auto& As = sm.As[cur];
+    auto& Bs = sm.Bs[cur];
 #pragma unroll
     for (int kk = 0; kk < TK / 4; ++kk) {
       const int krow = kk * 4 + (lane >> 4);
       const int lc = lane & 15;
-      T ar[NA][BM], ai[NA][BM], an[NA][BM];
-      T br[NB_][BN], bi[NB_][BN];
+      // a3/b3: re + im for the three-product form, -im for the four-product form
+      T ar[NA][BM], ai[NA][BM], a3[NA][BM];
+      T br[NB_][BN], bi[NB_][BN], b3[NB_][BN];
 #pragma unroll
       for (int j = 0; j < NA; ++j)
 #pragma unroll
@@ -217,7 +242,7 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
           const T2 v = As[j][krow][wm * WTM + i * 16 + lc];
           ar[j][i] = v.x;
           ai[j][i] = v.y;
-          an[j][i] = -v.y;
+          a3[j][i] = G3 ? v.x + v.y : -v.y;
         }
 #pragma unroll
       for (int j = 0; j < NB_; ++j)
@@ -226,6 +251,7 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
           const T2 v = Bs[j][krow][wn * WTN + q * 16 + lc];
           br[j][q] = v.x;
           bi[j][q] = v.y;
+          if constexpr (G3) b3[j][q] = v.x + v.y;
         }
 #pragma unroll
       for (int j = 0; j < NBW; ++j) {
@@ -235,18 +261,29 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
         for (int i = 0; i < BM; ++i)
 #pragma unroll
           for (int q = 0; q < BN; ++q) {
-            accr[j][i][q] = P::mfma(ar[ja][i], br[jb][q], accr[j][i][q]);
-            acci[j][i][q] = P::mfma(ar[ja][i], bi[jb][q], acci[j][i][q]);
+            if constexpr (G3) {
+              acc[0][j][i][q] = P::mfma(ar[ja][i], br[jb][q], acc[0][j][i][q]);
+              acc[1][j][i][q] = P::mfma(ai[ja][i], bi[jb][q], acc[1][j][i][q]);
+              acc[2][j][i][q] = P::mfma(a3[ja][i], b3[jb][q], acc[2][j][i][q]);
+            } else {
+              acc[0][j][i][q] = P::mfma(ar[ja][i], br[jb][q], acc[0][j][i][q]);
+              acc[1][j][i][q] = P::mfma(ar[ja][i], bi[jb][q], acc[1][j][i][q]);
+            }
           }
+        if constexpr (!G3) {
 #pragma unroll
-        for (int i = 0; i < BM; ++i)
+          for (int i = 0; i < BM; ++i)
 #pragma unroll
-          for (int q = 0; q < BN; ++q) {
-            accr[j][i][q] = P::mfma(an[ja][i], bi[jb][q], accr[j][i][q]);
-            acci[j][i][q] = P::mfma(ai[ja][i], br[jb][q], acci[j][i][q]);
-          }
+            for (int q = 0; q < BN; ++q) {
+              acc[0][j][i][q] = P::mfma(a3[ja][i], bi[jb][q], acc[0][j][i][q]);
+              acc[1][j][i][q] = P::mfma(ai[ja][i], br[jb][q], acc[1][j][i][q]);
+            }
+        }
       }
     }
+    // double buffer: stage t+1 goes into the other buffer; the barrier makes it visible and
+    // retires every read of this one before it is overwritten two stages later
+    if (NBUF == 2 && t + 1 < ntiles) store_tile(cur ^ 1);
     __syncthreads();
   }
 
@@ -267,7 +304,13 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
           const int gm = m0 + wm * WTM + i * 16 + P::row(lane, rr);
           if (gm < p.M && gn < p.N) {
             const int64_t off = gm * p.sCm + gn * p.sCn;
-            double2 v{(double)accr[j][i][q][rr], (double)acci[j][i][q][rr]};
+            double2 v;
+            if constexpr (G3) {
+              const double p1 = acc[0][j][i][q][rr], p2 = acc[1][j][i][q][rr], p3 = acc[2][j][i][q][rr];
+              v = double2{p1 - p2, p3 - (p1 + p2)};
+            } else {
+              v = double2{(double)acc[0][j][i][q][rr], (double)acc[1][j][i][q][rr]};
+            }
             if (db) {
               const double2 x = db[off];
               v.x = fma(p.beta, x.x, v.x);
@@ -280,8 +323,16 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
   }
 }
 
+// Register budget of the 32x32x16 tile: FP64 MFMAs of one wave issue far apart, so the
+// matrix core needs several resident waves per SIMD.  Two batches per workgroup (768 tiles
+// on 256 CUs for the D=512 stages) are held to three waves per SIMD, one batch per
+// workgroup (twice the tiles) to six.  Larger tiles are left to the compiler.
+constexpr int min_waves(int tm, int tn, int tk, int nbw) {
+  return (tm == 32 && tn == 32 && tk == 16) ? (nbw == 1 ? 6 : 3) : 1;
+}
+
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(min_waves(TM, TN, TK, NBW)))) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
   __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
   int tm_i, tn_i, z;
   tile_of(gm, gn, tm_i, tn_i, z);
@@ -291,7 +342,7 @@ __global__ void __launch_bounds__(kThreads) zbrgemm_kernel(const ZgemmDesc p, in
 // Grouped persistent form: the workgroups of one launch stride over the concatenated tile
 // lists of all problems (problem q owns tiles [tile_begin[q], tile_begin[q+1]) ).
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
-__global__ void __launch_bounds__(kThreads) zbrgemm_grouped_kernel(const GroupedProblem* __restrict__ probs, int nprob,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(min_waves(TM, TN, TK, NBW)))) zbrgemm_grouped_kernel(const GroupedProblem* __restrict__ probs, int nprob,
                                                                    const int* __restrict__ nprob_dev) {
   __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
   if (nprob_dev) nprob = *nprob_dev;
@@ -343,6 +394,19 @@ hipError_t launch_tk(const ZgemmDesc& p, TileCfg cfg, int nbw, hipStream_t s) {
 
 }  // namespace
 
+int choose_group(ZgemmDesc& p, int cfg_nbw) {
+  // One batch per workgroup on request (twice the workgroups, six waves per SIMD for the
+  // FP64 32x32x16 tile); by default two batches share the common operand's LDS tile.  The
+  // per-element sums are the same either way.
+  if (p.group == 0 && p.NB >= 2 && cfg_nbw == 1) {
+    p.group = 1;
+    p.sAz = p.sAb;
+    p.sBz = p.sBb;
+    p.sCz = p.sCb;
+  }
+  return resolve_group(p);
+}
+
 int resolve_group(ZgemmDesc& p) {
   // batches per workgroup: an explicit group, else two when one operand is shared by all
   if (p.group > 0) return (p.group > 2 || p.NB % p.group != 0) ? -1 : p.group;
@@ -389,10 +453,14 @@ hipError_t launch_zbrgemm_grouped(const GroupedProblem* d, int n, int grid, int 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& pin, TileCfg cfg, hipStream_t s) {
   if (pin.M <= 0 || pin.N <= 0 || pin.NB <= 0) return hipSuccess;
   ZgemmDesc p = pin;
-  const int nbw = resolve_group(p);
+  const int nbw = choose_group(p, cfg.nbw);
   if (nbw < 0) return hipErrorInvalidValue;
   if (p.arith == 2) return launch_zbrgemm_bf16x3(p, nbw, s);  // fixed 32x32 tiles
-  if (cfg.tm == 0 || cfg.tn == 0) {
+  if ((cfg.tm == 0 || cfg.tn == 0) && p.arith == 0) {
+    // FP64: 32x32 on every shape measured (D = 128..1024, w = 3, 5); larger FP64 tiles hold
+    // one or two waves per SIMD, too few to keep the FP64 MFMA issuing.
+    cfg.tm = 32, cfg.tn = 32;
+  } else if (cfg.tm == 0 || cfg.tn == 0) {
     // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.
     const int64_t zb = (p.NB + nbw - 1) / nbw;
     auto wgs = [&](int tm, int tn) { return (int64_t)((p.M + tm - 1) / tm) * ((p.N + tn - 1) / tn) * zb; };

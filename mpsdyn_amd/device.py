@@ -26,6 +26,7 @@ PREC_BF16X3 = 2
 OPT_TILE_K = 1
 OPT_SPLIT_STAGE3 = 2
 OPT_GRAPH = 3
+OPT_BATCH_GROUP = 4
 
 _lib = None
 
