@@ -138,6 +138,34 @@ int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t 
  * alone.  No counterpart in the reference (one solve at a time there). */
 int mpsd_heff_apply_grouped(mpsd_ctx* ctx, int n, mpsd_heff* const* h, const double* const* x,
                             double* const* y);
+
+/* ---- tile autotuner -------------------------------------------------------------------------
+ * Times the BRGEMM launches of one operation under each candidate tile configuration (output
+ * tile 32x32 / 64x32 / 32x64 / 64x64, K stage 16 / 32, one or two batches per workgroup) and
+ * keeps the fastest per BRGEMM shape class on the context; later launches of that shape class
+ * use it.  Every candidate accumulates each output element in the same order, so results are
+ * bit-identical with and without tuning.  An explicit mpsd_ctx_set_tile / tile option takes
+ * precedence over the table.  No counterpart in the reference (its GEMMs are BLAS calls).
+ *   tune_heff: reps matvecs y = Heff x per candidate (x, y device vectors; y ends holding Heff x)
+ *   tune_env : reps environment updates per candidate (side 0 = left, 1 = right; arguments as
+ *              mpsd_env_left / mpsd_env_right)
+ * out (optional) receives one row per shape class, *n_out the rows written. */
+typedef struct mpsd_tune_result {
+  int M, N, K, NB, arith, kind; /* shape class; kind = MPSD_KERNEL_* of the launch */
+  int tm, tn, tk, nbw;          /* chosen configuration (tm = tn = 0: built-in) */
+  double us_builtin, us_best;   /* average launch time, built-in choice and chosen */
+} mpsd_tune_result;
+int mpsd_tune_heff(mpsd_heff* h, const double* x, double* y, int reps, mpsd_tune_result* out, int max_out,
+                   int* n_out);
+int mpsd_tune_env(mpsd_ctx* ctx, int side, const mpsd_tensor* env, const mpsd_tensor* ket,
+                  const mpsd_tensor* W, const mpsd_tensor* bra, mpsd_tensor* out_env, int reps,
+                  mpsd_tune_result* out, int max_out, int* n_out);
+/* Shape classes in the context's table; clear it; save / load it as text (load returns the
+ * number of entries read and merges them into the table). */
+int mpsd_tune_count(const mpsd_ctx* ctx);
+int mpsd_tune_clear(mpsd_ctx* ctx);
+int mpsd_tune_save(const mpsd_ctx* ctx, const char* path);
+int mpsd_tune_load(mpsd_ctx* ctx, const char* path);
 /* Dense Heff (size x size, column-major) into a host buffer.  Replaces
  * TensorMultiplier::getFullTensor (TensorMultiplier.cpp:102-117) / the Offset variant
  * (TensorMultiplierOffset.cpp:21-25), used by Contractor::solveEigs below 801

@@ -50,6 +50,7 @@ static hipEvent_t pool_event(mpsd_ctx* ctx) {
 }
 
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch) {
+  ctx->cur_kind = kind;
   if (!ctx->timing) return launch();
   hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
   (void)hipEventRecord(a, ctx->stream);
@@ -142,7 +143,7 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
   g.b_kfast = 1;
   g.arith = arith;
   hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
-                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+                       [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
   if (e != hipSuccess) return hip_err(ctx, e, "stage1 launch");
   e = timed(ctx, MPSD_KERNEL_MIX, [&] {
     return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Db,
@@ -289,12 +290,18 @@ int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches
   return rc;
 }
 
+static void update_tile_user(mpsd_ctx* ctx) {
+  const mpsd::TileCfg& t = ctx->tile_override;
+  ctx->tile_user = !(t.tm == 0 && t.tn == 0 && t.tk == 16 && t.nbw == 0);
+}
+
 int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn) {
   if (!ctx) return MPSD_ERR_ARG;
   const bool ok = (tm == 0 && tn == 0) || ((tm == 32 || tm == 64) && (tn == 32 || tn == 64));
   if (!ok) return set_err(ctx, MPSD_ERR_ARG, "set_tile: tm, tn must be 32 or 64 (or both 0)");
   ctx->tile_override.tm = tm;
   ctx->tile_override.tn = tn;
+  update_tile_user(ctx);
   return MPSD_OK;
 }
 
@@ -315,6 +322,7 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_TILE_K:
       if (value != 16 && value != 32) return set_err(ctx, MPSD_ERR_ARG, "set_option: tile k must be 16 or 32");
       ctx->tile_override.tk = value;
+      update_tile_user(ctx);
       return MPSD_OK;
     case MPSD_OPT_SPLIT_STAGE3:
       if (value < -1 || value > 1) return set_err(ctx, MPSD_ERR_ARG, "set_option: split must be -1, 0 or 1");
@@ -326,6 +334,7 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");
       ctx->tile_override.nbw = value;
+      update_tile_user(ctx);
       return MPSD_OK;
     default:
       return set_err(ctx, MPSD_ERR_ARG, "set_option: unknown option");
@@ -494,7 +503,7 @@ int mpsd_env_left(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* ket
   g.b_kfast = 1;
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_ENV_STAGE3,
-                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); }),
+                       [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); }),
                  "env_left stage3");
 }
 
@@ -547,7 +556,7 @@ int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ke
   g.a_mfast = 1;
   g.b_kfast = 0;
   hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
-                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+                       [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
   if (e != hipSuccess) return hip_err(ctx, e, "env_right stage1");
   // U2[c, s', al, a] = sum_{be,s} W[s', al, s, be] U1[c, be, s, a]
   e = timed(ctx, MPSD_KERNEL_MIX, [&] {
@@ -578,7 +587,7 @@ int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ke
   h.b_kfast = 1;
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_ENV_STAGE3,
-                       [&] { return launch_zbrgemm_f64(h, ctx->tile_override, ctx->stream); }),
+                       [&] { return launch_zbrgemm_f64(h, tile_for(ctx, h), ctx->stream); }),
                  "env_right stage3");
 }
 
@@ -683,7 +692,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
     g.b_kfast = 0;
     g.arith = arith;
     hipError_t e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
-                         [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); });
+                         [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
     if (e != hipSuccess) return hip_err(ctx, e, "heff stage3 split");
     e = timed(ctx, MPSD_KERNEL_MIX, [&] {
       return launch_zsum_parts_f64(P, h->wr, ny, h->offset != 0.0 ? x : nullptr, h->offset, y, ny, ctx->stream);
@@ -717,7 +726,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   }
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
-                       [&] { return launch_zbrgemm_f64(g, ctx->tile_override, ctx->stream); }),
+                       [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); }),
                  "heff stage3");
 }
 

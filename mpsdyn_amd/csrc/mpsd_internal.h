@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "mpsd.h"
@@ -64,6 +66,16 @@ struct TileCfg {
 };
 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& p, TileCfg cfg, hipStream_t s);
+
+// Shape class of a BRGEMM launch: everything its fastest tile configuration depends on
+// (sizes, operand layouts, sharing, arithmetic), none of the addresses.
+struct GemmKey {
+  int M = 0, N = 0, K = 0, R = 0, NB = 0, flags = 0;
+  bool operator<(const GemmKey& o) const {
+    return std::tie(M, N, K, R, NB, flags) < std::tie(o.M, o.N, o.K, o.R, o.NB, o.flags);
+  }
+};
+GemmKey gemm_key(const ZgemmDesc& g);
 hipError_t launch_zbrgemm_bf16x3(const ZgemmDesc& p, int nbw, hipStream_t s);
 
 // Resolve a descriptor's batch group (sets group strides as launch_zbrgemm_f64 would) and
@@ -130,6 +142,13 @@ struct mpsd_ctx {
   size_t stage_bytes[2] = {0, 0};
   int64_t tensor_bytes = 0;
   mpsd::TileCfg tile_override{0, 0, 16};
+  bool tile_user = false;  // tile_override set away from the built-in choice
+  // Autotuner: fastest configuration per BRGEMM shape class (mpsd_tune_*), and, while a
+  // tuning run times one candidate, that candidate plus the (kind, shape) pairs it launched.
+  std::map<mpsd::GemmKey, mpsd::TileCfg> tuned;
+  const mpsd::TileCfg* tune_cand = nullptr;
+  std::vector<std::pair<int, mpsd::GemmKey>> tune_seen;
+  int cur_kind = -1;  // kind of the launch being issued (set by timed)
   int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
   bool graph = true;  // eigensolver steps as replayed hipGraphs where the Heff allows it
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
@@ -176,6 +195,10 @@ int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes);
 // Whether the matvec's last stage is split over the MPO bond (a function of the shape and
 // the context option only, so every launch path takes the same arithmetic).
 bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx);
+// Tile configuration of a BRGEMM launch: the candidate of a running tuning pass, else an
+// explicit context override, else the tuned entry of g's shape class, else built-in.  Every
+// configuration gives bit-identical results, so the choice only moves time.
+TileCfg tile_for(mpsd_ctx* ctx, const ZgemmDesc& g);
 // Run `launch` on the context's stream, bracketed by HIP events when timing is on.
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch);
 int flush_timing(mpsd_ctx* ctx);

@@ -113,6 +113,12 @@ def _optional(lib):
         "mpsd_ctx_set_tile": (I, [P, I, I]),
         "mpsd_ctx_set_option": (I, [P, I, I]),
         "mpsd_ctx_get_option": (I, [P, I]),
+        "mpsd_tune_heff": (I, [P, P, P, I, C.POINTER(TuneResult), I, C.POINTER(I)]),
+        "mpsd_tune_env": (I, [P, I, P, P, P, P, P, I, C.POINTER(TuneResult), I, C.POINTER(I)]),
+        "mpsd_tune_count": (I, [P]),
+        "mpsd_tune_clear": (I, [P]),
+        "mpsd_tune_save": (I, [P, C.c_char_p]),
+        "mpsd_tune_load": (I, [P, C.c_char_p]),
     }
     for name, (res, args) in extra.items():
         if hasattr(lib, name):
@@ -122,6 +128,15 @@ def _optional(lib):
 
 
 MODEL_ISING, MODEL_HEISENBERG, MODEL_SCHWINGER = 0, 1, 2
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("M", C.c_int), ("N", C.c_int), ("K", C.c_int), ("NB", C.c_int), ("arith", C.c_int),
+                ("kind", C.c_int), ("tm", C.c_int), ("tn", C.c_int), ("tk", C.c_int), ("nbw", C.c_int),
+                ("us_builtin", C.c_double), ("us_best", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class GsConfig(C.Structure):
@@ -314,6 +329,31 @@ class Context:
     def set_option(self, option, value):
         self.check(self.lib.mpsd_ctx_set_option(self._h, option, value))
 
+    # tile autotuner table (mpsd_tune_*)
+    def tune_env(self, side, env, ket, W, bra, out, reps=5):
+        rows = (TuneResult * 16)()
+        n = C.c_int()
+        self.check(self.lib.mpsd_tune_env(self._h, int(side), env.handle, ket.handle, W.handle, bra.handle,
+                                          out.handle, int(reps), rows, 16, C.byref(n)))
+        return [rows[i].as_dict() for i in range(n.value)]
+
+    def _count(self, rc):
+        if rc < 0:
+            raise MpsdError(rc, self.lib.mpsd_last_error(self._h).decode())
+        return rc
+
+    def tune_count(self):
+        return self._count(self.lib.mpsd_tune_count(self._h))
+
+    def tune_clear(self):
+        self.check(self.lib.mpsd_tune_clear(self._h))
+
+    def tune_save(self, path):
+        self.check(self.lib.mpsd_tune_save(self._h, str(path).encode()))
+
+    def tune_load(self, path):
+        return self._count(self.lib.mpsd_tune_load(self._h, str(path).encode()))
+
 
 class Tensor:
     """HBM-resident complex128 tensor (mpsd_tensor)."""
@@ -429,6 +469,14 @@ class Heff:
     def _out_len(self):
         L, W, R = self._keep
         return W.dims[0] * L.dims[0] * R.dims[0]
+
+    def tune(self, x_ptr, y_ptr, reps=5):
+        """Autotune this matvec's BRGEMM tiles (device vectors x, y); returns one row per
+        shape class with the chosen configuration and its timings."""
+        rows = (TuneResult * 16)()
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.mpsd_tune_heff(self._h, x_ptr, y_ptr, int(reps), rows, 16, C.byref(n)))
+        return [rows[i].as_dict() for i in range(n.value)]
 
     def eigs_lowest(self, x_tensor, tol=1e-10, max_krylov=30):
         """Lowest eigenpair with the device Lanczos solver; x_tensor holds the start vector
