@@ -78,6 +78,25 @@ def cpu_baseline(L, W, R, x, flops, seconds):
         orc.heff_product(L, W, R, x)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    # one-thread figure on a short sample (5 matvecs)
+    single = None
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1, user_api="blas"):
+            t1 = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                orc.heff_product(L, W, R, x)
+                t1.append(time.perf_counter() - t0)
+        single = flops / float(np.median(t1)) / 1e12
+    except Exception:
+        pass
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": flops / med / 1e12,
         "unit": "TFLOP/s",
@@ -85,6 +104,9 @@ def cpu_baseline(L, W, R, x, flops, seconds):
         "kind": "port",
         "sample": f"{len(times)} matvecs at the same shape (median {med * 1e3:.1f} ms each), numpy oracle "
                   f"with the reference's reshape/permute/ZGEMM order, OpenBLAS {threads} threads",
+        "single_thread_value": round(single, 5) if single else None,
+        "cpu_model": model,
+        "nproc": os.cpu_count(),
     }
 
 
@@ -285,6 +307,16 @@ def main():
     if traffic and traffic.get("workload", "").find(f"D={D}, w={w},") >= 0:
         dom_traffic = traffic["kernels"].get(dom, {}).get("bytes_per_launch")
 
+    # the same matvec with host buffers (x, y copied over PCIe each call): not the bench value
+    xh = np.ascontiguousarray(x)
+    for _ in range(3):
+        h.apply(xh)
+    n_host = 20
+    th0 = time.perf_counter()
+    for _ in range(n_host):
+        h.apply(xh)
+    host_ms = 1e3 * (time.perf_counter() - th0) / n_host
+
     # environment updates at the same shape (one of each per site in a sweep)
     env = {}
     for side, fn in (("left", dev.env_left), ("right", dev.env_right)):
@@ -332,6 +364,8 @@ def main():
                          "tile": [r["tm"], r["tn"], r["tk"], r["nbw"]], "us_builtin": round(r["us_builtin"], 2),
                          "us_best": round(r["us_best"], 2)} for r in tuned],
         "env_update": {k: round(v, 4) for k, v in env.items()} | {"env_left_tflops": round(env_tflops, 3)},
+        "host_buffer_matvec": {"ms": round(host_ms, 4), "tflops": round(F / (host_ms * 1e-3) / 1e12, 3),
+                               "note": "x and y in pageable host memory, PCIe both ways per call"},
         "gs": gs,
         "batched": batched,
         "served": served,

@@ -171,6 +171,20 @@ int mpsd_tune_load(mpsd_ctx* ctx, const char* path);
  * (TensorMultiplierOffset.cpp:21-25), used by Contractor::solveEigs below 801
  * (src/core/cpp/Contractor.cpp:3126-3133). */
 int mpsd_heff_full(mpsd_heff* h, double* host_out);
+/* Hermitian form: on = 1 makes the handle apply (Heff + Heff^H) x / 2 + offset x, with Heff^H
+ * the product over conjugate-transposed copies of L, W, R taken now (call again after changing
+ * them); mpsd_heff_full then returns (H + H^H) / 2 + offset I.  Replaces
+ * TensorMultiplierHermitian(Offset) (src/wrapper/multipliers/TensorMultiplierHermitian.cpp:8-66,
+ * TensorMultiplierHermitianOffset.cpp:8-16, .h:20-24), which findGroundState builds for the
+ * fulleig solver (src/core/cpp/Contractor.cpp:1403-1404).  Needs a square Heff (MPSD_ERR_SHAPE
+ * otherwise, where the reference exits 212).  on = 0 releases the copies. */
+int mpsd_heff_set_hermitian(mpsd_heff* h, int on);
+/* Lowest eigenpair from the dense matrix: mpsd_heff_full's matrix on the device, averaged with
+ * its adjoint, diagonalised by rocSOLVER zheevd; x (device, mpsd_heff_size elements) receives
+ * the eigenvector, *value the eigenvalue (offset included).  Replaces the dense branch of
+ * Contractor::solveEigs (src/core/cpp/Contractor.cpp:3087-3100 for fulleig, :3126-3133 below
+ * LOWEST_PRIMME) with wrapper::eig (src/wrapper/eig.cpp:46-145).  Local size <= 16384. */
+int mpsd_heff_eigs_dense(mpsd_heff* h, double* x, double* value);
 /* Change the arithmetic of an existing handle (MPSD_PREC_*). */
 int mpsd_heff_set_precision(mpsd_heff* h, int precision);
 

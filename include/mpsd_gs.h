@@ -32,7 +32,11 @@ typedef struct mpsd_gs_config {
   int max_halfsweeps; /* <= 0: until converged */
   int precision;      /* MPSD_PREC_* of the matvecs */
   uint64_t seed;      /* random initial MPS */
+  int solver;         /* MPSD_SOLVER_*: local eigensolver (Contractor::setEigenSolver) */
 } mpsd_gs_config;
+
+#define MPSD_SOLVER_LANCZOS 0 /* device Lanczos on the matvec (in place of primme / arpack) */
+#define MPSD_SOLVER_FULLEIG 1 /* fulleig: Hermitian multiplier + dense eigensolver, FP64 */
 
 #define MPSD_GS_MAX_HISTORY 256
 

@@ -241,6 +241,16 @@ class TensorMultiplierOffset : public Multiplier {
   std::vector<int> outdims_;
 };
 
+// TensorMultiplierHermitianOffset (src/wrapper/multipliers/TensorMultiplierHermitianOffset.h,
+// TensorMultiplierHermitian.cpp:8-66): product = (Heff + Heff^H) x / 2 + offset x and
+// getFullTensor = (H + H^H) / 2 + offset I.  The conjugate-transposed operand copies are taken
+// at construction, as in the reference constructor.
+class TensorMultiplierHermitianOffset : public TensorMultiplierOffset {
+ public:
+  TensorMultiplierHermitianOffset(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W, const mpsd_tensor* R,
+                                  double offset, int precision = MPSD_PREC_FP64);
+};
+
 // ------------------------------------------------------------------------------ eigensolver
 // Lowest eigenpair of a Hermitian Multiplier by restarted Lanczos with full (two-pass)
 // reorthogonalisation on device vectors; the tridiagonal Rayleigh-Ritz step is FP64 on the
@@ -261,6 +271,9 @@ class LanczosSolver {
   ~LanczosSolver();
   // x: device vector, initial guess on entry (warm start), eigenvector on exit (unit norm).
   EigStats lowest(Multiplier& H, double* x, double tol, int max_krylov = 30, int max_restarts = 40);
+  // Dense lowest eigenpair (mpsd_heff_eigs_dense): the fulleig solver and the reference's
+  // small-size switch (Contractor.cpp:3087-3100, 3126-3133).
+  EigStats dense(Multiplier& H, double* x);
 
  private:
   EigStats lowest_graph(Multiplier& H, mpsd_heff* h, double* x, double tol, int m, int max_restarts);

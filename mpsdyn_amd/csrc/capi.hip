@@ -634,7 +634,8 @@ int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* 
 
 int mpsd_heff_destroy(mpsd_heff* h) {
   if (!h) return MPSD_ERR_ARG;
-  delete h;  // owns no device memory (the mix matrix belongs to W's cache): no wait needed
+  heff_drop_hermitian(h);  // the Hermitian form's copies (hipFree waits for their last use)
+  delete h;  // otherwise owns no device memory (the mix matrix belongs to W's cache)
   return MPSD_OK;
 }
 
@@ -651,7 +652,7 @@ int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
 }
 
 // One device matvec y = Heff x (+ offset x) on device pointers.
-static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
+static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double offset) {
   mpsd_ctx* ctx = h->ctx;
   int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
   if (rc) return rc;
@@ -695,7 +696,7 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
                          [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
     if (e != hipSuccess) return hip_err(ctx, e, "heff stage3 split");
     e = timed(ctx, MPSD_KERNEL_MIX, [&] {
-      return launch_zsum_parts_f64(P, h->wr, ny, h->offset != 0.0 ? x : nullptr, h->offset, y, ny, ctx->stream);
+      return launch_zsum_parts_f64(P, h->wr, ny, offset != 0.0 ? x : nullptr, offset, y, ny, ctx->stream);
     });
     return hip_err(ctx, e, "heff stage3 sum");
   }
@@ -720,14 +721,47 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   g.a_mfast = 1;
   g.b_kfast = 0;
   g.arith = arith;
-  if (h->offset != 0.0) {
+  if (offset != 0.0) {
     g.Dadd = x;
-    g.beta = h->offset;
+    g.beta = offset;
   }
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_HEFF_STAGE3,
                        [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); }),
                  "heff stage3");
+}
+
+__global__ void __launch_bounds__(256) herm_combine_kernel(double2* __restrict__ y, const double2* __restrict__ yh,
+                                                           const double2* __restrict__ x, double offset, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    // TensorMultiplierHermitian(Offset)::product: result = .5 * (result + aux), then + offset * input
+    double2 v = y[i];
+    const double2 a = yh[i];
+    v.x = 0.5 * (v.x + a.x);
+    v.y = 0.5 * (v.y + a.y);
+    if (offset != 0.0) {
+      const double2 xi = x[i];
+      v.x = fma(offset, xi.x, v.x);
+      v.y = fma(offset, xi.y, v.y);
+    }
+    y[i] = v;
+  }
+}
+
+// One device matvec y = Heff x (+ offset x) on device pointers; the Hermitian form adds the
+// product over the conjugate-transposed operands and averages (TensorMultiplierHermitian.cpp:36-66).
+static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
+  if (!h->herm) return heff_apply_core(h, x, y, h->offset);
+  mpsd_ctx* ctx = h->ctx;
+  h->hh->ctx = ctx;
+  h->hh->precision = h->precision;
+  int rc = heff_apply_core(h, x, y, 0.0);
+  if (!rc) rc = heff_apply_core(h->hh, x, h->htmp, 0.0);
+  if (rc) return rc;
+  const int64_t n = (int64_t)h->d * h->Db * h->Dbr;
+  const unsigned blocks = (unsigned)std::min<int64_t>(2048, (n + 255) / 256);
+  hipLaunchKernelGGL(herm_combine_kernel, dim3(blocks), dim3(256), 0, ctx->stream, y, h->htmp, x, h->offset, n);
+  return hip_err(ctx, hipGetLastError(), "heff hermitian combine");
 }
 
 int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx, int64_t ldy,
@@ -942,8 +976,8 @@ int mpsd_heff_apply_grouped(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const do
     if (!h || !x[i] || !y[i]) return set_err(ctx, MPSD_ERR_ARG, "apply_grouped: null handle or vector");
     int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);
     if (rc) return rc;
-    if (h->precision == MPSD_PREC_BF16X3 || h->d != 2) {
-      // split-BF16 and d != 2 take the single-problem path on this stream
+    if (h->precision == MPSD_PREC_BF16X3 || h->d != 2 || h->herm) {
+      // split-BF16, d != 2 and the Hermitian form take the single-problem path on this stream
       mpsd_ctx* own = h->ctx;
       h->ctx = ctx;
       rc = heff_apply_dev(h, (const double2*)x[i], (double2*)y[i]);
@@ -970,8 +1004,7 @@ int mpsd_heff_full(mpsd_heff* h, double* host_out) {
   const int64_t nr = (int64_t)h->d * h->Db * h->Dbr, nc = mpsd_heff_size(h);
   int rc = ensure_stage(ctx, 0, sizeof(double2) * (size_t)(nr * nc));
   if (rc) return rc;
-  hipError_t e = launch_heff_full_f64(h->L->ptr, h->W->ptr, h->R->ptr, (double2*)ctx->stage[0], h->d,
-                                      h->Db, h->wl, h->Dk, h->Dbr, h->wr, h->Dkr, h->offset, ctx->stream);
+  hipError_t e = heff_full_dev(h, (double2*)ctx->stage[0]);
   if (e == hipSuccess)
     e = hipMemcpyAsync(host_out, ctx->stage[0], sizeof(double2) * (size_t)(nr * nc), hipMemcpyDeviceToHost,
                        ctx->stream);

@@ -38,6 +38,8 @@ int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_p
                   mpsd_gs_result* out) {
   if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
   if (cfg->L < 2 || cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need L >= 2 and D >= 1");
+  if (cfg->solver != MPSD_SOLVER_LANCZOS && cfg->solver != MPSD_SOLVER_FULLEIG)
+    return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: unknown solver");
   memset(out, 0, sizeof(*out));
   try {
     auto H = make_model(cfg);
@@ -49,6 +51,7 @@ int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_p
     c.setEigTol(cfg->eigtol);
     c.setMaxHalfSweeps(cfg->max_halfsweeps);
     c.setPrecision(cfg->precision);
+    if (cfg->solver == MPSD_SOLVER_FULLEIG) c.setEigenSolver(fulleig);
     out->initial_energy = c.contract(mps, H->getHMPO(), mps).real();
     double E = 0.0;
     auto t0 = std::chrono::steady_clock::now();

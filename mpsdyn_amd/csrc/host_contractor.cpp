@@ -83,7 +83,10 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   // phase each site is first solved with the FP32 matvec and then refined with the FP64
   // one from that vector, and only FP64 energies enter the stopping rule and the
   // energy-increase guard.
-  const bool lowp = precision_ != MPSD_PREC_FP64;
+  // fulleig: the Hermitian multiplier and the dense eigensolver, in FP64
+  // (Contractor.cpp:1403-1404, 3087-3100)
+  const bool dense = solver_ == fulleig;
+  const bool lowp = precision_ != MPSD_PREC_FP64 && !dense;
   int phase = precision_;  // MPSD_PREC_BF16X3 (2) -> MPSD_PREC_FP32 (1) -> FP64 refinement (0)
   auto low_tol = [](int p) { return p == MPSD_PREC_FP32 ? 1e-6 : 1e-4; };      // local residual
   // relative dE per half-sweep below which a phase ends (or when its energy stops decreasing:
@@ -99,10 +102,15 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     double* xv = (double*)mpsd_tensor_devptr(site.device());
     {
       const int first = lowp ? (refine ? MPSD_PREC_FP32 : phase) : MPSD_PREC_FP64;
-      std::unique_ptr<Multiplier> mp =
-          factory_ ? factory_(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first)
-                   : std::unique_ptr<Multiplier>(new TensorMultiplierOffset(
-                         ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first));
+      std::unique_ptr<Multiplier> mp;
+      if (factory_)
+        mp = factory_(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first);
+      else if (dense)
+        mp.reset(new TensorMultiplierHermitianOffset(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_),
+                                                     mgr.operR(pos).t, offset, first));
+      else
+        mp.reset(new TensorMultiplierOffset(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t,
+                                            offset, first));
       Multiplier& multi = *mp;
       if (lowp) {
         EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol(first)));
@@ -113,7 +121,7 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
       }
       if (refine) {
         if (lowp) multi.setPrecision(MPSD_PREC_FP64);
-        EigStats es = eig_->lowest(multi, xv, tol);
+        EigStats es = dense ? eig_->dense(multi, xv) : eig_->lowest(multi, xv, tol);
         stats_.matvecs += es.matvecs;
         stats_.final_residual = es.residual;
         val = es.value - offset;

@@ -29,6 +29,13 @@ TensorMultiplierOffset::~TensorMultiplierOffset() {
   if (h_) mpsd_heff_destroy(h_);
 }
 
+TensorMultiplierHermitianOffset::TensorMultiplierHermitianOffset(mpsd_ctx* ctx, const mpsd_tensor* L,
+                                                                 const mpsd_tensor* W, const mpsd_tensor* R,
+                                                                 double offset, int precision)
+    : TensorMultiplierOffset(ctx, L, W, R, offset, precision) {
+  check(ctx, mpsd_heff_set_hermitian(handle(), 1), "heff_set_hermitian");
+}
+
 int64_t TensorMultiplierOffset::getSize() const { return mpsd_heff_size(h_); }
 
 void TensorMultiplierOffset::product_device(const double* x, double* y) {
@@ -140,6 +147,16 @@ static int lanczos_chunk(int j) { return j < 6 ? 3 : 4; }
 
 LanczosSolver::~LanczosSolver() {
   if (lz_) mpsd_lanczos_destroy(lz_);
+}
+
+EigStats LanczosSolver::dense(Multiplier& H, double* x) {
+  mpsd_heff* h = H.heff();
+  if (!h) throw Error(MPSD_ERR_UNSUPPORTED, "dense eigensolver: the multiplier has no device handle");
+  EigStats st;
+  check(ctx_, mpsd_heff_eigs_dense(h, x, &st.value), "heff_eigs_dense");
+  st.residual = 0.0;
+  st.converged = true;
+  return st;
 }
 
 EigStats LanczosSolver::lowest_graph(Multiplier& H, mpsd_heff* h, double* x, double tol, int m, int max_restarts) {

@@ -172,6 +172,13 @@ static int handle_of(mpsd_ctx* ctx, rocblas_handle* h) {
   return MPSD_OK;
 }
 
+int blas_handle(mpsd_ctx* ctx, void** out) {
+  rocblas_handle h = nullptr;
+  const int rc = handle_of(ctx, &h);
+  *out = h;
+  return rc;
+}
+
 void destroy_blas(mpsd_ctx* ctx) {
   if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
   ctx->blas = nullptr;

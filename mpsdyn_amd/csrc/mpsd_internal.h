@@ -186,9 +186,24 @@ struct mpsd_heff {
   int precision = MPSD_PREC_FP64;
   int d = 0, Db = 0, wl = 0, Dk = 0, Dbr = 0, wr = 0, Dkr = 0;
   double2* mix = nullptr;  // device (d*wr) x (wl*d) mix matrix from W (owned by W's cache)
+  // Hermitian form (mpsd_heff_set_hermitian): y = (H x + H' x) / 2 + offset x, where H' is the
+  // product over conjugate-transposed copies of L, W, R taken when the form is switched on.
+  int herm = 0;
+  mpsd_tensor* Lh = nullptr;
+  mpsd_tensor* Wh = nullptr;
+  mpsd_tensor* Rh = nullptr;
+  mpsd_heff* hh = nullptr;   // H' (offset 0), applied on this handle's context
+  double2* htmp = nullptr;   // H' x, d * Db * Dbr elements
 };
 
 namespace mpsd {
+// Release the Hermitian-form copies of a handle (no-op when the form is off).
+void heff_drop_hermitian(mpsd_heff* h);
+// Dense Heff (column-major, rows = output index) into device memory on h's stream: H + offset I,
+// or (H + H^H) / 2 + offset I for the Hermitian form.
+hipError_t heff_full_dev(mpsd_heff* h, double2* out);
+// The context's rocBLAS handle (created on first use, bound to the context's stream).
+int blas_handle(mpsd_ctx* ctx, void** out);
 int set_err(mpsd_ctx* ctx, int code, const std::string& msg);
 int hip_err(mpsd_ctx* ctx, hipError_t e, const char* where);
 int ensure_scratch(mpsd_ctx* ctx, int slot, size_t bytes);

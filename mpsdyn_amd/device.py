@@ -71,6 +71,8 @@ def load():
         "mpsd_heff_apply": (I, [P, P, P, I, I64, I64, I]),
         "mpsd_heff_full": (I, [P, P]),
         "mpsd_heff_set_precision": (I, [P, I]),
+        "mpsd_heff_set_hermitian": (I, [P, I]),
+        "mpsd_heff_eigs_dense": (I, [P, P, C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -143,7 +145,7 @@ class GsConfig(C.Structure):
     """mpsd_gs_config (include/mpsd_gs.h)."""
     _fields_ = [("model", C.c_int), ("L", C.c_int), ("D", C.c_int), ("p", C.c_double * 4),
                 ("offset", C.c_double), ("convtol", C.c_double), ("eigtol", C.c_double),
-                ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64)]
+                ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64), ("solver", C.c_int)]
 
 
 class GsResult(C.Structure):
@@ -218,11 +220,14 @@ def gs_footprint(cfg):
     return int(load().mpsd_gs_footprint(C.byref(cfg)))
 
 
+SOLVER_LANCZOS, SOLVER_FULLEIG = 0, 1
+
+
 def gs_config(model, L, D, p, convtol=1e-10, eigtol=0.0, max_halfsweeps=0, precision=PREC_FP64,
-              seed=1, offset=0.0):
+              seed=1, offset=0.0, solver=SOLVER_LANCZOS):
     pp = list(p) + [0.0] * (4 - len(p))
     return GsConfig(model, L, D, (C.c_double * 4)(*pp), offset, convtol, eigtol, max_halfsweeps,
-                    precision, seed)
+                    precision, seed, solver)
 
 
 def gs_solve(ctx, cfg, export_path=None):
@@ -469,6 +474,16 @@ class Heff:
     def _out_len(self):
         L, W, R = self._keep
         return W.dims[0] * L.dims[0] * R.dims[0]
+
+    def set_hermitian(self, on=True):
+        self.ctx.check(self.ctx.lib.mpsd_heff_set_hermitian(self._h, 1 if on else 0))
+
+    def eigs_dense(self, x_tensor):
+        """Lowest eigenpair from the dense matrix (rocSOLVER zheevd); x_tensor receives the
+        eigenvector.  Returns the eigenvalue."""
+        v = C.c_double()
+        self.ctx.check(self.ctx.lib.mpsd_heff_eigs_dense(self._h, x_tensor.devptr(), C.byref(v)))
+        return v.value
 
     def tune(self, x_ptr, y_ptr, reps=5):
         """Autotune this matvec's BRGEMM tiles (device vectors x, y); returns one row per

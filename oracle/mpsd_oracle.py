@@ -24,6 +24,9 @@ Pinning (how each function is checked; tests/test_oracle.py):
   contract_left/right -- env of the full chain against <psi|H|psi> from the dense
                        Hamiltonian; explicit loops on tiny shapes.
   heff_full         -- against heff_product on unit vectors and the dense projection.
+  heff_product_hermitian -- against (F + F^H) / 2 applied to x, F = heff_full (the dense
+                       adjoint, independent of the conjugate-permuted copies), on
+                       non-Hermitian operands where F != F^H.
   *_mpo builders    -- MPO contracted to a dense matrix against the Pauli-sum Hamiltonian.
   find_ground_state -- reference energies recorded in tests/golden (config 1 value from
                        the compiled reference), exact diagonalisation and the open-chain
@@ -150,6 +153,19 @@ def heff_full(L, W, R, offset=0.0):
     if offset != 0.0:
         res = res + offset * np.eye(res.shape[0])
     return res
+
+
+def heff_product_hermitian(L, W, R, x, offset=0.0):
+    """(Heff + Heff^H) x / 2 + offset x.  Restates TensorMultiplierHermitian: the constructor's
+    conjugate-permuted copies ALh = conj(permute(L,(3,2,1))), Aoph = conj(permute(W,(3,2,1,4))),
+    ARh = conj(permute(R,(3,2,1))) (TensorMultiplierHermitian.cpp:21-26), ::product's second
+    contraction over them and result = .5*(result+aux) (:36-66), and the offset of
+    TensorMultiplierHermitianOffset::product (TensorMultiplierHermitianOffset.cpp:13-16)."""
+    Lh = np.conj(permute(L, (3, 2, 1)))
+    Wh = np.conj(permute(W, (3, 2, 1, 4)))
+    Rh = np.conj(permute(R, (3, 2, 1)))
+    y = 0.5 * (heff_product(L, W, R, x) + heff_product(Lh, Wh, Rh, x))
+    return y + offset * np.asarray(x).reshape(y.shape, order="F") if offset != 0.0 else y
 
 
 def contract_left(L, ket, W, bra):

@@ -236,6 +236,22 @@ def test_heff_hermitian_and_linear():
 
 # ---------------------------------------------------------------- RNG
 
+@pytest.mark.parametrize("Dl,Dr,wl,wr,offset", [(3, 4, 2, 3, 0.0), (5, 5, 3, 3, 0.75), (1, 6, 1, 3, -1.5)])
+def test_heff_product_hermitian_is_dense_average(Dl, Dr, wl, wr, offset):
+    # non-Hermitian operands (F != F^H), so the conjugate-permuted copies must reproduce the
+    # dense adjoint exactly and the average must differ from the plain product
+    L = orc.complex_gaussian(81, (Dl, wl, Dl))
+    W = orc.complex_gaussian(82, (2, wl, 2, wr))
+    R = orc.complex_gaussian(83, (Dr, wr, Dr))
+    x = orc.complex_gaussian(84, (2 * Dl * Dr,))
+    F = orc.heff_full(L, W, R)
+    assert np.linalg.norm(F - F.conj().T) > 1e-3 * np.linalg.norm(F)
+    want = 0.5 * (F + F.conj().T) @ x + offset * x
+    got = orc.heff_product_hermitian(L, W, R, x, offset)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    assert np.linalg.norm(got - (orc.heff_product(L, W, R, x) + offset * x)) > 1e-3 * np.linalg.norm(want)
+
+
 def test_rng_reproducible_and_normal():
     a = orc.complex_gaussian(7, (4, 5))
     b = orc.complex_gaussian(7, (4, 5))
