@@ -130,6 +130,39 @@ def ground_states(ctx, dev, n):
             "max_rel_energy_error_vs_reference": float(max(errs)), "matvecs_per_solve": mv / n}
 
 
+def forms_table(dev, gaussian):
+    """Matvec time per precision form (FP64, complex FP32, split-BF16) over interior-site
+    shapes: one JSON line per (D, w, form), HIP-event times per kernel kind."""
+    ctx = dev.Context(0)
+    d = 2
+    for D, w in [(512, 3), (512, 5), (400, 5), (256, 3), (128, 3)]:
+        L, W, R = gaussian(1, (D, w, D)), gaussian(2, (d, w, d, w)), gaussian(3, (D, w, D))
+        tL, tW, tR = ctx.tensor(L), ctx.tensor(W), ctx.tensor(R)
+        tx, ty = ctx.tensor(gaussian(4, (d, D, D))), ctx.tensor(dims=(d, D, D))
+        F = sum(matvec_flops(D, D, D, D, w, w, d))
+        for prec, name in ((dev.PREC_FP64, "fp64"), (dev.PREC_FP32, "complex_fp32"), (dev.PREC_BF16X3, "split_bf16x3")):
+            h = dev.Heff(ctx, tL, tW, tR, precision=prec)
+            for _ in range(3):
+                h.apply_device(tx.devptr(), ty.devptr())
+            ctx.sync()
+            ctx.set_timing(True)
+            n = 20
+            t0 = time.perf_counter()
+            for _ in range(n):
+                h.apply_device(tx.devptr(), ty.devptr())
+            ctx.sync()
+            t = (time.perf_counter() - t0) / n
+            us = {}
+            for k, nm in ((0, "stage1_us"), (1, "mix_sum_us"), (2, "stage3_us")):
+                ms, c = ctx.timing(k)
+                us[nm] = round(1e3 * ms / max(c, 1), 1)
+            ctx.set_timing(False)
+            print(json.dumps({"D": D, "w": w, "form": name, "ms": round(t * 1e3, 4),
+                              "tflops": round(F / t / 1e12, 2)} | us), flush=True)
+            h.close()
+    ctx.close()
+
+
 def traffic_probe(dev, gaussian, D, w):
     """Workload of the HBM-traffic counter runs: a device copy of known size (the scale of
     FETCH_SIZE / WRITE_SIZE) followed by 5 matvecs at the benchmark shape."""
@@ -231,6 +264,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2.0)
     ap.add_argument("--full-serving", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner (built-in tiles)")
+    ap.add_argument("--forms-table", action="store_true",
+                    help="only the per-precision-form matvec table (one JSON line per shape and form)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 256 MiB device copy (3x) and 5 matvecs, for rocprofv3 --pmc counter runs")
     args = ap.parse_args()
@@ -245,6 +280,9 @@ def main():
 
     if args.traffic_probe:
         traffic_probe(dev, complex_gaussian, args.D, args.w)
+        return
+    if args.forms_table:
+        forms_table(dev, complex_gaussian)
         return
 
     D, w, d = args.D, args.w, 2
