@@ -74,6 +74,7 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
 #define MPSD_OPT_TILE_K 1       /* K depth of a BRGEMM LDS stage: 16 (default) or 32 */
 #define MPSD_OPT_SPLIT_STAGE3 2 /* matvec last stage split over the MPO bond: -1 auto, 0 off, 1 on */
 #define MPSD_OPT_GRAPH 3        /* eigensolver steps as replayed hipGraphs: 1 (default) or 0 */
+#define MPSD_OPT_GAUGE_QR 5     /* gauge-step QR / LQ: 0 library Householder kernels (default), 1 rocSOLVER */
 #define MPSD_OPT_BATCH_GROUP 4  /* batches per BRGEMM workgroup: 0 built-in (default), 1, or 2 when an
                                    operand is shared by the batches; results are the same for all */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);

@@ -312,6 +312,7 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_SPLIT_STAGE3: return ctx->split3;
     case MPSD_OPT_GRAPH: return ctx->graph ? 1 : 0;
     case MPSD_OPT_BATCH_GROUP: return ctx->tile_override.nbw;
+    case MPSD_OPT_GAUGE_QR: return ctx->gauge_qr;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -330,6 +331,10 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
       return MPSD_OK;
     case MPSD_OPT_GRAPH:
       ctx->graph = value != 0;
+      return MPSD_OK;
+    case MPSD_OPT_GAUGE_QR:
+      if (value != 0 && value != 1) return set_err(ctx, MPSD_ERR_ARG, "set_option: gauge QR must be 0 or 1");
+      ctx->gauge_qr = value;
       return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");

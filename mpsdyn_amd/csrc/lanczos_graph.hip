@@ -24,7 +24,8 @@ namespace {
 
 constexpr int kT = kVecThreads;
 constexpr int kDotBlocks = kDotMaxBlocks;
-constexpr int kPersist = 512;    // most workgroups of a persistent matvec launch (x3 for one-batch FP64 tiles)
+constexpr int kPersist = 512;    // most workgroups of a persistent elementwise launch
+constexpr int kPersistGemm = 768;  // most workgroups of a persistent BRGEMM launch: 3 per CU (x2 one-batch tiles)
 constexpr int kVecBlocks = 512;  // most blocks of a grid-stride vector kernel
 
 // Launch shape of one captured step: grids sized to the bound site's shapes.
@@ -325,7 +326,8 @@ int mpsd_lanczos_begin(mpsd_lanczos* lz, mpsd_heff* h, const double* x) {
   lz->n = n;
   lz->jn = 0;
   auto cap = [](int64_t v, int hi) { return (int)std::max<int64_t>(1, std::min<int64_t>(v, hi)); };
-  const int p1 = b.st1.desc.group == 1 ? 3 * kPersist : kPersist, p3 = b.st3.desc.group == 1 ? 3 * kPersist : kPersist;
+  const int p1 = b.st1.desc.group == 1 ? 2 * kPersistGemm : kPersistGemm;
+  const int p3 = b.st3.desc.group == 1 ? 2 * kPersistGemm : kPersistGemm;
   lz->grids = GridKey{arith, 0, cap(b.st1.ntiles, p1), cap(((int64_t)b.mix.Bx * b.mix.Ax + kT - 1) / kT, kPersist),
                       cap(b.st3.ntiles, p3), split ? cap((ny + kT - 1) / kT, kPersist) : 1,
                       cap((n + kT - 1) / kT, kVecBlocks), dot_blocks(n), b.st1.desc.group, b.st3.desc.group};

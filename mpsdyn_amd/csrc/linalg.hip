@@ -137,6 +137,41 @@ __global__ void __launch_bounds__(kT) unpermute_q_kernel(const double2* __restri
   site[t] = Q[i + (int64_t)ldq * (s + (int64_t)d * c)];
 }
 
+// B[s + d c, a] = conj(site[s, a, c]): the adjoint of the LQ matrix M[a, s + d c] (B: n x Dl).
+__global__ void __launch_bounds__(kT) site_to_bh_kernel(const double2* __restrict__ site, int d, int Dl, int Dr,
+                                                        double2* __restrict__ B) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  const int64_t n = (int64_t)d * Dr;
+  if (t >= n * Dl) return;
+  const int s = (int)(t % d);
+  const int a = (int)((t / d) % Dl);
+  const int64_t c = t / ((int64_t)d * Dl);
+  const double2 v = site[t];
+  B[(s + (int64_t)d * c) + n * a] = double2{v.x, -v.y};
+}
+
+// out[i, j] = conj(in[j, i]) for in (r x c), out (c x r)
+__global__ void __launch_bounds__(kT) conj_transpose_kernel(const double2* __restrict__ in, int r, int c,
+                                                            double2* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)r * c) return;
+  const int i = (int)(t % c), j = (int)(t / c);
+  const double2 v = in[j + (int64_t)r * i];
+  out[t] = double2{v.x, -v.y};
+}
+
+// site[s, i, c] = conj(Q'[s + d c, i]) for Q' (n x k), n = d * Dr
+__global__ void __launch_bounds__(kT) qh_to_site_kernel(const double2* __restrict__ Qp, int n, int k, int d, int Dr,
+                                                        double2* __restrict__ site) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)d * k * Dr) return;
+  const int s = (int)(t % d);
+  const int i = (int)((t / d) % k);
+  const int64_t c = t / ((int64_t)d * k);
+  const double2 v = Qp[(s + (int64_t)d * c) + (int64_t)n * i];
+  site[t] = double2{v.x, -v.y};
+}
+
 // Edge phase of Site::gaugeR/gaugeL (Site.cpp:202-207, 232-237): the 1x1 factor f becomes
 // |f| and its phase f/|f| is multiplied into the site tensor.
 __global__ void __launch_bounds__(kT) edge_phase_kernel(double2* __restrict__ site, int64_t n,
@@ -340,6 +375,25 @@ int mpsd_site_gauge_r(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out) {
   if (!ctx || !site || !R_out || site == R_out) return MPSD_ERR_ARG;
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int m = d * Dl, n = Dr, k = m < n ? m : n;
+  if (ctx->gauge_qr == 0 && m <= house_qr_max_rows()) {
+    // library Householder QR: site (m x n) = Q R, Q (m x k) back into the site
+    int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * k);
+    const int rd[2] = {k, n};
+    if (!rc) rc = mpsd_tensor_reshape(R_out, 2, rd);
+    if (!rc) rc = house_qr(ctx, site->ptr, m, n, (double2*)ctx->scratch[3], R_out->ptr);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(site->ptr, ctx->scratch[3], sizeof(double2) * (size_t)m * k,
+                                  hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "gauge_r copy");
+    const int sd[3] = {d, Dl, k};
+    rc = mpsd_tensor_reshape(site, 3, sd);
+    if (rc) return rc;
+    if (k == 1 && n == 1)
+      hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, R_out->ptr);
+    ++site->version;
+    ++R_out->version;
+    return hip_err(ctx, hipGetLastError(), "gauge_r");
+  }
   rocblas_handle h;
   int rc = handle_of(ctx, &h);
   if (rc) return rc;
@@ -370,6 +424,33 @@ int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out) {
   if (!ctx || !site || !L_out || site == L_out) return MPSD_ERR_ARG;
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int m = Dl, n = d * Dr, k = m < n ? m : n;
+  if (ctx->gauge_qr == 0 && n <= house_qr_max_rows()) {
+    // LQ of M[a, s + d c] = site[s, a, c] as the QR of B = M^H (n x m): M = R'^H Q'^H, so
+    // L = R'^H (m x k) and the site becomes Q'^H reshaped to [d, k, Dr]
+    int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)n * m);
+    if (!rc) rc = ensure_scratch(ctx, 7, sizeof(double2) * (size_t)(n * k + k * m));
+    if (rc) return rc;
+    double2* B = (double2*)ctx->scratch[3];
+    double2* Qp = (double2*)ctx->scratch[7];
+    double2* Rp = Qp + (size_t)n * k;
+    hipLaunchKernelGGL(site_to_bh_kernel, dim3(nblk((int64_t)m * n)), dim3(kT), 0, ctx->stream, site->ptr, d, Dl,
+                       Dr, B);
+    rc = house_qr(ctx, B, n, m, Qp, Rp);
+    const int ld[2] = {m, k};
+    if (!rc) rc = mpsd_tensor_reshape(L_out, 2, ld);
+    const int sd[3] = {d, k, Dr};
+    if (!rc) rc = mpsd_tensor_reshape(site, 3, sd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(conj_transpose_kernel, dim3(nblk((int64_t)m * k)), dim3(kT), 0, ctx->stream, Rp, k, m,
+                       L_out->ptr);
+    hipLaunchKernelGGL(qh_to_site_kernel, dim3(nblk((int64_t)d * k * Dr)), dim3(kT), 0, ctx->stream, Qp, n, k, d, Dr,
+                       site->ptr);
+    if (k == 1 && m == 1)
+      hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, L_out->ptr);
+    ++site->version;
+    ++L_out->version;
+    return hip_err(ctx, hipGetLastError(), "gauge_l");
+  }
   rocblas_handle h;
   int rc = handle_of(ctx, &h);
   if (rc) return rc;

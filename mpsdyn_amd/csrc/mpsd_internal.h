@@ -119,10 +119,11 @@ struct mpsd_ctx {
   std::string err;
   // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
   // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step;
-  // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec.
-  static constexpr int kScratch = 6;
-  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0};
+  // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec; 6 = Householder QR workspace
+  // (V, Z, Y, W); 7 = second gauge buffer.
+  static constexpr int kScratch = 8;
+  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0, 0, 0};
   // Problem tables of grouped launches: device buffer + pinned host staging, reused after
   // the event of the previous upload has completed.
   void* gtab = nullptr;
@@ -151,6 +152,7 @@ struct mpsd_ctx {
   int cur_kind = -1;  // kind of the launch being issued (set by timed)
   int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
   bool graph = true;  // eigensolver steps as replayed hipGraphs where the Heff allows it
+  int gauge_qr = 0;   // gauge-step QR / LQ: 0 the library's Householder kernels, 1 rocSOLVER
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   struct Pending {
@@ -202,6 +204,10 @@ void heff_drop_hermitian(mpsd_heff* h);
 // Dense Heff (column-major, rows = output index) into device memory on h's stream: H + offset I,
 // or (H + H^H) / 2 + offset I for the Hermitian form.
 hipError_t heff_full_dev(mpsd_heff* h, double2* out);
+// Householder QR of A (m x n, column-major, in place) on ctx's stream: Q (m x k) and R
+// (k x n), k = min(m, n), in the LAPACK zgeqrf / zungqr conventions.  m <= house_qr_max_rows().
+int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R);
+int house_qr_max_rows();
 // The context's rocBLAS handle (created on first use, bound to the context's stream).
 int blas_handle(mpsd_ctx* ctx, void** out);
 int set_err(mpsd_ctx* ctx, int code, const std::string& msg);

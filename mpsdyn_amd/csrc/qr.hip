@@ -1,0 +1,447 @@
+// Blocked Householder QR for the gauge steps (the role of wrapper::qr / lq, src/wrapper/qr.cpp:33-188,
+// behind Site::gaugeR / gaugeL, src/core/cpp/Site.cpp:179-238).
+//
+// A (m x n, column-major) is factorised in panels of PB columns: A = Q R,
+// Q = H_0 H_1 ... H_{k-1}, H_i = I - tau_i v_i v_i^H with LAPACK's zgeqrf / zlarfg conventions
+// (v_i(i) = 1, real R diagonal beta = -sign(Re alpha) * ||(alpha, x)||).
+//
+// * One workgroup factorises a panel with the panel held in registers (a thread owns up to
+//   RPT rows); each column costs two workgroup reductions.  It also forms the panel's
+//   compact-WY factor T (zlarft, forward / columnwise) from the Gram products it reduces
+//   alongside, and writes V, Y = V (-T)^H and Z = V (-T).
+// * The trailing columns take C <- (I - V T^H V^H) C = C + Y (V^H C), and Q is accumulated
+//   backwards, Q <- (I - V T V^H) Q = Q + Z (V^H Q).  V^H C (16 rows, long K) is a chunked
+//   product with an ordered chunk sum; the rank-16 update C += Y W is a BRGEMM launch.
+//
+// A panel step is a handful of launches where rocSOLVER's blocked QR issues several per
+// column, which is what the small and medium gauge steps of a sweep are bound by.
+
+#include "mpsd_internal.h"
+
+namespace mpsd {
+namespace {
+
+constexpr int kQT = 256;  // threads of the small-panel kernel (one wave per SIMD: a large register budget)
+constexpr int kPB = 16;   // panel width
+constexpr int kQWmax = 8; // waves of the largest panel workgroup (512 threads)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// conj(a) * b
+__device__ __forceinline__ double2 cjmul(double2 a, double2 b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
+
+// Workgroup sum of NV complex values per thread, fixed order; out[u] visible to all threads
+// on return.
+template <int NV, int NW>
+__device__ __forceinline__ void block_sum(const double2 (&v)[NV], double2 (*red)[NV], double2* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    double x = v[u].x, y = v[u].y;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      x += __shfl_xor(x, o);
+      y += __shfl_xor(y, o);
+    }
+    if (lane == 0) red[wave][u] = double2{x, y};
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double2 s{0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      s.x += red[w][threadIdx.x].x;
+      s.y += red[w][threadIdx.x].y;
+    }
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Workgroup sum of kPB (16) complex values per thread, fixed order, into out[0..16) for all
+// threads.  Within a wave the values are reduced and scattered at once over lane bits 5..2
+// (8 + 4 + 2 + 1 exchanges instead of 16 x 4), then over bits 1..0; lane 4u ends with the
+// wave's sum of value u.
+__device__ __forceinline__ double2 shfl2(double2 v, int o) { return double2{__shfl_xor(v.x, o), __shfl_xor(v.y, o)}; }
+__device__ __forceinline__ double2 add2(double2 a, double2 b) { return double2{a.x + b.x, a.y + b.y}; }
+
+template <int NW>
+__device__ __forceinline__ void block_sum16(const double2 (&v)[kPB], double2 (*red)[kPB], double2* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+  double2 w8[8], w4[4], w2[2];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) w8[u] = add2(b5 ? v[u + 8] : v[u], shfl2(b5 ? v[u] : v[u + 8], 32));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) w4[u] = add2(b4 ? w8[u + 4] : w8[u], shfl2(b4 ? w8[u] : w8[u + 4], 16));
+#pragma unroll
+  for (int u = 0; u < 2; ++u) w2[u] = add2(b3 ? w4[u + 2] : w4[u], shfl2(b3 ? w4[u] : w4[u + 2], 8));
+  double2 w1 = add2(b2 ? w2[1] : w2[0], shfl2(b2 ? w2[0] : w2[1], 4));
+  w1 = add2(w1, shfl2(w1, 2));
+  w1 = add2(w1, shfl2(w1, 1));
+  // this lane now holds value u = 8 b5 + 4 b4 + 2 b3 + b2
+  if ((lane & 3) == 0) red[wave][lane >> 2] = w1;
+  __syncthreads();
+  if (threadIdx.x < kPB) {
+    double2 sm{0.0, 0.0};
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) sm = add2(sm, red[ww][threadIdx.x]);
+    out[threadIdx.x] = sm;
+  }
+  __syncthreads();
+}
+
+// Workgroup maximum of one non-negative value per thread; the result is returned to all threads.
+template <int NW>
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) r = fmax(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+// The Householder vector of the panel column whose diagonal row is c, at row r (a: the stored value).
+__device__ __forceinline__ double2 vval(int r, int c, double2 a) {
+  return r > c ? a : (r == c ? double2{1.0, 0.0} : double2{0.0, 0.0});
+}
+
+// Factorise columns [j0, j0+pb) of A (rows j0..m-1), pb <= kPB, m - j0 <= RPT * NT.
+// Writes R / v in place, V[:, j0+u] and Z[:, j0+u] (ld = m, rows >= j0) and Y[:, u] (ld = m).
+template <int NT, int RPT>
+__global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, int m, int j0, int pb,
+                                                       double2* __restrict__ V, double2* __restrict__ Y,
+                                                       double2* __restrict__ Z) {
+  constexpr int NW = NT / 64;
+  __shared__ double2 red[kQWmax][kPB];
+  __shared__ double2 sums[kPB];
+  __shared__ double2 G[kPB][kPB];   // G[a][b] = V[:,a]^H V[:,b] for a < b
+  __shared__ double2 Tn[kPB][kPB];  // -T, upper triangular
+  __shared__ double2 tau_s[kPB];
+  __shared__ double2 scal_s;
+  const int tid = threadIdx.x;
+
+  double2 a[RPT][kPB];
+  int row[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    row[i] = j0 + tid + i * NT;
+#pragma unroll
+    for (int u = 0; u < kPB; ++u)
+      a[i][u] = (row[i] < m && u < pb) ? A[row[i] + (int64_t)m * (j0 + u)] : double2{0.0, 0.0};
+  }
+
+  // t is unrolled so every register index below is a compile-time constant
+#pragma clang loop unroll(full)
+  for (int t = 0; t < kPB; ++t) {
+    if (t < pb) {
+      const int c = j0 + t;
+      // squared norm of the column below the diagonal
+      double2 s1[1] = {{0.0, 0.0}};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+        if (row[i] > c) s1[0].x += a[i][t].x * a[i][t].x + a[i][t].y * a[i][t].y;
+      block_sum<1, NW>(s1, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
+      // A column whose squares underflow is rescaled by its largest entry first (the role of
+      // zlarfg's safmin loop); the test reads shared memory, so every thread takes the branch.
+      double xs2 = sums[0].x, S = 1.0;
+      if (xs2 < 1e-280) {
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (row[i] >= c) mx = fmax(mx, fmax(fabs(a[i][t].x), fabs(a[i][t].y)));
+        S = block_max<NW>(mx, &red[0][0].x);
+        double2 s2[1] = {{0.0, 0.0}};
+        if (S > 0.0) {
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (row[i] > c) {
+              const double xr = a[i][t].x / S, xi = a[i][t].y / S;
+              s2[0].x += xr * xr + xi * xi;
+            }
+        }
+        block_sum<1, NW>(s2, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
+        xs2 = sums[0].x;
+        if (S == 0.0) S = 1.0;
+      }
+      // zlarfg on the thread that owns row c (c - j0 = t < kPB <= NT: thread t, slot 0)
+      if (tid == t) {
+        const double2 alpha = a[0][t];
+        double2 tau{0.0, 0.0}, scal{0.0, 0.0};
+        if (!(xs2 == 0.0 && alpha.y == 0.0)) {
+          const double ar = alpha.x / S, ai = alpha.y / S;
+          const double nrm = S * sqrt(ar * ar + ai * ai + xs2);
+          const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+          tau = double2{(beta - alpha.x) / beta, -alpha.y / beta};
+          // scal = 1 / (alpha - beta), scaled so |alpha - beta|^2 cannot underflow
+          const double2 dd{alpha.x - beta, alpha.y};
+          const double sd = fmax(fabs(dd.x), fabs(dd.y));
+          const double2 ds{dd.x / sd, dd.y / sd};
+          const double den = (ds.x * ds.x + ds.y * ds.y) * sd;
+          scal = double2{ds.x / den, -ds.y / den};
+          a[0][t] = double2{beta, 0.0};
+        }
+        tau_s[t] = tau;
+        scal_s = scal;
+      }
+      __syncthreads();
+      const double2 tau = tau_s[t], scal = scal_s;
+      const bool reflect = !(tau.x == 0.0 && tau.y == 0.0);
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+        if (row[i] > c) a[i][t] = reflect ? cmul(a[i][t], scal) : double2{0.0, 0.0};
+      // one reduction: w_u = v^H A[:, u] (u > t) and G[u][t] = V[:, u]^H v (u < t)
+      double2 part[kPB];
+#pragma unroll
+      for (int u = 0; u < kPB; ++u) part[u] = double2{0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const double2 vr = vval(row[i], j0 + t, a[i][t]);
+#pragma unroll
+        for (int u = 0; u < kPB; ++u) {
+          if (u > t) {
+            const double2 p = cjmul(vr, a[i][u]);
+            part[u].x += p.x;
+            part[u].y += p.y;
+          } else if (u < t) {
+            const double2 p = cjmul(vval(row[i], j0 + u, a[i][u]), vr);
+            part[u].x += p.x;
+            part[u].y += p.y;
+          }
+        }
+      }
+      block_sum16<NW>(part, red, sums);
+      if (tid < t) G[tid][t] = sums[tid];
+      // H^H on the remaining panel columns: a_u -= conj(tau) v w_u
+      const double2 ctau{tau.x, -tau.y};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        if (row[i] < c) continue;
+        const double2 cv = cmul(ctau, vval(row[i], j0 + t, a[i][t]));
+#pragma unroll
+        for (int u = 0; u < kPB; ++u) {
+          if (u > t && u < pb) {
+            const double2 p = cmul(cv, sums[u]);
+            a[i][u].x -= p.x;
+            a[i][u].y -= p.y;
+          }
+        }
+      }
+      __syncthreads();  // sums / red are rewritten by the next column
+    }
+  }
+
+  // -T of the panel (zlarft, forward columnwise): T[t][t] = tau_t,
+  // T[0:t, t] = -tau_t T[0:t, 0:t] G[0:t, t]; with Tn = -T: Tn[r][t] = -tau_t sum_j Tn[r][j] G[j][t]
+  for (int t = 0; t < kPB; ++t) {
+    if (tid < kPB) {
+      const int r = tid;
+      double2 v{0.0, 0.0};
+      if (t < pb) {
+        if (r == t) {
+          v = double2{-tau_s[t].x, -tau_s[t].y};
+        } else if (r < t) {
+          double2 acc{0.0, 0.0};
+          for (int j = r; j < t; ++j) {
+            const double2 p = cmul(Tn[r][j], G[j][t]);
+            acc.x += p.x;
+            acc.y += p.y;
+          }
+          const double2 q = cmul(tau_s[t], acc);
+          v = double2{-q.x, -q.y};
+        }
+      }
+      Tn[r][t] = v;
+    }
+    __syncthreads();
+  }
+
+  // outputs: A (R / v), V, Y = V Tn^H, Z = V Tn
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    if (row[i] >= m) continue;
+    double2 vr[kPB];
+#pragma unroll
+    for (int u = 0; u < kPB; ++u) vr[u] = u < pb ? vval(row[i], j0 + u, a[i][u]) : double2{0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < kPB; ++u) {
+      if (u >= pb) continue;
+      const int64_t col = (int64_t)m * (j0 + u);
+      A[row[i] + col] = a[i][u];
+      V[row[i] + col] = vr[u];
+      double2 y{0.0, 0.0}, z{0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) {
+        const double2 py = cjmul(Tn[u][q], vr[q]);  // vr[q] * conj(Tn[u][q])
+        y.x += py.x;
+        y.y += py.y;
+        const double2 pz = cmul(vr[q], Tn[q][u]);
+        z.x += pz.x;
+        z.y += pz.y;
+      }
+      Y[row[i] + (int64_t)m * u] = y;
+      Z[row[i] + col] = z;
+    }
+  }
+}
+
+// Q = I (m x k)
+__global__ void __launch_bounds__(256) eye_kernel(double2* __restrict__ Q, int m, int k) {
+  const int64_t total = (int64_t)m * k;
+  for (int64_t t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t i = t % m, j = t / m;
+    Q[t] = double2{i == j ? 1.0 : 0.0, 0.0};
+  }
+}
+
+// R (k x n) = upper trapezoid of the factorised A (ld m)
+__global__ void __launch_bounds__(256) upper_kernel(const double2* __restrict__ A, int m, int k, int n,
+                                                    double2* __restrict__ R) {
+  const int64_t total = (int64_t)k * n;
+  for (int64_t t = blockIdx.x * 256LL + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+    const int64_t i = t % k, j = t / k;
+    R[t] = i <= j ? A[i + (int64_t)m * j] : double2{0.0, 0.0};
+  }
+}
+
+unsigned grid1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)); }
+
+// W = V^H C for a panel (pb <= 16 rows of W) over a long K: the rows are cut into chunks of
+// kVC, each workgroup forms the 16 x 16 partial product of one chunk and one block of 16
+// columns from LDS, and a second kernel sums the chunks in order (deterministic, no atomics).
+constexpr int kVC = 64;
+constexpr int kVJ = 16;
+
+__global__ void __launch_bounds__(256) vh_partial_kernel(const double2* __restrict__ V, int ldv, int rows, int pb,
+                                                         const double2* __restrict__ C, int ldc, int cols,
+                                                         double2* __restrict__ P) {
+  __shared__ double2 Vs[kVC][kPB + 1];
+  __shared__ double2 Cs[kVC][kVJ + 1];
+  const int tid = threadIdx.x;
+  const int jb = blockIdx.x * kVJ, r0 = blockIdx.y * kVC;
+  for (int e = tid; e < kVC * kPB; e += 256) {
+    const int r = e % kVC, a = e / kVC;
+    Vs[r][a] = (r0 + r < rows && a < pb) ? V[(r0 + r) + (int64_t)ldv * a] : double2{0.0, 0.0};
+  }
+  for (int e = tid; e < kVC * kVJ; e += 256) {
+    const int r = e % kVC, j = e / kVC;
+    Cs[r][j] = (r0 + r < rows && jb + j < cols) ? C[(r0 + r) + (int64_t)ldc * (jb + j)] : double2{0.0, 0.0};
+  }
+  __syncthreads();
+  const int a = tid % kPB, j = tid / kPB;
+  double2 acc{0.0, 0.0};
+#pragma unroll 8
+  for (int r = 0; r < kVC; ++r) {
+    const double2 p = cjmul(Vs[r][a], Cs[r][j]);
+    acc.x += p.x;
+    acc.y += p.y;
+  }
+  if (a < pb && jb + j < cols) P[((int64_t)blockIdx.y * cols + jb + j) * kPB + a] = acc;
+}
+
+__global__ void __launch_bounds__(256) vh_reduce_kernel(const double2* __restrict__ P, int nch, int pb, int cols,
+                                                        double2* __restrict__ W) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)pb * cols) return;
+  const int a = (int)(t % pb), j = (int)(t / pb);
+  double2 s{0.0, 0.0};
+  for (int ch = 0; ch < nch; ++ch) {
+    const double2 v = P[((int64_t)ch * cols + j) * kPB + a];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  W[a + (int64_t)pb * j] = s;
+}
+
+// W (pb x cols, ld pb) = V^H C, V rows x pb (ld ldv), C rows x cols (ld ldc); P: partials.
+hipError_t vh_times(const double2* V, int ldv, int rows, int pb, const double2* C, int ldc, int cols, double2* W,
+                    double2* P, hipStream_t s) {
+  const int nch = (rows + kVC - 1) / kVC;
+  hipLaunchKernelGGL(vh_partial_kernel, dim3((cols + kVJ - 1) / kVJ, nch), dim3(256), 0, s, V, ldv, rows, pb, C, ldc,
+                     cols, P);
+  hipLaunchKernelGGL(vh_reduce_kernel, dim3(grid1((int64_t)pb * cols)), dim3(256), 0, s, P, nch, pb, cols, W);
+  return hipGetLastError();
+}
+
+hipError_t plus_times(const double2* P, int ldp, int rows, int pb, const double2* W, double2* C, int ldc, int cols,
+                      hipStream_t s) {
+  // C += P W (in place: each output element reads its own Dadd value first)
+  ZgemmDesc g;
+  g.A = P;
+  g.B = W;
+  g.C = C;
+  g.Dadd = C;
+  g.beta = 1.0;
+  g.M = rows;
+  g.N = cols;
+  g.K = pb;
+  g.NB = 1;
+  g.sAm = 1;
+  g.sAk = ldp;
+  g.sBk = 1;
+  g.sBn = pb;
+  g.sCm = 1;
+  g.sCn = ldc;
+  g.a_mfast = 1;
+  g.b_kfast = 1;
+  return launch_zbrgemm_f64(g, TileCfg{32, 32}, s);
+}
+
+}  // namespace
+
+int house_qr_max_rows() { return 1024; }
+
+int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
+  if (m < 1 || n < 1 || m > house_qr_max_rows()) return set_err(ctx, MPSD_ERR_ARG, "house_qr: unsupported size");
+  const int k = std::min(m, n);
+  const int64_t mk = (int64_t)m * k;
+  const int64_t wn = std::max(n, k), nch = (m + kVC - 1) / kVC;
+  const size_t need = sizeof(double2) * (size_t)(2 * mk + (int64_t)m * kPB + kPB * wn * (1 + nch));
+  int rc = ensure_scratch(ctx, 6, need);
+  if (rc) return rc;
+  double2* V = (double2*)ctx->scratch[6];
+  double2* Z = V + mk;
+  double2* Y = Z + mk;
+  double2* W = Y + (int64_t)m * kPB;
+  double2* P = W + kPB * wn;  // chunk partials of V^H C
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  for (int j0 = 0; j0 < k && e == hipSuccess; j0 += kPB) {
+    const int pb = std::min(kPB, k - j0), rows = m - j0;
+    // up to 256 rows: one wave per SIMD; beyond, 512 threads (two waves per SIMD) with one or
+    // two rows per thread
+    if (rows <= kQT)
+      hipLaunchKernelGGL((qr_panel_kernel<kQT, 1>), dim3(1), dim3(kQT), 0, s, A, m, j0, pb, V, Y, Z);
+    else if (rows <= 512)
+      hipLaunchKernelGGL((qr_panel_kernel<512, 1>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Y, Z);
+    else
+      hipLaunchKernelGGL((qr_panel_kernel<512, 2>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Y, Z);
+    e = hipGetLastError();
+    const int cols = n - j0 - pb;
+    if (e == hipSuccess && cols > 0) {
+      double2* C = A + j0 + (int64_t)m * (j0 + pb);
+      e = vh_times(V + j0 + (int64_t)m * j0, m, rows, pb, C, m, cols, W, P, s);
+      if (e == hipSuccess) e = plus_times(Y + j0, m, rows, pb, W, C, m, cols, s);
+    }
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(upper_kernel, dim3(grid1((int64_t)k * n)), dim3(256), 0, s, A, m, k, n, R);
+    hipLaunchKernelGGL(eye_kernel, dim3(grid1(mk)), dim3(256), 0, s, Q, m, k);
+    e = hipGetLastError();
+  }
+  // Q = H_0 ... H_{k-1} I, accumulated from the last panel back
+  const int last = ((k - 1) / kPB) * kPB;
+  for (int j0 = last; j0 >= 0 && e == hipSuccess; j0 -= kPB) {
+    const int pb = std::min(kPB, k - j0), rows = m - j0, cols = k - j0;
+    double2* Qs = Q + j0 + (int64_t)m * j0;
+    e = vh_times(V + j0 + (int64_t)m * j0, m, rows, pb, Qs, m, cols, W, P, s);
+    if (e == hipSuccess) e = plus_times(Z + j0 + (int64_t)m * j0, m, rows, pb, W, Qs, m, cols, s);
+  }
+  return hip_err(ctx, e, "house_qr");
+}
+
+}  // namespace mpsd

@@ -27,6 +27,7 @@ OPT_TILE_K = 1
 OPT_SPLIT_STAGE3 = 2
 OPT_GRAPH = 3
 OPT_BATCH_GROUP = 4
+OPT_GAUGE_QR = 5
 
 _lib = None
 
