@@ -75,6 +75,7 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
 #define MPSD_OPT_SPLIT_STAGE3 2 /* matvec last stage split over the MPO bond: -1 auto, 0 off, 1 on */
 #define MPSD_OPT_GRAPH 3        /* eigensolver steps as replayed hipGraphs: 1 (default) or 0 */
 #define MPSD_OPT_GAUGE_QR 5     /* gauge-step QR / LQ: 0 library Householder kernels (default), 1 rocSOLVER */
+#define MPSD_OPT_FUSED_LANCZOS 6 /* small local problems on the one-workgroup eigensolver: 1 (default) or 0 */
 #define MPSD_OPT_BATCH_GROUP 4  /* batches per BRGEMM workgroup: 0 built-in (default), 1, or 2 when an
                                    operand is shared by the batches; results are the same for all */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
@@ -186,6 +187,15 @@ int mpsd_heff_set_hermitian(mpsd_heff* h, int on);
  * Contractor::solveEigs (src/core/cpp/Contractor.cpp:3087-3100 for fulleig, :3126-3133 below
  * LOWEST_PRIMME) with wrapper::eig (src/wrapper/eig.cpp:46-145).  Local size <= 16384. */
 int mpsd_heff_eigs_dense(mpsd_heff* h, double* x, double* value);
+/* Lowest eigenpair of a small FP64 Heff (local size <= 2048, matvec <= 1.5 MFLOP, Krylov basis
+ * <= 63) by restarted Lanczos run entirely inside one workgroup: matvec, two-pass Gram-Schmidt,
+ * tridiagonal eigenpair by multisection + inverse iteration, restarts and the Ritz vector, in
+ * one launch.  x (device) holds the start vector and receives the eigenvector.  out6 (host):
+ * eigenvalue, residual estimate, matvecs, restarts, converged (0/1), status (0 ok, 2 start
+ * vector zero or not finite, 3 non-finite coefficients).  MPSD_ERR_UNSUPPORTED when the
+ * problem does not fit.  Stands in for the PRIMME/ARPACK solve of small local problems
+ * (src/core/cpp/Contractor.cpp:3067-3138, src/wrapper/eigs.cpp:325-422). */
+int mpsd_heff_eigs_fused(mpsd_heff* h, double* x, double tol, int max_krylov, int max_restarts, double* out6);
 /* Change the arithmetic of an existing handle (MPSD_PREC_*). */
 int mpsd_heff_set_precision(mpsd_heff* h, int precision);
 

@@ -219,6 +219,9 @@ class Multiplier {
   // The C-ABI handle when the product is a plain device Heff (enables the graph-driven
   // eigensolver path), else null.
   virtual mpsd_heff* heff() const { return nullptr; }
+  // The C-ABI handle the one-workgroup eigensolver may use for a small local problem (it
+  // then runs the whole eigensolve itself), else null.
+  virtual mpsd_heff* smallHeff() const { return heff(); }
 };
 
 // TensorMultiplierOffset (TensorMultiplierOffset.cpp:8-25): Heff + offset, on the device.

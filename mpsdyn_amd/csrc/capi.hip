@@ -313,6 +313,7 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_GRAPH: return ctx->graph ? 1 : 0;
     case MPSD_OPT_BATCH_GROUP: return ctx->tile_override.nbw;
     case MPSD_OPT_GAUGE_QR: return ctx->gauge_qr;
+    case MPSD_OPT_FUSED_LANCZOS: return ctx->fused_lanczos;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -335,6 +336,9 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_GAUGE_QR:
       if (value != 0 && value != 1) return set_err(ctx, MPSD_ERR_ARG, "set_option: gauge QR must be 0 or 1");
       ctx->gauge_qr = value;
+      return MPSD_OK;
+    case MPSD_OPT_FUSED_LANCZOS:
+      ctx->fused_lanczos = value != 0;
       return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");

@@ -241,6 +241,24 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
   const int64_t n = H.getSize();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>(max_krylov, n));
   mpsd_heff* hh = H.heff();
+  mpsd_heff* hs = H.smallHeff();
+  if (hs && mpsd_ctx_get_option(ctx_, MPSD_OPT_FUSED_LANCZOS) == 1) {
+    // small local problems: the whole eigensolve in one workgroup, one launch
+    double o[6];
+    const int rc = mpsd_heff_eigs_fused(hs, x, tol, m, max_restarts, o);
+    if (rc == MPSD_OK && o[5] == 3.0)
+      throw Error(MPSD_ERR_NUMERIC, "Lanczos: non-finite value (NaN/Inf in the operator or vector)");
+    if (rc == MPSD_OK && o[5] == 0.0) {
+      st.value = o[0];
+      st.residual = o[1];
+      st.matvecs = (int)o[2];
+      st.restarts = (int)o[3];
+      st.converged = o[4] != 0.0;
+      return st;
+    }
+    if (rc != MPSD_OK && rc != MPSD_ERR_UNSUPPORTED) check(ctx_, rc, "eigs_fused");
+    // status 2 (zero / non-finite start vector) and problems that do not fit continue below
+  }
   if (hh && m >= 3 && mpsd_ctx_get_option(ctx_, MPSD_OPT_GRAPH) == 1) {
     // the graph path needs d = 2 and the FP64 / FP32 forms; anything else runs below
     try {

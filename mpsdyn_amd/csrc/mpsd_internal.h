@@ -120,10 +120,10 @@ struct mpsd_ctx {
   // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
   // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step;
   // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec; 6 = Householder QR workspace
-  // (V, Z, Y, W); 7 = second gauge buffer.
-  static constexpr int kScratch = 8;
-  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // (V, Z, Y, W); 7 = second gauge buffer; 8 = Krylov basis of the one-workgroup eigensolver.
+  static constexpr int kScratch = 9;
+  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // Problem tables of grouped launches: device buffer + pinned host staging, reused after
   // the event of the previous upload has completed.
   void* gtab = nullptr;
@@ -153,6 +153,7 @@ struct mpsd_ctx {
   int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
   bool graph = true;  // eigensolver steps as replayed hipGraphs where the Heff allows it
   int gauge_qr = 0;   // gauge-step QR / LQ: 0 the library's Householder kernels, 1 rocSOLVER
+  int fused_lanczos = 1;  // small local problems: one-workgroup eigensolver (1) or the general one (0)
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   struct Pending {
@@ -208,6 +209,11 @@ hipError_t heff_full_dev(mpsd_heff* h, double2* out);
 // (k x n), k = min(m, n), in the LAPACK zgeqrf / zungqr conventions.  m <= house_qr_max_rows().
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R);
 int house_qr_max_rows();
+// One-workgroup restarted Lanczos for small Heffs (lanczos_fused.hip): whether h fits, and
+// the solve (out6: theta, residual, matvecs, restarts, converged, status 0 / 2 start vector
+// zero or not finite / 3 non-finite coefficients).
+bool lanczos_fused_fits(const mpsd_heff* h, int m);
+int lanczos_fused(mpsd_heff* h, double* x, double tol, int m, int max_restarts, double* out6);
 // The context's rocBLAS handle (created on first use, bound to the context's stream).
 int blas_handle(mpsd_ctx* ctx, void** out);
 int set_err(mpsd_ctx* ctx, int code, const std::string& msg);

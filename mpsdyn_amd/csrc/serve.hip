@@ -188,6 +188,8 @@ class BatchedMultiplier : public Multiplier {
   }
   mpsd_ctx* ctx() const override { return ctx_; }
   void setPrecision(int p) override { check(ctx_, mpsd_heff_set_precision(h_, p), "set_precision"); }
+  // small local problems are solved by the one-workgroup eigensolver, as they are alone
+  mpsd_heff* smallHeff() const override { return h_; }
 
  private:
   Batcher* b_;

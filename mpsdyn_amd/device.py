@@ -28,6 +28,7 @@ OPT_SPLIT_STAGE3 = 2
 OPT_GRAPH = 3
 OPT_BATCH_GROUP = 4
 OPT_GAUGE_QR = 5
+OPT_FUSED_LANCZOS = 6
 
 _lib = None
 

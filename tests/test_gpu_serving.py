@@ -58,7 +58,8 @@ def _cfgs(dev):
         dev.gs_config(dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12, seed=1),
         dev.gs_config(dev.MODEL_HEISENBERG, 16, 24, [1.0, 1.0, 0.8, 0.0], convtol=1e-12, seed=2),
         dev.gs_config(dev.MODEL_SCHWINGER, 12, 32, [2.5, 100.0, 0.0, 0.0], convtol=1e-12, seed=3),
-        dev.gs_config(dev.MODEL_ISING, 32, 16, [-1.0, -0.7, -0.2], convtol=1e-12, seed=4),
+        dev.gs_config(dev.MODEL_ISING, 32, 48, [-1.0, -0.7, -0.2], convtol=1e-12, seed=4),
+        dev.gs_config(dev.MODEL_HEISENBERG, 24, 64, [1.0, 1.0, 1.0, 0.0], convtol=1e-12, seed=7),
         dev.gs_config(dev.MODEL_ISING, 24, 32, [-1.0, -1.3, 0.0], convtol=1e-12, seed=5, precision=1),
         dev.gs_config(dev.MODEL_HEISENBERG, 20, 16, [1.0, 1.0, 1.2, 0.0], convtol=1e-12, seed=6, precision=2),
     ]
