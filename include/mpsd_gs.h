@@ -68,6 +68,33 @@ int mpsd_gs_solve_export(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* e
 int mpsd_gs_expectation(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mps_path, double* re,
                         double* im);
 
+/* MPO files.  MPSD_MPO_BINARY is MPO::saveMPO / loadMPO (src/core/cpp/MPO.cpp:214-232,
+ * 293-316): int length, then per operator mwArray::save (int rank, int dims[rank], 2*n doubles
+ * of interleaved complex128, column-major [du, Dl, dd, Dr]).  MPSD_MPO_TEXT is
+ * MPO::exportMPOtext / importMPOtext (MPO.cpp:234-271): "length " then per operator
+ * mwArray::savetext, "rank dims... re im ..." separated by spaces (written with 17 significant
+ * digits; files with fewer digits read the same way). */
+#define MPSD_MPO_BINARY 0
+#define MPSD_MPO_TEXT 1
+
+/* Write the Hamiltonian MPO of cfg's model (cfg->model, L, p) to `path`.  Host-only: ctx may
+ * be NULL (then no error message is kept). */
+int mpsd_mpo_save(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* path, int format);
+
+/* Ground state of the chain Hamiltonian read from an MPO file (the reference's programs that
+ * importMPOtext / loadMPO their Hamiltonian, e.g. src/programs/banulsm/bosehub/bosehubFile.cpp,
+ * then call Contractor::findGroundState).  The file fixes L and the physical dimension of each
+ * site (any d >= 1, du == dd); cfg supplies D, offset, tolerances, precision, seed and solver,
+ * and its model, L and p are not read.  Rejects files whose operators are not rank 4, not square
+ * in the physical index or whose bonds do not chain (MPSD_ERR_SHAPE).  export_path as in
+ * mpsd_gs_solve_export. */
+int mpsd_gs_solve_mpo(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format,
+                      const char* export_path, mpsd_gs_result* out);
+
+/* <psi|H|psi> of an MPS file (exportMPS layout) for the MPO in an MPO file. */
+int mpsd_mpo_expectation(mpsd_ctx* ctx, const char* mpo_path, int format, const char* mps_path, double* re,
+                         double* im);
+
 /* Lowest eigenpair of an Heff by the device FP64 Lanczos solver.  Replaces
  * Contractor::solveEigs(multi, 1, "SR", ...) (src/core/cpp/Contractor.cpp:3067-3138) and the
  * eigensolver wrappers behind it (src/wrapper/eigs.cpp:71-422).  x_dev: device vector of

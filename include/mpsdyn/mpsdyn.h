@@ -66,6 +66,11 @@ class mwArray {
   // binary layout of mwArray::save/load (src/wrapper/mwArray.cpp:382-443)
   void save(std::ofstream& out) const;
   void load(std::ifstream& in);
+  // text layout of mwArray::savetext/loadtext (mwArray.cpp:1672-1717): "rank dims... re im ..."
+  // separated by spaces, or only the real parts when real is set; written with 17 significant
+  // digits so that a save/load round trip is exact
+  void savetext(std::ostream& out, bool real = false) const;
+  void loadtext(std::istream& in, bool real = false);
 
  private:
   std::vector<int> dims_;
@@ -120,6 +125,14 @@ class MPO {
   void setOpShared(int k, const std::shared_ptr<const Operator>& op) { ops_[k] = op; }
   const Operator& getOp(int k) const { return *ops_[k]; }
   std::vector<int> getDimensions() const;  // physical dimension per site
+  // MPO::saveMPO / loadMPO (src/core/cpp/MPO.cpp:214-232, 293-316): int length, then each
+  // operator's [du, Dl, dd, Dr] tensor in mwArray::save layout.
+  void saveMPO(const char* filename) const;
+  void loadMPO(const char* filename);
+  // MPO::exportMPOtext / importMPOtext (MPO.cpp:234-271): "length " then mwArray::savetext of
+  // each operator.
+  void exportMPOtext(const char* filename) const;
+  void importMPOtext(const char* filename);
 
  private:
   std::vector<std::shared_ptr<const Operator>> ops_;
@@ -155,6 +168,8 @@ class MPS {
   MPS() {}
   // MPS(len, bond, d) (src/core/cpp/MPS.cpp:15-35): sites [d,1,D], [d,D,D]..., [d,D,1].
   MPS(int length, int bond, int d, mpsd_ctx* ctx = nullptr);
+  // MPS(len, bond, vector<int> d) (MPS.cpp:68-87): site k has physical dimension d[k].
+  MPS(int length, int bond, const std::vector<int>& d, mpsd_ctx* ctx = nullptr);
   int getLength() const { return (int)A_.size(); }
   int getBond() const { return bond_; }
   const Site& getA(int pos) const { return *A_[pos]; }

@@ -30,20 +30,60 @@ int fail(mpsd_ctx* ctx, const Error& e) {
   return e.code ? e.code : MPSD_ERR_ARG;
 }
 
+// Reads an MPO file (MPO::loadMPO / MPO::importMPOtext, src/core/cpp/MPO.cpp:250-316) and checks
+// that it is a chain Hamiltonian the sweep can use: rank-4 [du, Dl, dd, Dr] operators with
+// du == dd, matching bonds between neighbours and bond 1 at both ends.
+void read_mpo(MPO& mpo, const char* path, int format) {
+  if (!path) throw Error(MPSD_ERR_ARG, "MPO path is NULL");
+  if (format == MPSD_MPO_BINARY)
+    mpo.loadMPO(path);
+  else if (format == MPSD_MPO_TEXT)
+    mpo.importMPOtext(path);
+  else
+    throw Error(MPSD_ERR_ARG, "unknown MPO file format");
+  const int L = mpo.getLength();
+  if (L < 2) throw Error(MPSD_ERR_SHAPE, "MPO file: need at least two sites");
+  for (int k = 0; k < L; ++k) {
+    const Operator& W = mpo.getOp(k);
+    if (W.getDimension(0) != W.getDimension(2))
+      throw Error(MPSD_ERR_SHAPE, "MPO file: operator " + std::to_string(k) + " is not square in the physical index");
+    const int left = k == 0 ? 1 : mpo.getOp(k - 1).getDimension(3);
+    if (W.getDimension(1) != left)
+      throw Error(MPSD_ERR_SHAPE, "MPO file: bond mismatch at operator " + std::to_string(k));
+  }
+  if (mpo.getOp(L - 1).getDimension(3) != 1) throw Error(MPSD_ERR_SHAPE, "MPO file: last bond is not 1");
+}
+
 }  // namespace
 
 namespace mpsdyn {
+int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo, const char* export_path,
+                      const MultiplierFactory* factory, mpsd_gs_result* out);
+
 // One solve on ctx with an optional multiplier factory (used by the serving loop).
 int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, const MultiplierFactory* factory,
                   mpsd_gs_result* out) {
   if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
   if (cfg->L < 2 || cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need L >= 2 and D >= 1");
+  try {
+    auto H = make_model(cfg);
+    return gs_solve_mpo_with(ctx, cfg, H->getHMPO(), export_path, factory, out);
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  }
+}
+
+// The sweep itself, on any chain MPO; cfg supplies D, the tolerances, precision, seed and
+// solver (its model, L and p are not read).
+int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo, const char* export_path,
+                      const MultiplierFactory* factory, mpsd_gs_result* out) {
+  if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
+  if (cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need D >= 1");
   if (cfg->solver != MPSD_SOLVER_LANCZOS && cfg->solver != MPSD_SOLVER_FULLEIG)
     return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: unknown solver");
   memset(out, 0, sizeof(*out));
   try {
-    auto H = make_model(cfg);
-    MPS mps(cfg->L, cfg->D, 2, ctx);
+    MPS mps(hmpo.getLength(), cfg->D, hmpo.getDimensions(), ctx);
     mps.setRandomState(cfg->seed);
     Contractor c(ctx);
     if (factory) c.setMultiplierFactory(*factory);
@@ -52,10 +92,10 @@ int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_p
     c.setMaxHalfSweeps(cfg->max_halfsweeps);
     c.setPrecision(cfg->precision);
     if (cfg->solver == MPSD_SOLVER_FULLEIG) c.setEigenSolver(fulleig);
-    out->initial_energy = c.contract(mps, H->getHMPO(), mps).real();
+    out->initial_energy = c.contract(mps, hmpo, mps).real();
     double E = 0.0;
     auto t0 = std::chrono::steady_clock::now();
-    c.findGroundState(H->getHMPO(), cfg->D, &E, mps, cfg->offset);
+    c.findGroundState(hmpo, cfg->D, &E, mps, cfg->offset);
     check(ctx, mpsd_ctx_sync(ctx), "sync");
     auto t1 = std::chrono::steady_clock::now();
     const SolveStats& st = c.lastStats();
@@ -136,6 +176,63 @@ int mpsd_gs_solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, mpsd_gs_result* out)
 
 int mpsd_gs_solve_export(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, mpsd_gs_result* out) {
   return solve(ctx, cfg, export_path, out);
+}
+
+int mpsd_mpo_save(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* path, int format) {
+  if (!cfg || !path) return MPSD_ERR_ARG;
+  if (cfg->L < 2) return ctx ? mpsd::set_err(ctx, MPSD_ERR_ARG, "mpo_save: need L >= 2") : MPSD_ERR_ARG;
+  try {
+    auto H = make_model(cfg);
+    if (format == MPSD_MPO_BINARY)
+      H->getHMPO().saveMPO(path);
+    else if (format == MPSD_MPO_TEXT)
+      H->getHMPO().exportMPOtext(path);
+    else
+      throw Error(MPSD_ERR_ARG, "unknown MPO file format");
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_gs_solve_mpo(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format,
+                      const char* export_path, mpsd_gs_result* out) {
+  if (!ctx || !cfg || !mpo_path || !out) return MPSD_ERR_ARG;
+  try {
+    MPO mpo;
+    read_mpo(mpo, mpo_path, format);
+    return mpsdyn::gs_solve_mpo_with(ctx, cfg, mpo, export_path, nullptr, out);
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_mpo_expectation(mpsd_ctx* ctx, const char* mpo_path, int format, const char* mps_path, double* re,
+                         double* im) {
+  if (!ctx || !mpo_path || !mps_path || !re || !im) return MPSD_ERR_ARG;
+  try {
+    MPO mpo;
+    read_mpo(mpo, mpo_path, format);
+    MPS mps(1, 1, 2, ctx);
+    mps.importMPS(mps_path);
+    if (mps.getLength() != mpo.getLength()) throw Error(MPSD_ERR_SHAPE, "mpo_expectation: MPS and MPO lengths differ");
+    Contractor c(ctx);
+    complex_t e = c.contract(mps, mpo, mps);
+    *re = e.real();
+    *im = e.imag();
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
 }
 
 int mpsd_gs_expectation(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mps_path, double* re, double* im) {

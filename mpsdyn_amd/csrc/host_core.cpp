@@ -118,6 +118,38 @@ void mwArray::load(std::ifstream& in) {
   if (!in) throw Error(MPSD_ERR_ARG, "mwArray::load: truncated data");
 }
 
+void mwArray::savetext(std::ostream& out, bool real) const {
+  const int rank = getRank();
+  out << rank << " ";
+  for (int k = 0; k < rank; ++k) out << dims_[k] << " ";
+  const std::streamsize prec = out.precision(17);
+  for (const complex_t& c : data_) {
+    out << c.real() << " ";
+    if (!real) out << c.imag() << " ";
+  }
+  out.precision(prec);
+  if (!out) throw Error(MPSD_ERR_ARG, "mwArray::savetext: write failed");
+}
+
+void mwArray::loadtext(std::istream& in, bool real) {
+  int rank = 0;
+  in >> rank;
+  if (!in || rank < 0 || rank > 64) throw Error(MPSD_ERR_ARG, "mwArray::loadtext: bad rank");
+  dims_.assign(rank, 0);
+  for (int k = 0; k < rank; ++k) {
+    in >> dims_[k];
+    if (!in || dims_[k] < 0) throw Error(MPSD_ERR_ARG, "mwArray::loadtext: bad dimensions");
+  }
+  data_.assign(prod(dims_), complex_t(0, 0));
+  for (complex_t& c : data_) {
+    double re = 0.0, im = 0.0;
+    in >> re;
+    if (!real) in >> im;
+    if (!in) throw Error(MPSD_ERR_ARG, "mwArray::loadtext: truncated data");
+    c = complex_t(re, im);
+  }
+}
+
 // ------------------------------------------------------------------------------ DevTensor
 DevTensor::DevTensor(mpsd_ctx* c, const std::vector<int>& dims) : ctx(c) {
   std::vector<int> d = dims.empty() ? std::vector<int>{1} : dims;
@@ -171,6 +203,52 @@ void MPO::setOp(int k, Operator* op, bool myop) {
     ops_[k] = std::shared_ptr<const Operator>(op);
   else
     ops_[k] = std::shared_ptr<const Operator>(op, [](const Operator*) {});
+}
+
+void MPO::saveMPO(const char* filename) const {
+  std::ofstream out(filename, std::ios::binary);
+  if (!out.is_open()) throw Error(MPSD_ERR_ARG, std::string("saveMPO: cannot open ") + filename);
+  const int length = getLength();
+  out.write((const char*)&length, sizeof(int));
+  for (int k = 0; k < length; ++k) getOp(k).getFullData().save(out);
+  if (!out) throw Error(MPSD_ERR_ARG, "saveMPO: write failed");
+}
+
+void MPO::loadMPO(const char* filename) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in.is_open()) throw Error(MPSD_ERR_ARG, std::string("loadMPO: cannot open ") + filename);
+  int length = 0;
+  in.read((char*)&length, sizeof(int));
+  if (!in || length < 0 || length > (1 << 20)) throw Error(MPSD_ERR_ARG, "loadMPO: bad length");
+  initLength(length);
+  for (int k = 0; k < length; ++k) {
+    mwArray a;
+    a.load(in);
+    setOp(k, new Operator(a), true);
+  }
+}
+
+void MPO::exportMPOtext(const char* filename) const {
+  std::ofstream out(filename);
+  if (!out.is_open()) throw Error(MPSD_ERR_ARG, std::string("exportMPOtext: cannot open ") + filename);
+  const int length = getLength();
+  out << length << " ";
+  for (int k = 0; k < length; ++k) getOp(k).getFullData().savetext(out);
+  if (!out) throw Error(MPSD_ERR_ARG, "exportMPOtext: write failed");
+}
+
+void MPO::importMPOtext(const char* filename) {
+  std::ifstream in(filename);
+  if (!in.is_open()) throw Error(MPSD_ERR_ARG, std::string("importMPOtext: cannot open ") + filename);
+  int length = 0;
+  in >> length;
+  if (!in || length < 0 || length > (1 << 20)) throw Error(MPSD_ERR_ARG, "importMPOtext: bad length");
+  initLength(length);
+  for (int k = 0; k < length; ++k) {
+    mwArray a;
+    a.loadtext(in);
+    setOp(k, new Operator(a), true);
+  }
 }
 
 std::vector<int> MPO::getDimensions() const {
@@ -230,24 +308,26 @@ void Site::reduceDimL(const DevTensor& rightterm) {
 }
 
 // ------------------------------------------------------------------------------ MPS
-MPS::MPS(int length, int bond, int d, mpsd_ctx* ctx) : ctx_(ctx ? ctx : default_ctx()), bond_(bond) {
+MPS::MPS(int length, int bond, int d, mpsd_ctx* ctx) : MPS(length, bond, std::vector<int>(length, d), ctx) {}
+
+MPS::MPS(int length, int bond, const std::vector<int>& d, mpsd_ctx* ctx) : ctx_(ctx ? ctx : default_ctx()), bond_(bond) {
+  if ((int)d.size() < length) throw Error(MPSD_ERR_ARG, "MPS: fewer physical dimensions than sites");
   for (int k = 0; k < length; ++k) {
+    if (d[k] < 1) throw Error(MPSD_ERR_ARG, "MPS: physical dimension below 1");
     const int Dl = (k == 0) ? 1 : bond;
     const int Dr = (k == length - 1) ? 1 : bond;
-    A_.emplace_back(new Site(ctx_, k + 1, {d, Dl, Dr}));
+    A_.emplace_back(new Site(ctx_, k + 1, {d[k], Dl, Dr}));
   }
 }
 
 void MPS::setRandomState(uint64_t seed) {
   const int L = getLength();
+  // bond k is capped by the state-space size on either side: prod d[0..k-1] and prod d[k..L-1]
+  std::vector<int64_t> left(L + 1, 1), right(L + 1, 1);
+  for (int k = 0; k < L; ++k) left[k + 1] = std::min<int64_t>(left[k] * A_[k]->getd(), bond_);
+  for (int k = L - 1; k >= 0; --k) right[k] = std::min<int64_t>(right[k + 1] * A_[k]->getd(), bond_);
   std::vector<int> bd(L + 1);
-  for (int k = 0; k <= L; ++k) {
-    double cap = (double)bond_;
-    const int d = A_[std::min(k, L - 1)]->getd();
-    cap = std::min(cap, std::pow((double)d, std::min(k, 60)));
-    cap = std::min(cap, std::pow((double)d, std::min(L - k, 60)));
-    bd[k] = (int)cap;
-  }
+  for (int k = 0; k <= L; ++k) bd[k] = (int)std::min<int64_t>(std::min<int64_t>(left[k], right[k]), bond_);
   for (int k = 0; k < L; ++k) {
     const int d = A_[k]->getd();
     A_[k]->setA(complex_gaussian(seed * 1000003ull + (uint64_t)k, {d, bd[k], bd[k + 1]}));

@@ -102,6 +102,9 @@ def _optional(lib):
         "mpsd_gs_solve": (I, [P, C.POINTER(GsConfig), C.POINTER(GsResult)]),
         "mpsd_gs_solve_export": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
+        "mpsd_mpo_save": (I, [P, C.POINTER(GsConfig), C.c_char_p, I]),
+        "mpsd_gs_solve_mpo": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, C.POINTER(GsResult)]),
+        "mpsd_mpo_expectation": (I, [P, C.c_char_p, I, C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
         "mpsd_heff_apply_grouped": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "mpsd_gs_footprint": (I64, [C.POINTER(GsConfig)]),
@@ -240,6 +243,37 @@ def gs_solve(ctx, cfg, export_path=None):
     else:
         ctx.check(ctx.lib.mpsd_gs_solve_export(ctx.handle, C.byref(cfg), export_path.encode(), C.byref(res)))
     return res
+
+
+MPO_BINARY, MPO_TEXT = 0, 1
+
+
+def mpo_save(cfg, path, fmt=MPO_BINARY, ctx=None):
+    """Write the Hamiltonian MPO of cfg's model to `path` (MPO::saveMPO or exportMPOtext
+    layout).  Host-only: works without a GPU when ctx is None."""
+    lib = ctx.lib if ctx is not None else load()
+    rc = lib.mpsd_mpo_save(ctx.handle if ctx is not None else None, C.byref(cfg), str(path).encode(), fmt)
+    if ctx is not None:
+        ctx.check(rc)
+    elif rc != 0:
+        raise MpsdError(rc, "mpsd_mpo_save failed")
+
+
+def gs_solve_mpo(ctx, cfg, mpo_path, fmt=MPO_BINARY, export_path=None):
+    """Ground state of the Hamiltonian in an MPO file; cfg supplies D, tolerances, precision,
+    seed and solver.  Returns GsResult."""
+    res = GsResult()
+    ctx.check(ctx.lib.mpsd_gs_solve_mpo(ctx.handle, C.byref(cfg), str(mpo_path).encode(), fmt,
+                                        None if export_path is None else str(export_path).encode(), C.byref(res)))
+    return res
+
+
+def mpo_expectation(ctx, mpo_path, fmt, mps_path):
+    """<psi|H|psi> of an exported MPS for the MPO in an MPO file."""
+    re_, im_ = C.c_double(), C.c_double()
+    ctx.check(ctx.lib.mpsd_mpo_expectation(ctx.handle, str(mpo_path).encode(), fmt, str(mps_path).encode(),
+                                           C.byref(re_), C.byref(im_)))
+    return complex(re_.value, im_.value)
 
 
 def exported_symbols():

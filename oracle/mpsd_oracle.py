@@ -28,6 +28,12 @@ Pinning (how each function is checked; tests/test_oracle.py):
                        adjoint, independent of the conjugate-permuted copies), on
                        non-Hermitian operands where F != F^H.
   *_mpo builders    -- MPO contracted to a dense matrix against the Pauli-sum Hamiltonian.
+  chain_mpo, spin_ops -- dense contraction against Kronecker-product sums with mixed site
+                       dimensions; spin_ops against the spin-1/2 Pauli matrices and the
+                       commutation relations / S^2 = S(S+1) for S = 1, 3/2.
+  mpo_write/mpo_read -- a hand-written text file and a hand-assembled binary file laid out
+                       as the reference's save / savetext code writes them (known answers),
+                       and against the device library's own writer.
   find_ground_state -- reference energies recorded in tests/golden (config 1 value from
                        the compiled reference), exact diagonalisation and the open-chain
                        free-fermion closed form.
@@ -45,7 +51,8 @@ __all__ = [
     "reshape_f", "permute", "heff_product", "heff_full", "contract_left", "contract_right",
     "ising_mpo", "heisenberg_mpo", "schwinger_mpo", "random_mps", "gauge_right_all",
     "gauge_left_all", "find_ground_state", "complex_gaussian", "splitmix64",
-    "matvec_flops", "to_colmajor", "from_colmajor",
+    "matvec_flops", "to_colmajor", "from_colmajor", "mpo_write", "mpo_read", "spin_ops",
+    "chain_mpo",
 ]
 
 
@@ -325,6 +332,122 @@ def schwinger_mpo(N, mu, x, l0=0.0, alpha=0.0, offset=0.0, nu=0.0, y=1.0):
             C[1, 0, 1] = math.sqrt(x / 2)
             C[2, 0, 2] = math.sqrt(x / 2)
         mpo.append(_w_from_c(C, ops))
+    return mpo
+
+
+def spin_ops(S2):
+    """Spin-S matrices (Sx, Sy, Sz) for 2S = S2, basis m = S, S-1, ..., -S (d = S2 + 1):
+    <m+1|S+|m> = sqrt(S(S+1) - m(m+1))."""
+    S = S2 / 2.0
+    m = S - np.arange(S2 + 1)
+    sp_ = np.zeros((S2 + 1, S2 + 1), dtype=np.complex128)
+    for i in range(1, S2 + 1):
+        sp_[i - 1, i] = math.sqrt(S * (S + 1) - m[i] * (m[i] + 1))
+    sx = 0.5 * (sp_ + sp_.conj().T)
+    sy = -0.5j * (sp_ - sp_.conj().T)
+    return sx, sy, np.diag(m).astype(np.complex128)
+
+
+def chain_mpo(bond_ops, field_ops):
+    """Nearest-neighbour chain H = sum_k sum_t A_t(k) B_t(k+1) + sum_k F(k) as an MPO in the
+    reference's [du, Dl, dd, Dr] layout with the lower-triangular finite-automaton form the
+    reference's models use (IsingHamiltonian.cpp:70-121): bond index 0 = 'nothing yet',
+    1..T = 'A_t placed', T+1 = 'done'.  bond_ops[k] = list of (A_t, B_t) pairs for bond
+    (k, k+1), field_ops[k] = F on site k; site k's dimension is F's.  Sites may have
+    different dimensions."""
+    L = len(field_ops)
+    T = len(bond_ops[0]) if L > 1 else 0
+    w = T + 2
+    mpo = []
+    for k in range(L):
+        F = np.asarray(field_ops[k], dtype=np.complex128)
+        d = F.shape[0]
+        Dl = 1 if k == 0 else w
+        Dr = 1 if k == L - 1 else w
+        W = np.zeros((d, Dl, d, Dr), dtype=np.complex128)
+        last_r = Dr - 1
+        eye = np.eye(d, dtype=np.complex128)
+        if k != L - 1:
+            W[:, 0, :, 0] = eye
+            for t, (A, _) in enumerate(bond_ops[k]):
+                W[:, 0, :, 1 + t] = A
+        if k != 0:
+            W[:, Dl - 1, :, last_r] = eye
+            for t, (_, B) in enumerate(bond_ops[k - 1]):
+                W[:, 1 + t, :, last_r] = B
+        W[:, 0, :, last_r] += F
+        mpo.append(W)
+    return mpo
+
+
+# --------------------------------------------------------------------------- MPO files
+
+def mpo_write(path, mpo, fmt, digits=None):
+    """Write an MPO as MPO::saveMPO (fmt 'binary', src/core/cpp/MPO.cpp:214-232) or
+    MPO::exportMPOtext (fmt 'text', MPO.cpp:234-248).  Binary: int32 length, then per operator
+    mwArray::save (wrapper/mwArray.cpp:430-443): int32 rank, int32 dims[rank], then the
+    column-major data as interleaved (re, im) float64.  Text: "length " then per operator
+    mwArray::savetext (mwArray.cpp:1672-1688): "rank dims... re im ..." each followed by one
+    space; digits = significant digits (None: 6, the default stream precision the reference
+    writes with)."""
+    if fmt == "binary":
+        with open(path, "wb") as f:
+            f.write(np.int32(len(mpo)).tobytes())
+            for W in mpo:
+                W = np.asarray(W, dtype=np.complex128)
+                f.write(np.int32(W.ndim).tobytes())
+                f.write(np.asarray(W.shape, dtype=np.int32).tobytes())
+                f.write(to_colmajor(W).view(np.float64).tobytes())
+    elif fmt == "text":
+        p = 6 if digits is None else digits
+        parts = [f"{len(mpo)} "]
+        for W in mpo:
+            W = np.asarray(W, dtype=np.complex128)
+            parts.append(f"{W.ndim} " + "".join(f"{n} " for n in W.shape))
+            parts.append("".join(f"{v:.{p}g} " for v in to_colmajor(W).view(np.float64)))
+        with open(path, "w") as f:
+            f.write("".join(parts))
+    else:
+        raise ValueError(fmt)
+
+
+def mpo_read(path, fmt):
+    """Read a file written as by mpo_write (MPO::loadMPO, MPO.cpp:293-316, and
+    MPO::importMPOtext, MPO.cpp:250-271)."""
+    mpo = []
+    if fmt == "binary":
+        raw = open(path, "rb").read()
+        off = 0
+
+        def ints(n):
+            nonlocal off
+            v = np.frombuffer(raw, dtype=np.int32, count=n, offset=off)
+            off += 4 * n
+            return [int(x) for x in v]
+
+        (L,) = ints(1)
+        for _ in range(L):
+            (rank,) = ints(1)
+            dims = ints(rank)
+            n = int(np.prod(dims))
+            v = np.frombuffer(raw, dtype=np.float64, count=2 * n, offset=off).view(np.complex128)
+            off += 16 * n
+            mpo.append(from_colmajor(v.copy(), tuple(dims)))
+        if off != len(raw):
+            raise ValueError("trailing bytes")
+    elif fmt == "text":
+        tok = open(path).read().split()
+        pos = 0
+        L = int(tok[pos]); pos += 1
+        for _ in range(L):
+            rank = int(tok[pos]); pos += 1
+            dims = [int(t) for t in tok[pos:pos + rank]]; pos += rank
+            n = int(np.prod(dims))
+            v = np.array([float(t) for t in tok[pos:pos + 2 * n]], dtype=np.float64).view(np.complex128)
+            pos += 2 * n
+            mpo.append(from_colmajor(v, tuple(dims)))
+    else:
+        raise ValueError(fmt)
     return mpo
 
 

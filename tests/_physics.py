@@ -59,6 +59,28 @@ def schwinger_hamiltonian(N, mu, x, l0=0.0, alpha=0.0, offset=0.0, nu=0.0, y=1.0
     return H + offset * Id
 
 
+def op_on(op, k, dims):
+    """op acting on site k of a chain with per-site dimensions dims (site 0 fastest)."""
+    out = sp.identity(1, dtype=np.complex128, format="csr")
+    for j in range(len(dims) - 1, -1, -1):
+        out = sp.kron(out, sp.csr_matrix(op) if j == k else sp.identity(dims[j], dtype=np.complex128,
+                                                                          format="csr"), format="csr")
+    return out
+
+
+def chain_hamiltonian(bond_ops, field_ops):
+    """sum_k sum_t A_t(k) B_t(k+1) + sum_k F(k) as a sparse Kronecker sum (no MPO)."""
+    dims = [np.asarray(F).shape[0] for F in field_ops]
+    n = int(np.prod(dims))
+    H = sp.csr_matrix((n, n), dtype=np.complex128)
+    for k, F in enumerate(field_ops):
+        H = H + op_on(F, k, dims)
+    for k, terms in enumerate(bond_ops):
+        for A, B in terms:
+            H = H + op_on(A, k, dims) @ op_on(B, k + 1, dims)
+    return H
+
+
 def ground_energy(H):
     if H.shape[0] <= 512:
         return float(np.linalg.eigvalsh(H.toarray())[0])
@@ -84,7 +106,8 @@ def mpo_to_dense(mpo):
     ups = list(range(0, 2 * L, 2))
     dns = list(range(1, 2 * L, 2))
     T = np.transpose(T, ups[::-1] + dns[::-1])
-    return T.reshape(2 ** L, 2 ** L)
+    dim = int(np.prod([W.shape[0] for W in mpo]))
+    return T.reshape(dim, dim)
 
 
 def mps_to_state(mps):
