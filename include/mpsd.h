@@ -225,6 +225,12 @@ int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n,
  * (w = 0 when beta_j underflows). */
 int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev, int j, double* ab_dev,
                       double* w, int64_t n);
+/* Projector terms of TensorMultiplierProjMulti::product (src/wrapper/multipliers/
+ * TensorMultiplierProjMulti.cpp:87-96): y -= sum_j weights[j] <phi_j|x> phi_j, phi_j at
+ * Phi + 2*j*ldphi, j < m <= 32; weights is a host array of m reals; x and y distinct device
+ * vectors.  No host synchronisation. */
+int mpsd_vec_proj_update(mpsd_ctx* ctx, const double* Phi, int64_t ldphi, int m, const double* weights,
+                         const double* x, double* y, int64_t n);
 /* y = a*x + b*y with complex scalars a, b given as {re, im}. */
 int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double* b, double* y, int64_t n);
 /* y = x ; x = 0 (plain stores: a NaN in the old y does not survive). */

@@ -95,6 +95,28 @@ int mpsd_gs_solve_mpo(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_
 int mpsd_mpo_expectation(mpsd_ctx* ctx, const char* mpo_path, int format, const char* mps_path, double* re,
                          double* im);
 
+/* The lowest nlevels eigenstates of the Hamiltonian of cfg's model, or of the MPO file at
+ * mpo_path (format MPSD_MPO_*) when it is not NULL: level 0 by Contractor::findGroundState,
+ * level k > 0 by Contractor::findNextExcitedState against levels 0..k-1
+ * (src/core/cpp/Contractor.cpp:2676-2727), as the reference's excited-state programs chain
+ * them.  Each level starts from its own random state (seed cfg->seed + 7919 k) with bond
+ * cfg->D; excited levels are solved in FP64 with the Hermitian projector multiplier
+ * (TensorMultiplierProjMulti).  exc_orth_tol > 0 enables Contractor::setExcOrthTol.
+ * energies: nlevels values.  overlaps (may be NULL): nlevels x nlevels, column-major,
+ * |<level_i|level_j>|.  results (may be NULL): per-level statistics. */
+int mpsd_gs_solve_levels(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, int nlevels,
+                         double exc_orth_tol, double* energies, double* overlaps, mpsd_gs_result* results);
+
+/* One partial sweep of the MPS in file mps_in (exportMPS layout) over sites pos0 .. lim
+ * (excluded; moving right when lim >= pos0): Contractor::sweepPart (Contractor.cpp:1260-1350)
+ * when nproj = 0, else Contractor::sweepPartWithProjectorPenalty (Contractor.cpp:2575-2666) with
+ * the states in proj_paths and the multiplier weights `weights` (H - sum_k weights[k] |k><k|).
+ * The Hamiltonian is cfg's model or the MPO file at mpo_path.  energy (may be NULL) receives
+ * <psi|H|psi> / <psi|psi> of the result; mps_out (may be NULL) receives the result. */
+int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, const char* mps_in,
+                       int nproj, const char* const* proj_paths, const double* weights, int pos0, int lim,
+                       const char* mps_out, double* energy);
+
 /* Lowest eigenpair of an Heff by the device FP64 Lanczos solver.  Replaces
  * Contractor::solveEigs(multi, 1, "SR", ...) (src/core/cpp/Contractor.cpp:3067-3138) and the
  * eigensolver wrappers behind it (src/wrapper/eigs.cpp:71-422).  x_dev: device vector of

@@ -183,6 +183,7 @@ class MPS {
   void gaugeCond(char dir, bool normal = true);        // MPS.cpp:306-364
   void gaugeCond(int pos, char dir, bool next = false);  // MPS.cpp:366-419
   Gauge getGauge() const { return gauge_; }
+  double getNormFact() const { return normfact_; }
   // Binary format of MPS::exportMPS/importMPS (MPS.cpp:442-509, Site.cpp:561-578).
   void exportMPS(const char* filename) const;
   void importMPS(const char* filename);
@@ -216,6 +217,22 @@ class TmpManager {
  private:
   std::vector<DevTensor> L_, R_;
   std::vector<bool> upL_, upR_;
+};
+
+// TmpManagerExc (src/core/headers/TmpManagerExc.h): the operator environments of the state
+// being optimised plus, per projected level k, its overlap environments with that state.
+// One set of up-to-date flags (the operator slots') covers all of them, as they are
+// refreshed together.  Overlap slot k holds M[a, 1, l] = <state|level_k> contracted up to the
+// bond (a: the state's bond, l: the level's), the conjugate transpose of the reference's
+// normL/normR, so that the projected vector phi_k is a plain Heff product.
+struct TmpManagerExc {
+  TmpManagerExc(int length, int nlevels, mpsd_ctx* ctx) : ops(length, ctx) {
+    for (int k = 0; k < nlevels; ++k) norms.emplace_back(new TmpManager(length, ctx));
+  }
+  void outofdateL(int k) { ops.outofdateL(k); }
+  void outofdateR(int k) { ops.outofdateR(k); }
+  TmpManager ops;
+  std::vector<std::unique_ptr<TmpManager>> norms;
 };
 
 // ------------------------------------------------------------------------------ Multiplier
@@ -267,6 +284,38 @@ class TensorMultiplierHermitianOffset : public TensorMultiplierOffset {
  public:
   TensorMultiplierHermitianOffset(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W, const mpsd_tensor* R,
                                   double offset, int precision = MPSD_PREC_FP64);
+};
+
+// TensorMultiplierProjMulti (src/wrapper/multipliers/TensorMultiplierProjMulti.cpp:13-100):
+// product = B x - sum_k weights[k] <phi_k|x> phi_k + offset x, with B the plain Heff or, when
+// hermitian, its Hermitian average (Heff + Heff^H) / 2.  getFullTensor is that matrix,
+// averaged with its adjoint as in the reference.  The reference takes the projected vectors as
+// bras v_k (rows, from Contractor::contractMbraId); here they are the kets phi_k = v_k^H as
+// device vectors of getSize() complex numbers, which the caller keeps alive.
+class TensorMultiplierProjMulti : public Multiplier {
+ public:
+  TensorMultiplierProjMulti(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W, const mpsd_tensor* R,
+                            const std::vector<double>& weights, const std::vector<const double*>& phis,
+                            bool hermitian = false, double offset = 0.0);
+  int64_t getSize() const override { return base_->getSize(); }
+  void product_device(const double* x, double* y) override;
+  void getFullTensor(mwArray& result) override;
+  mpsd_ctx* ctx() const override { return ctx_; }
+
+ private:
+  mpsd_ctx* ctx_;
+  std::unique_ptr<TensorMultiplierOffset> base_;
+  std::vector<double> weights_;
+  std::vector<const double*> phis_;
+};
+
+// TensorMultiplierProj (TensorMultiplierProj.cpp:9-41): one projected vector, no offset:
+// product = B x - weight <phi|x> phi.
+class TensorMultiplierProj : public TensorMultiplierProjMulti {
+ public:
+  TensorMultiplierProj(mpsd_ctx* ctx, const mpsd_tensor* L, const mpsd_tensor* W, const mpsd_tensor* R, double weight,
+                       const double* phi, bool hermitian = false)
+      : TensorMultiplierProjMulti(ctx, L, W, R, {weight}, {phi}, hermitian, 0.0) {}
 };
 
 // ------------------------------------------------------------------------------ eigensolver
@@ -346,11 +395,65 @@ class Contractor {
                        int knr = 1, const std::string& tmpfile = "", int freqSv = 200, bool useLM = false);
   // <bra|ops|ket> via a full left-environment sweep (Contractor.cpp:528).
   complex_t contract(const MPS& ket, const MPO& ops, const MPS& bra);
+  // <bra|ket> times both norm factors (Contractor::contract(ket, bra), Contractor.cpp:467-499).
+  complex_t contract(const MPS& ket, const MPS& bra);
   const SolveStats& lastStats() const { return stats_; }
+
+  // Part of a ground-state sweep, sites pos0 .. lim (excluded), moving right when lim >= pos0
+  // (Contractor::sweepPart, Contractor.cpp:1260-1350).  Sites before pos0 are right-gauged
+  // and sites after it left-gauged first; each site's gauge factor is dropped except at the
+  // last step, as in the reference.  The start state's bond dimension is used as it is.
+  void sweepPart(int pos0, int lim, const MPO& ops, int D, MPS& init, double offset = 0., int knr = 1);
+
+  // Excited states by projector penalties (Contractor.cpp:2561-2957).  The local problem at
+  // each site is TensorMultiplierProjMulti in its Hermitian form with the projected levels'
+  // effective vectors, solved by the device Lanczos solver whatever setEigenSolver says.
+  //  * findGroundStateWithProjectorPenalty: ground state of H + sum_k penalties[k] |k><k|
+  //    (the reference's `energies` argument, Contractor.cpp:2739-2956).
+  //  * findNextExcitedState: the penalties are max(20 (maxE - minE), |offset|) over the given
+  //    levels' energies, or L/2 for a single (or degenerate) level; with setExcOrthTol > 0 the
+  //    solve repeats, doubling the penalty of each level the result overlaps by more than that
+  //    (Contractor.cpp:2676-2727).
+  //  * sweepPartWithProjectorPenalty: sweepPart on H - sum_k weights[k] |k><k| (the sign the
+  //    reference's multiplier gives its `energies` there, Contractor.cpp:2575-2666);
+  //    sweepPartNextExcitedState uses weights = the levels' energies + offset;
+  //    sweepPartWithProjPenalty adds mu |projected><projected| (Contractor.cpp:3640-3734).
+  // The largest-magnitude variants (useLM) are not part of this path.
+  void findGroundStateWithProjectorPenalty(const MPO& ops, int D, const std::vector<double>& penalties,
+                                           const std::vector<MPS*>& projected, double* lambdak, MPS& init,
+                                           double offset = 0., int knr = 1, const std::string& tmpfile = "",
+                                           int maxTime = 0, bool useLM = false);
+  void findNextExcitedState(const MPO& ops, int D, const std::vector<MPS*>& computedLevels, double* lambdak,
+                            MPS& init, double offset = 0., int knr = 1, const std::string& tmpfile = "",
+                            int maxTime = 0, bool useLM = false);
+  void sweepPartWithProjectorPenalty(int pos0, int lim, const MPO& ops, int D, const std::vector<double>& weights,
+                                     const std::vector<MPS*>& projected, MPS& init, double offset = 0., int knr = 1);
+  void sweepPartNextExcitedState(int pos0, int lim, const MPO& ops, int D, const std::vector<MPS*>& computedLevels,
+                                 MPS& init, double offset = 0., int knr = 1);
+  void sweepPartWithProjPenalty(int pos0, int lim, const MPO& ops, int D, double mu, MPS& projected, MPS& init,
+                                double offset = 0., int knr = 1);
+  void setExcOrthTol(double v) { excorthtol_ = v; }
+  double getExcOrthTol() const { return excorthtol_; }
 
  private:
   void calculateL(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra);
   void calculateR(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra);
+  void calculateLexck(int pos, TmpManagerExc& mgr, const MPO& ops, const MPS& ket, const std::vector<MPS*>& levels);
+  void calculateRexck(int pos, TmpManagerExc& mgr, const MPO& ops, const MPS& ket, const std::vector<MPS*>& levels);
+  // phi_k for every level at pos into phis_ (one column per level); returns the column pointers
+  std::vector<const double*> effectiveVectors(int pos, TmpManagerExc& mgr, const MPS& init,
+                                              const std::vector<MPS*>& levels);
+  // the [d, 1, d, 1] identity operator on this context
+  const mpsd_tensor* identityOp(int d);
+  // one site of a projector-penalty sweep: solve, store, return the eigenvalue minus offset
+  double solvePenaltySite(int pos, TmpManagerExc& mgr, const MPO& ops, const std::vector<double>& weights,
+                          const std::vector<MPS*>& levels, MPS& init, double offset);
+  void penaltySweepPart(int pos0, int lim, const MPO& ops, const std::vector<double>& weights,
+                        const std::vector<MPS*>& levels, MPS& init, double offset, bool passAlways);
+  void checkLevels(const MPO& ops, const MPS& init, const std::vector<MPS*>& levels) const;
+  double excorthtol_ = 0.0;
+  std::vector<std::pair<int, std::unique_ptr<Operator>>> ident_;
+  DevTensor phis_;
   mpsd_ctx* ctx_;
   double convtol_ = 1e-5, eigtol_ = 0.0;
   EigSolver solver_ = primme;

@@ -54,6 +54,43 @@ void read_mpo(MPO& mpo, const char* path, int format) {
   if (mpo.getOp(L - 1).getDimension(3) != 1) throw Error(MPSD_ERR_SHAPE, "MPO file: last bond is not 1");
 }
 
+// The Hamiltonian of a call: read from an MPO file when mpo_path is set, else cfg's model.
+const MPO& load_hamiltonian(const mpsd_gs_config* cfg, const char* mpo_path, int format,
+                            std::unique_ptr<Hamiltonian>& model, MPO& file) {
+  if (mpo_path) {
+    read_mpo(file, mpo_path, format);
+    return file;
+  }
+  if (cfg->L < 2) throw Error(MPSD_ERR_ARG, "need L >= 2");
+  model = make_model(cfg);
+  return model->getHMPO();
+}
+
+void configure(Contractor& c, const mpsd_gs_config* cfg) {
+  if (cfg->solver != MPSD_SOLVER_LANCZOS && cfg->solver != MPSD_SOLVER_FULLEIG)
+    throw Error(MPSD_ERR_ARG, "unknown solver");
+  c.setConvTol(cfg->convtol > 0 ? cfg->convtol : 1e-5);
+  c.setEigTol(cfg->eigtol);
+  c.setMaxHalfSweeps(cfg->max_halfsweeps);
+  c.setPrecision(cfg->precision);
+  if (cfg->solver == MPSD_SOLVER_FULLEIG) c.setEigenSolver(fulleig);
+}
+
+void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, mpsd_gs_result* out) {
+  const SolveStats& st = c.lastStats();
+  out->energy = E;
+  out->halfsweeps = st.halfsweeps;
+  out->matvecs = st.matvecs;
+  out->env_updates = st.env_updates;
+  out->residual = st.final_residual;
+  out->seconds = seconds;
+  out->device_bytes = mpsd_ctx_device_bytes(ctx);
+  out->matvecs_lowp = st.matvecs_lowp;
+  out->refine_from = st.refine_from;
+  out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
+  for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+}
+
 }  // namespace
 
 namespace mpsdyn {
@@ -226,6 +263,88 @@ int mpsd_mpo_expectation(mpsd_ctx* ctx, const char* mpo_path, int format, const 
     complex_t e = c.contract(mps, mpo, mps);
     *re = e.real();
     *im = e.imag();
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_gs_solve_levels(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, int nlevels,
+                         double exc_orth_tol, double* energies, double* overlaps, mpsd_gs_result* results) {
+  if (!ctx || !cfg || !energies) return MPSD_ERR_ARG;
+  if (nlevels < 1 || nlevels > 32) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve_levels: need 1 <= nlevels <= 32");
+  if (cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve_levels: need D >= 1");
+  try {
+    std::unique_ptr<Hamiltonian> model;
+    MPO file;
+    const MPO& H = load_hamiltonian(cfg, mpo_path, format, model, file);
+    Contractor c(ctx);
+    configure(c, cfg);
+    c.setExcOrthTol(exc_orth_tol);
+    std::vector<std::unique_ptr<MPS>> levels;
+    std::vector<MPS*> done;
+    for (int k = 0; k < nlevels; ++k) {
+      std::unique_ptr<MPS> m(new MPS(H.getLength(), cfg->D, H.getDimensions(), ctx));
+      m->setRandomState(cfg->seed + 7919ull * (uint64_t)k);
+      double E = 0.0;
+      auto t0 = std::chrono::steady_clock::now();
+      if (k == 0)
+        c.findGroundState(H, cfg->D, &E, *m, cfg->offset);
+      else
+        c.findNextExcitedState(H, cfg->D, done, &E, *m, cfg->offset);
+      check(ctx, mpsd_ctx_sync(ctx), "sync");
+      auto t1 = std::chrono::steady_clock::now();
+      energies[k] = E;
+      if (results) {
+        memset(&results[k], 0, sizeof(results[k]));
+        fill_result(ctx, c, E, std::chrono::duration<double>(t1 - t0).count(), &results[k]);
+      }
+      levels.push_back(std::move(m));
+      done.push_back(levels.back().get());
+    }
+    if (overlaps)
+      for (int j = 0; j < nlevels; ++j)
+        for (int i = 0; i < nlevels; ++i)
+          overlaps[i + nlevels * j] = std::abs(c.contract(*levels[i], *levels[j]));
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, const char* mps_in,
+                       int nproj, const char* const* proj_paths, const double* weights, int pos0, int lim,
+                       const char* mps_out, double* energy) {
+  if (!ctx || !cfg || !mps_in || nproj < 0 || (nproj > 0 && (!proj_paths || !weights))) return MPSD_ERR_ARG;
+  try {
+    std::unique_ptr<Hamiltonian> model;
+    MPO file;
+    const MPO& H = load_hamiltonian(cfg, mpo_path, format, model, file);
+    MPS init(1, 1, 2, ctx);
+    init.importMPS(mps_in);
+    std::vector<std::unique_ptr<MPS>> proj;
+    std::vector<MPS*> ptrs;
+    for (int k = 0; k < nproj; ++k) {
+      if (!proj_paths[k]) throw Error(MPSD_ERR_ARG, "gs_sweep_part: null projected-state path");
+      proj.emplace_back(new MPS(1, 1, 2, ctx));
+      proj.back()->importMPS(proj_paths[k]);
+      ptrs.push_back(proj.back().get());
+    }
+    Contractor c(ctx);
+    configure(c, cfg);
+    if (nproj == 0)
+      c.sweepPart(pos0, lim, H, cfg->D, init, cfg->offset);
+    else
+      c.sweepPartWithProjectorPenalty(pos0, lim, H, cfg->D, std::vector<double>(weights, weights + nproj), ptrs, init,
+                                      cfg->offset);
+    if (energy) *energy = c.contract(init, H, init).real() / c.contract(init, init).real();
+    if (mps_out) init.exportMPS(mps_out);
     return MPSD_OK;
   } catch (const Error& e) {
     return fail(ctx, e);

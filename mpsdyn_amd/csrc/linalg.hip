@@ -47,6 +47,27 @@ __global__ void __launch_bounds__(kT) gemv_update_kernel(const double2* __restri
   if (i < n) gemv_elem(V, ldv, m, c, sign, w, i);
 }
 
+// y[i] -= sum_j w_j h_j Phi[j*ldp + i]: the projector terms of TensorMultiplierProjMulti with
+// h_j = <phi_j|x> already in device memory.
+constexpr int kProjMax = 32;
+struct ProjWeights {
+  double w[kProjMax];
+};
+__global__ void __launch_bounds__(kT) proj_update_kernel(const double2* __restrict__ Phi, int64_t ldp, int m,
+                                                         const double2* __restrict__ h, ProjWeights pw,
+                                                         double2* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = y[i];
+  for (int j = 0; j < m; ++j) {
+    const double2 hj = h[j], p = Phi[(int64_t)j * ldp + i];
+    const double2 c{pw.w[j] * hj.x, pw.w[j] * hj.y};
+    acc.x -= c.x * p.x - c.y * p.y;
+    acc.y -= c.x * p.y + c.y * p.x;
+  }
+  y[i] = acc;
+}
+
 // Lanczos coefficients of step j from the two Gram-Schmidt passes (h1 = V^H w before the
 // first pass, h2 = [V_0..V_j, w]^H w after it): alpha_j = Re(h1[j] + h2[j]),
 // beta_j = sqrt(|w|^2 - sum_i |h2[i]|^2) (the norm after the second pass), then
@@ -314,6 +335,28 @@ int mpsd_vec_gemv_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, const 
   hipLaunchKernelGGL(gemv_update_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)V, ldv, m,
                      (const double2*)c_dev, sign, (double2*)w, n);
   return hip_err(ctx, hipGetLastError(), "vec_gemv_dev");
+}
+
+int mpsd_vec_proj_update(mpsd_ctx* ctx, const double* Phi, int64_t ldphi, int m, const double* weights,
+                         const double* x, double* y, int64_t n) {
+  if (!ctx || (m > 0 && (!Phi || !weights)) || !x || !y || m < 0 || n < 0 || (m > 1 && ldphi < n))
+    return MPSD_ERR_ARG;
+  if (m > kProjMax) return set_err(ctx, MPSD_ERR_ARG, "vec_proj_update: at most 32 projected vectors");
+  if (m == 0 || n == 0) return MPSD_OK;
+  if (x == y) return set_err(ctx, MPSD_ERR_ARG, "vec_proj_update: x and y must differ");
+  int rc = ensure_dots(ctx, m);
+  if (rc) return rc;
+  double2* partial = (double2*)ctx->dots;
+  double2* h = partial + (size_t)m * kDotBlocks;
+  const int nb = dot_blocks(n);
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, m), dim3(kT), 0, ctx->stream, (const double2*)Phi, ldphi,
+                     (const double2*)x, n, nb, partial);
+  hipLaunchKernelGGL(dots_final_kernel, dim3(m), dim3(kDotBlocks), 0, ctx->stream, partial, nb, h);
+  ProjWeights pw{};
+  for (int j = 0; j < m; ++j) pw.w[j] = weights[j];
+  hipLaunchKernelGGL(proj_update_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)Phi, ldphi, m,
+                     (const double2*)h, pw, (double2*)y, n);
+  return hip_err(ctx, hipGetLastError(), "vec_proj_update");
 }
 
 int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev, int j, double* ab_dev, double* w,

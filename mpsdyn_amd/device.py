@@ -103,6 +103,11 @@ def _optional(lib):
         "mpsd_gs_solve_export": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_mpo_save": (I, [P, C.POINTER(GsConfig), C.c_char_p, I]),
+        "mpsd_vec_proj_update": (I, [P, P, I64, I, C.POINTER(D), P, P, I64]),
+        "mpsd_gs_solve_levels": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, I, D, C.POINTER(D), C.POINTER(D),
+                                     C.POINTER(GsResult)]),
+        "mpsd_gs_sweep_part": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, I, C.POINTER(C.c_char_p),
+                                   C.POINTER(D), I, I, C.c_char_p, C.POINTER(D)]),
         "mpsd_gs_solve_mpo": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_mpo_expectation": (I, [P, C.c_char_p, I, C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
@@ -274,6 +279,30 @@ def mpo_expectation(ctx, mpo_path, fmt, mps_path):
     ctx.check(ctx.lib.mpsd_mpo_expectation(ctx.handle, str(mpo_path).encode(), fmt, str(mps_path).encode(),
                                            C.byref(re_), C.byref(im_)))
     return complex(re_.value, im_.value)
+
+
+def gs_solve_levels(ctx, cfg, nlevels, mpo_path=None, fmt=MPO_BINARY, exc_orth_tol=0.0):
+    """Ground state and the next nlevels-1 excited states (findGroundState, then
+    findNextExcitedState).  Returns (energies, |<i|j>| matrix, per-level GsResult list)."""
+    e = (C.c_double * nlevels)()
+    ov = (C.c_double * (nlevels * nlevels))()
+    res = (GsResult * nlevels)()
+    ctx.check(ctx.lib.mpsd_gs_solve_levels(ctx.handle, C.byref(cfg), None if mpo_path is None else str(mpo_path).encode(),
+                                           fmt, nlevels, exc_orth_tol, e, ov, res))
+    return (np.array(e[:]), np.array(ov[:]).reshape(nlevels, nlevels, order="F"), [res[k] for k in range(nlevels)])
+
+
+def gs_sweep_part(ctx, cfg, mps_in, pos0, lim, mps_out=None, proj=(), weights=(), mpo_path=None, fmt=MPO_BINARY):
+    """Contractor::sweepPart (no proj) or sweepPartWithProjectorPenalty on an exported MPS;
+    returns <psi|H|psi>/<psi|psi> of the result."""
+    n = len(proj)
+    paths = (C.c_char_p * max(n, 1))(*[str(p).encode() for p in proj])
+    w = (C.c_double * max(n, 1))(*[float(x) for x in weights])
+    e = C.c_double()
+    ctx.check(ctx.lib.mpsd_gs_sweep_part(ctx.handle, C.byref(cfg), None if mpo_path is None else str(mpo_path).encode(),
+                                         fmt, str(mps_in).encode(), n, paths, w, pos0, lim,
+                                         None if mps_out is None else str(mps_out).encode(), C.byref(e)))
+    return e.value
 
 
 def exported_symbols():

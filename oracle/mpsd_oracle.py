@@ -31,6 +31,12 @@ Pinning (how each function is checked; tests/test_oracle.py):
   chain_mpo, spin_ops -- dense contraction against Kronecker-product sums with mixed site
                        dimensions; spin_ops against the spin-1/2 Pauli matrices and the
                        commutation relations / S^2 = S(S+1) for S = 1, 3/2.
+  overlap_left/right, effective_bra, proj_multi_* -- overlaps against dense state vectors; the
+                       effective vector against P^H |level> and the projector multiplier
+                       against P^H (H + sum p|k><k|) P, P the brute-force site isometry.
+  sweep_part, penalty_sweep_part, find_*_excited_state -- ED excited energies; alternating
+                       partial sweeps reach the ED ground energy; penalised energy decreases.
+  mps_write/mps_read -- a hand-assembled exportMPS file (known answer).
   mpo_write/mpo_read -- a hand-written text file and a hand-assembled binary file laid out
                        as the reference's save / savetext code writes them (known answers),
                        and against the device library's own writer.
@@ -52,7 +58,9 @@ __all__ = [
     "ising_mpo", "heisenberg_mpo", "schwinger_mpo", "random_mps", "gauge_right_all",
     "gauge_left_all", "find_ground_state", "complex_gaussian", "splitmix64",
     "matvec_flops", "to_colmajor", "from_colmajor", "mpo_write", "mpo_read", "spin_ops",
-    "chain_mpo",
+    "chain_mpo", "mps_write", "mps_read", "overlap_left", "overlap_right", "effective_bra",
+    "proj_multi_product", "proj_multi_full", "sweep_part", "penalty_sweep_part",
+    "find_ground_state_with_projector_penalty", "find_next_excited_state", "overlap",
 ]
 
 
@@ -633,3 +641,249 @@ def find_ground_state(mpo, mps, convtol=1e-12, offset=0.0, max_halfsweeps=200, d
             else:
                 pos -= 1
     return energy, mps, rounds, history
+
+
+# --------------------------------------------------------------------------- MPS files
+
+def mps_write(path, mps, bond=None, normfact=1.0, gauge=0):
+    """MPS::exportMPS (src/core/cpp/MPS.cpp:440-459): int32 length, int32 bond, float64
+    normfact, int32 gauge, then per site Site::save (Site.cpp:570-578): int32 position
+    (1-based) and mwArray::save of A[d, Dl, Dr]."""
+    bond = max(max(a.shape[1], a.shape[2]) for a in mps) if bond is None else bond
+    with open(path, "wb") as f:
+        f.write(np.int32(len(mps)).tobytes() + np.int32(bond).tobytes() + np.float64(normfact).tobytes()
+                + np.int32(gauge).tobytes())
+        for k, A in enumerate(mps):
+            A = np.asarray(A, dtype=np.complex128)
+            f.write(np.int32(k + 1).tobytes() + np.int32(3).tobytes() + np.asarray(A.shape, dtype=np.int32).tobytes())
+            f.write(to_colmajor(A).view(np.float64).tobytes())
+
+
+def mps_read(path):
+    """MPS::importMPS (MPS.cpp:487-509).  Returns (sites, normfact)."""
+    raw = open(path, "rb").read()
+    L, _bond = np.frombuffer(raw, dtype=np.int32, count=2, offset=0)
+    normfact = float(np.frombuffer(raw, dtype=np.float64, count=1, offset=8)[0])
+    off = 20
+    sites = []
+    for _ in range(int(L)):
+        _pos, rank = np.frombuffer(raw, dtype=np.int32, count=2, offset=off)
+        dims = [int(x) for x in np.frombuffer(raw, dtype=np.int32, count=int(rank), offset=off + 8)]
+        off += 8 + 4 * int(rank)
+        n = int(np.prod(dims))
+        v = np.frombuffer(raw, dtype=np.float64, count=2 * n, offset=off).view(np.complex128)
+        off += 16 * n
+        sites.append(from_colmajor(v.copy(), tuple(dims)))
+    return sites, normfact
+
+
+# --------------------------------------------------------------------------- partial sweeps
+
+def _gauge_site(mps, pos, direction, nxt):
+    """MPS::gaugeCond(pos, dir, next) (MPS.cpp:366-419) on a list of sites, in place: the
+    site becomes an isometry; its factor goes to the neighbour when next is set or the bond
+    shrank.  A 1x1 factor's phase stays in the site (Site.cpp:202-207, 232-237)."""
+    L = len(mps)
+    if direction == "R":
+        old = mps[pos].shape[2]
+        q, r = _qr_site(mps[pos])
+        if r.shape == (1, 1) and r[0, 0] != 0:
+            q, r = q * (r[0, 0] / abs(r[0, 0])), np.abs(r)
+        mps[pos] = q
+        if pos < L - 1 and (q.shape[2] != old or nxt):
+            mps[pos + 1] = _absorb_right(mps[pos + 1], r)
+    else:
+        old = mps[pos].shape[1]
+        q, lm = _lq_site(mps[pos])
+        if lm.shape == (1, 1) and lm[0, 0] != 0:
+            q, lm = q * (lm[0, 0] / abs(lm[0, 0])), np.abs(lm)
+        mps[pos] = q
+        if pos > 0 and (q.shape[1] != old or nxt):
+            mps[pos - 1] = _absorb_left(mps[pos - 1], lm)
+
+
+def _envs_at(mps, mpo, pos):
+    """Operator environments of site pos (Contractor::calculateL/R, Contractor.cpp:723-764)."""
+    one = np.ones((1, 1, 1), dtype=np.complex128)
+    EL = one
+    for k in range(pos):
+        EL = contract_left(EL, mps[k], mpo[k], mps[k])
+    ER = one
+    for k in range(len(mps) - 1, pos, -1):
+        ER = contract_right(ER, mps[k], mpo[k], mps[k])
+    return EL, ER
+
+
+def sweep_part(mpo, mps, pos0, lim, offset=0.0):
+    """Contractor::sweepPart (Contractor.cpp:1260-1350): sites before pos0 right-gauged, after
+    it left-gauged, then solve sites pos0 .. lim (excluded), passing the gauge factor on only at
+    the last step.  Local problems by dense diagonalisation.  Returns (mps, last value)."""
+    mps = [a.copy() for a in mps]
+    L = len(mps)
+    right = lim >= pos0
+    for k in range(pos0):
+        _gauge_site(mps, k, "R", True)
+    for k in range(L - 1, pos0, -1):
+        _gauge_site(mps, k, "L", True)
+    pos, val = pos0, None
+    while True:
+        EL, ER = _envs_at(mps, mpo, pos)
+        val, vec = _local_ground(EL, mpo[pos], ER, to_colmajor(mps[pos]), offset, dense_below=10 ** 9)
+        val -= offset
+        mps[pos] = from_colmajor(vec, mps[pos].shape)
+        pass_on = (right and pos == lim - 1) or (not right and pos == lim + 1) or pos == lim
+        _gauge_site(mps, pos, "R" if right else "L", pass_on)
+        pos += 1 if right else -1
+        if (pos >= lim) if right else (pos <= lim):
+            return mps, val
+
+
+# --------------------------------------------------------------------------- projector penalties
+
+def overlap_left(N, ket, bra):
+    """Contractor::contractL (Contractor.cpp:819-838): N'[r_bra, r_ket] =
+    sum conj(bra[s, l, r]) N[l, l'] ket[s, l', r']."""
+    d, Dl, Dr = bra.shape
+    dp, Dlp, Drp = ket.shape
+    res = N @ reshape_f(permute(ket, (2, 1, 3)), (Dlp, dp * Drp))
+    res = reshape_f(res, (Dl * d, Drp))
+    aux = reshape_f(permute(bra, (3, 2, 1)).conj(), (Dr, Dl * d))
+    return aux @ res
+
+
+def overlap_right(N, ket, bra):
+    """Contractor::contractR (Contractor.cpp:840-858): N'[l_bra, l_ket] =
+    sum conj(bra[s, l, r]) N[r, r'] ket[s, l', r']."""
+    d, Dl, Dr = bra.shape
+    dp, Dlp, Drp = ket.shape
+    res = reshape_f(permute(bra, (2, 1, 3)).conj(), (Dl * d, Dr)) @ N
+    res = reshape_f(res, (Dl, d * Drp))
+    return res @ reshape_f(permute(ket, (1, 3, 2)), (dp * Drp, Dlp))
+
+
+def effective_bra(NL, bra, NR):
+    """Contractor::contractMbraId (Contractor.cpp:880-903): the row vector
+    v[s, a, c] = sum conj(bra[s, l, r]) NL[l, a] NR[r, c], flat column-major."""
+    dp, Dlp, Drp = bra.shape
+    res = reshape_f(bra.conj(), (dp * Dlp, Drp)) @ NR
+    res = reshape_f(permute(reshape_f(res, (dp, Dlp, NR.shape[1])), (1, 3, 2)), (dp * NR.shape[1], Dlp))
+    res = res @ NL
+    res = permute(reshape_f(res, (dp, NR.shape[1], NL.shape[1])), (1, 3, 2))
+    return reshape_f(res, (-1,))
+
+
+def proj_multi_product(L, W, R, x, weights, vbras, hermitian=True, offset=0.0):
+    """TensorMultiplierProjMulti::product (TensorMultiplierProjMulti.cpp:87-96): the basic
+    (Hermitian-averaged when hermitian) product, minus weight_k (v_k . x) v_k^H, plus offset x."""
+    x = np.asarray(x, dtype=np.complex128)
+    y = heff_product_hermitian(L, W, R, x) if hermitian else heff_product(L, W, R, x)
+    for w, v in zip(weights, vbras):
+        y = y - w * (v @ x) * v.conj()
+    return y + offset * x
+
+
+def proj_multi_full(L, W, R, weights, vbras, hermitian=True, offset=0.0):
+    """TensorMultiplierProjMulti::getFullTensor (TensorMultiplierProjMulti.cpp:98-106)."""
+    F = heff_full(L, W, R)
+    if hermitian:
+        F = 0.5 * (F + F.conj().T)
+    for w, v in zip(weights, vbras):
+        F = F - w * np.outer(v.conj(), v)
+    F = F + offset * np.eye(F.shape[0])
+    return 0.5 * (F + F.conj().T)
+
+
+def _norm_envs_at(mps, level, pos):
+    """normL[k][pos], normR[k][pos] of TmpManagerExc (Contractor.cpp:2958-2989): overlaps of
+    the level (bra) with the state (ket) up to site pos."""
+    NL = np.ones((1, 1), dtype=np.complex128)
+    for k in range(pos):
+        NL = overlap_left(NL, mps[k], level[k])
+    NR = np.ones((1, 1), dtype=np.complex128)
+    for k in range(len(mps) - 1, pos, -1):
+        NR = overlap_right(NR, mps[k], level[k])
+    return NL, NR
+
+
+def overlap(ket, bra):
+    """<bra|ket> (Contractor::contract(ket, bra), Contractor.cpp:467-499), unit norm factors."""
+    N = np.ones((1, 1), dtype=np.complex128)
+    for k in range(len(ket)):
+        N = overlap_left(N, ket[k], bra[k])
+    return complex(N[0, 0])
+
+
+def _penalty_site(mpo, mps, levels, weights, pos, offset):
+    EL, ER = _envs_at(mps, mpo, pos)
+    vb = []
+    for lev in levels:
+        NL, NR = _norm_envs_at(mps, lev, pos)
+        vb.append(effective_bra(NL, lev[pos], NR))
+    F = proj_multi_full(EL, mpo[pos], ER, weights, vb, True, offset)
+    vals, vecs = np.linalg.eigh(F)
+    return float(vals[0]) - offset, vecs[:, 0]
+
+
+def penalty_sweep_part(mpo, mps, weights, levels, pos0, lim, offset=0.0, pass_always=False):
+    """Contractor::sweepPartWithProjectorPenalty (Contractor.cpp:2575-2666): sweepPart on
+    H - sum_k weights[k] |k><k| (the weights enter the multiplier as given);
+    pass_always gives sweepPartWithProjPenalty (Contractor.cpp:3640-3734)."""
+    mps = [a.copy() for a in mps]
+    L = len(mps)
+    right = lim >= pos0
+    for k in range(pos0):
+        _gauge_site(mps, k, "R", True)
+    for k in range(L - 1, pos0, -1):
+        _gauge_site(mps, k, "L", True)
+    pos, val = pos0, None
+    while True:
+        val, vec = _penalty_site(mpo, mps, levels, weights, pos, offset)
+        mps[pos] = from_colmajor(vec, mps[pos].shape)
+        pass_on = pass_always or (right and pos == lim - 1) or (not right and pos == lim + 1) or pos == lim
+        _gauge_site(mps, pos, "R" if right else "L", pass_on)
+        pos += 1 if right else -1
+        if (pos >= lim) if right else (pos <= lim):
+            return mps, val
+
+
+def find_ground_state_with_projector_penalty(mpo, mps, penalties, levels, convtol=1e-12, offset=0.0,
+                                             max_halfsweeps=400):
+    """Contractor::findGroundStateWithProjectorPenalty (Contractor.cpp:2739-2956): ground state
+    of H + sum_k penalties[k] |k><k| by full sweeps, each site's problem the Hermitian
+    projector multiplier with weights -penalties.  Returns (energy, mps, halfsweeps)."""
+    L = len(mps)
+    mps = gauge_left_all(gauge_right_all(mps))
+    energy = expectation(mps, mpo).real + offset
+    for p, lev in zip(penalties, levels):
+        energy += p * abs(overlap(mps, lev)) ** 2
+    energy -= offset
+    weights = [-p for p in penalties]
+    pos, right, rounds = 0, True, 0
+    while True:
+        val, vec = _penalty_site(mpo, mps, levels, weights, pos, offset)
+        mps[pos] = from_colmajor(vec, mps[pos].shape)
+        _gauge_site(mps, pos, "R" if right else "L", True)
+        end = (right and pos == L - 1) or (not right and pos == 0)
+        if end:
+            done = abs(energy - val) < convtol * abs(energy + offset) or abs(energy - val) < 1e-12
+            done = done or rounds + 1 >= max_halfsweeps
+            energy = val
+            rounds += 1
+            if done:
+                return energy, mps, rounds
+            _gauge_site(mps, pos, "L" if right else "R", True)
+            pos += -1 if right else 1
+            right = not right
+        else:
+            pos += 1 if right else -1
+
+
+def find_next_excited_state(mpo, mps, levels, convtol=1e-12, offset=0.0):
+    """Contractor::findNextExcitedState (Contractor.cpp:2676-2727) without the orthogonality
+    retry: penalty max(20 (maxE - minE), |offset|), or L/2 for one (or degenerate) level."""
+    es = [expectation(lev, mpo).real + offset for lev in levels]
+    pen = max(20 * (max(es) - min(es)), abs(offset))
+    if len(levels) == 1 or pen < 1e-6:
+        pen = len(levels[0]) * 0.5
+    return find_ground_state_with_projector_penalty(mpo, mps, [pen] * len(levels), levels, convtol, offset)
+
