@@ -250,6 +250,15 @@ int mpsd_site_gauge_r(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out);
 /* site := Q of the LQ of site permuted to Dl x (d*Dr); L_out := L (Dl x k).  Replaces
  * Site::gaugeL (Site.cpp:210-238) over wrapper::lq (src/wrapper/qr.cpp:113-188). */
 int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out);
+/* Site::gaugeR / gaugeL with cutD > 0 (Site.cpp:178-238): an SVD (rocSOLVER zgesvd) in place of
+ * the QR/LQ, truncated as wrapper::svd does with tol = 0 and Xi = cutD (src/wrapper/svd.cpp:
+ * 23-118): the leading singular values above zero among the first min(cutD, k) are kept.
+ * gauge_r_svd: site (d*Dl x Dr) = U S V^H; the site becomes U [d, Dl, X] and R_out = S V^H
+ * (X x Dr).  gauge_l_svd: M[a, s + d c] = U S V^H; L_out = U S (Dl x X) and the site becomes
+ * V^H [d, X, Dr].  The 1x1 edge factor's phase stays in the site, as with the QR forms.
+ * Synchronises the stream (the kept count is decided on the host). */
+int mpsd_site_gauge_r_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out, int cutD);
+int mpsd_site_gauge_l_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, int cutD);
 /* site[s, :, c] := R . site[s, :, c]  -- Site::reduceDimR (Site.cpp:240-261). */
 int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R);
 /* site[s, a, :] := site[s, a, :] . L  -- Site::reduceDimL (Site.cpp:263-283). */

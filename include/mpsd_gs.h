@@ -117,6 +117,11 @@ int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo
                        int nproj, const char* const* proj_paths, const double* weights, int pos0, int lim,
                        const char* mps_out, double* energy);
 
+/* MPS(const MPS&, D) (src/core/cpp/MPS.cpp:761-773) on an exported state: the copy is cut to
+ * bond D by gaugeCond('L', false, D), truncated SVDs from the right with the norm kept in the
+ * norm factor; written to mps_out in exportMPS layout. */
+int mpsd_mps_truncate(mpsd_ctx* ctx, const char* mps_in, int D, const char* mps_out);
+
 /* Lowest eigenpair of an Heff by the device FP64 Lanczos solver.  Replaces
  * Contractor::solveEigs(multi, 1, "SR", ...) (src/core/cpp/Contractor.cpp:3067-3138) and the
  * eigensolver wrappers behind it (src/wrapper/eigs.cpp:71-422).  x_dev: device vector of

@@ -150,8 +150,9 @@ class Site {
   mwArray getA() const { return dev_.download(); }  // host copy
   void setA(const mwArray& a);
   mpsd_tensor* device() const { return dev_.t; }
-  void gaugeR(DevTensor& rightterm);   // Site::gaugeR, cutD = 0 (Site.cpp:179-208)
-  void gaugeL(DevTensor& leftterm);    // Site::gaugeL (Site.cpp:210-238)
+  // Site::gaugeR / gaugeL (Site.cpp:178-238): QR / LQ, or a truncated SVD when cutD > 0
+  void gaugeR(DevTensor& rightterm, int cutD = 0);
+  void gaugeL(DevTensor& leftterm, int cutD = 0);
   void reduceDimR(const DevTensor& leftterm);   // Site.cpp:240-261
   void reduceDimL(const DevTensor& rightterm);  // Site.cpp:263-283
 
@@ -170,6 +171,8 @@ class MPS {
   MPS(int length, int bond, int d, mpsd_ctx* ctx = nullptr);
   // MPS(len, bond, vector<int> d) (MPS.cpp:68-87): site k has physical dimension d[k].
   MPS(int length, int bond, const std::vector<int>& d, mpsd_ctx* ctx = nullptr);
+  // MPS(const MPS&, D) (MPS.cpp:761-773): a copy truncated to bond D by gaugeCond('L', false, D).
+  MPS(const MPS& other, int D);
   int getLength() const { return (int)A_.size(); }
   int getBond() const { return bond_; }
   const Site& getA(int pos) const { return *A_[pos]; }
@@ -180,7 +183,9 @@ class MPS {
   // at the canonical bond profile min(d^k, D, d^(L-k)), then right-gauged.  The reference
   // fills uniform [0,1) values from libc random() (MPS.cpp:282-294).
   void setRandomState(uint64_t seed = 1);
-  void gaugeCond(char dir, bool normal = true);        // MPS.cpp:306-364
+  // MPS.cpp:306-364: gauge every site; with cutD > 0 each bond is cut by a truncated SVD to at
+  // most cutD; without normalisation the last factor's magnitude goes into the norm factor
+  void gaugeCond(char dir, bool normal = false, int cutD = 0);
   void gaugeCond(int pos, char dir, bool next = false);  // MPS.cpp:366-419
   Gauge getGauge() const { return gauge_; }
   double getNormFact() const { return normfact_; }

@@ -354,6 +354,22 @@ int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo
   }
 }
 
+int mpsd_mps_truncate(mpsd_ctx* ctx, const char* mps_in, int D, const char* mps_out) {
+  if (!ctx || !mps_in || !mps_out || D < 1) return MPSD_ERR_ARG;
+  try {
+    MPS in(1, 1, 2, ctx);
+    in.importMPS(mps_in);
+    MPS out(in, D);
+    out.exportMPS(mps_out);
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
 int mpsd_gs_expectation(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mps_path, double* re, double* im) {
   if (!ctx || !cfg || !mps_path || !re || !im) return MPSD_ERR_ARG;
   try {

@@ -295,9 +295,19 @@ void Site::setA(const mwArray& a) {
   dev_.upload(b);
 }
 
-void Site::gaugeR(DevTensor& rightterm) { check(ctx_, mpsd_site_gauge_r(ctx_, dev_.t, rightterm.t), "gaugeR"); }
+void Site::gaugeR(DevTensor& rightterm, int cutD) {
+  if (cutD > 0)
+    check(ctx_, mpsd_site_gauge_r_svd(ctx_, dev_.t, rightterm.t, cutD), "gaugeR");
+  else
+    check(ctx_, mpsd_site_gauge_r(ctx_, dev_.t, rightterm.t), "gaugeR");
+}
 
-void Site::gaugeL(DevTensor& leftterm) { check(ctx_, mpsd_site_gauge_l(ctx_, dev_.t, leftterm.t), "gaugeL"); }
+void Site::gaugeL(DevTensor& leftterm, int cutD) {
+  if (cutD > 0)
+    check(ctx_, mpsd_site_gauge_l_svd(ctx_, dev_.t, leftterm.t, cutD), "gaugeL");
+  else
+    check(ctx_, mpsd_site_gauge_l(ctx_, dev_.t, leftterm.t), "gaugeL");
+}
 
 void Site::reduceDimR(const DevTensor& leftterm) {
   check(ctx_, mpsd_site_reduce_dim_r(ctx_, dev_.t, leftterm.t), "reduceDimR");
@@ -340,18 +350,18 @@ DevTensor& MPS::factor() {
   return *factor_;
 }
 
-void MPS::gaugeCond(char dir, bool normal) {
+void MPS::gaugeCond(char dir, bool normal, int cutD) {
   DevTensor& tmp = factor();
   const int L = getLength();
   if (dir == 'L' || dir == 'l') {
     for (int pos = L - 1; pos >= 0; --pos) {
-      A_[pos]->gaugeL(tmp);
+      A_[pos]->gaugeL(tmp, cutD);
       if (pos > 0) A_[pos - 1]->reduceDimL(tmp);
     }
     gauge_ = gLeft;
   } else if (dir == 'R' || dir == 'r') {
     for (int pos = 0; pos < L; ++pos) {
-      A_[pos]->gaugeR(tmp);
+      A_[pos]->gaugeR(tmp, cutD);
       if (pos < L - 1) A_[pos + 1]->reduceDimR(tmp);
     }
     gauge_ = gRight;
@@ -361,7 +371,26 @@ void MPS::gaugeCond(char dir, bool normal) {
   int maxb = 1;
   for (auto& s : A_) maxb = std::max(maxb, std::max(s->getDl(), s->getDr()));
   bond_ = std::min(bond_, maxb);
-  if (normal) normfact_ = 1.0;
+  if (normal) {
+    normfact_ = 1.0;
+  } else if (L > 0) {
+    // the last factor is the 1x1 norm of the state (its phase stayed in the edge site)
+    mwArray f = tmp.download();
+    if (f.getNrComponents() != 1) throw Error(MPSD_ERR_SHAPE, "MPS::gaugeCond: last factor is not a scalar");
+    normfact_ *= std::abs(f.getElement(0));
+  }
+}
+
+MPS::MPS(const MPS& other, int D) : ctx_(other.ctx_), bond_(other.bond_), normfact_(other.normfact_), gauge_(other.gauge_) {
+  for (int k = 0; k < other.getLength(); ++k) {
+    const Site& s = other.getA(k);
+    A_.emplace_back(new Site(ctx_, s.getPos(), s.dims()));
+    check(ctx_,
+          mpsd_vec_copy(ctx_, (const double*)mpsd_tensor_devptr(s.device()),
+                        (double*)mpsd_tensor_devptr(A_.back()->device()), mpsd_tensor_size(s.device())),
+          "MPS copy");
+  }
+  gaugeCond('L', false, D);
 }
 
 void MPS::gaugeCond(int pos, char dir, bool next) {

@@ -158,6 +158,27 @@ __global__ void __launch_bounds__(kT) unpermute_q_kernel(const double2* __restri
   site[t] = Q[i + (int64_t)ldq * (s + (int64_t)d * c)];
 }
 
+// R[i + X j] = S[i] V[i + ldv j] for i < X, j < n (the S V^H factor of a truncated SVD).
+__global__ void __launch_bounds__(kT) scale_rows_kernel(const double2* __restrict__ V, int ldv, const double* __restrict__ S,
+                                                        int X, int n, double2* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)X * n) return;
+  const int i = (int)(t % X);
+  const int64_t j = t / X;
+  const double2 v = V[i + (int64_t)ldv * j];
+  R[t] = double2{S[i] * v.x, S[i] * v.y};
+}
+
+// L[a + m i] = U[a + m i] S[i] for i < X (the U S factor of a truncated SVD).
+__global__ void __launch_bounds__(kT) scale_cols_kernel(const double2* __restrict__ U, int m, const double* __restrict__ S,
+                                                        int X, double2* __restrict__ L) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= (int64_t)m * X) return;
+  const int i = (int)(t / m);
+  const double2 u = U[t];
+  L[t] = double2{S[i] * u.x, S[i] * u.y};
+}
+
 // B[s + d c, a] = conj(site[s, a, c]): the adjoint of the LQ matrix M[a, s + d c] (B: n x Dl).
 __global__ void __launch_bounds__(kT) site_to_bh_kernel(const double2* __restrict__ site, int d, int Dl, int Dr,
                                                         double2* __restrict__ B) {
@@ -522,6 +543,105 @@ int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out) {
   ++site->version;
   ++L_out->version;
   return hip_err(ctx, hipGetLastError(), "gauge_l");
+}
+
+}  // extern "C"
+
+namespace mpsd {
+// Thin SVD of the m x n matrix at A (destroyed): U (m x k) and V^H (k x n, leading dimension k)
+// into U, V, singular values into S[0..k); returns the number kept under the rule of
+// wrapper::svd with tol = 0 and Xi = cutD (src/wrapper/svd.cpp:23-118): the leading values
+// above zero among the first min(cutD, k).
+static int svd_keep(mpsd_ctx* ctx, double2* A, int m, int n, double2* U, double2* V, double* S, int cutD, int* kept) {
+  rocblas_handle h;
+  int rc = handle_of(ctx, &h);
+  if (rc) return rc;
+  const int k = m < n ? m : n;
+  double* E = S + k;
+  rocblas_int* info = (rocblas_int*)(S + 2 * k + 1);
+  rc = solver_err(ctx,
+                  rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, m, n, (rocblas_double_complex*)A,
+                                   m, S, (rocblas_double_complex*)U, m, (rocblas_double_complex*)V, k, E,
+                                   rocblas_outofplace, info),
+                  "zgesvd");
+  if (rc) return rc;
+  std::vector<double> hs(2 * k + 2);
+  hipError_t e = hipMemcpyAsync(hs.data(), S, sizeof(double) * (2 * k + 2), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "svd values");
+  rocblas_int hinfo = 0;
+  memcpy(&hinfo, &hs[2 * k + 1], sizeof(hinfo));
+  if (hinfo != 0) return set_err(ctx, MPSD_ERR_NUMERIC, "zgesvd did not converge");
+  const int lim = cutD < k ? cutD : k;
+  int X = 0;
+  while (X < lim && hs[X] > 0.0) ++X;
+  if (X == 0) return set_err(ctx, MPSD_ERR_NUMERIC, "svd: no singular value above zero");
+  *kept = X;
+  return MPSD_OK;
+}
+}  // namespace mpsd
+
+extern "C" {
+
+int mpsd_site_gauge_r_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out, int cutD) {
+  if (!ctx || !site || !R_out || site == R_out || cutD < 1) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = d * Dl, n = Dr, k = m < n ? m : n;
+  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * k);
+  if (!rc) rc = ensure_scratch(ctx, 7, sizeof(double2) * (size_t)k * n);
+  if (!rc) rc = ensure_scratch(ctx, 2, sizeof(double) * (size_t)(2 * k + 2));
+  if (rc) return rc;
+  double2* U = (double2*)ctx->scratch[3];
+  double2* V = (double2*)ctx->scratch[7];
+  double* S = (double*)ctx->scratch[2];
+  int X = 0;
+  rc = svd_keep(ctx, site->ptr, m, n, U, V, S, cutD, &X);
+  const int rd[2] = {X, n};
+  if (!rc) rc = mpsd_tensor_reshape(R_out, 2, rd);
+  const int sd[3] = {d, Dl, X};
+  if (!rc) rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(site->ptr, U, sizeof(double2) * (size_t)m * X, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_err(ctx, e, "gauge_r_svd copy");
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(nblk((int64_t)X * n)), dim3(kT), 0, ctx->stream, V, k, S, X, n,
+                     R_out->ptr);
+  if (X == 1 && n == 1)
+    hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, R_out->ptr);
+  ++site->version;
+  ++R_out->version;
+  return hip_err(ctx, hipGetLastError(), "gauge_r_svd");
+}
+
+int mpsd_site_gauge_l_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, int cutD) {
+  if (!ctx || !site || !L_out || site == L_out || cutD < 1) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = Dl, n = d * Dr, k = m < n ? m : n;
+  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * n);
+  if (!rc) rc = ensure_scratch(ctx, 6, sizeof(double2) * (size_t)m * k);
+  if (!rc) rc = ensure_scratch(ctx, 7, sizeof(double2) * (size_t)k * n);
+  if (!rc) rc = ensure_scratch(ctx, 2, sizeof(double) * (size_t)(2 * k + 2));
+  if (rc) return rc;
+  double2* T = (double2*)ctx->scratch[3];
+  double2* U = (double2*)ctx->scratch[6];
+  double2* V = (double2*)ctx->scratch[7];
+  double* S = (double*)ctx->scratch[2];
+  // M[a, s + d c] = site[s, a, c]; M = U S V^H, leftterm = U S, site = V^H reshaped
+  hipLaunchKernelGGL(permute_sad_kernel, dim3(nblk((int64_t)m * n)), dim3(kT), 0, ctx->stream, site->ptr, d, Dl, Dr, T);
+  int X = 0;
+  rc = svd_keep(ctx, T, m, n, U, V, S, cutD, &X);
+  const int ld[2] = {m, X};
+  if (!rc) rc = mpsd_tensor_reshape(L_out, 2, ld);
+  const int sd[3] = {d, X, Dr};
+  if (!rc) rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(nblk((int64_t)m * X)), dim3(kT), 0, ctx->stream, U, m, S, X, L_out->ptr);
+  hipLaunchKernelGGL(unpermute_q_kernel, dim3(nblk((int64_t)d * X * Dr)), dim3(kT), 0, ctx->stream, V, k, X, d, Dr,
+                     site->ptr);
+  if (X == 1 && m == 1)
+    hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, L_out->ptr);
+  ++site->version;
+  ++L_out->version;
+  return hip_err(ctx, hipGetLastError(), "gauge_l_svd");
 }
 
 int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R) {

@@ -104,6 +104,9 @@ def _optional(lib):
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_mpo_save": (I, [P, C.POINTER(GsConfig), C.c_char_p, I]),
         "mpsd_vec_proj_update": (I, [P, P, I64, I, C.POINTER(D), P, P, I64]),
+        "mpsd_site_gauge_r_svd": (I, [P, P, P, I]),
+        "mpsd_site_gauge_l_svd": (I, [P, P, P, I]),
+        "mpsd_mps_truncate": (I, [P, C.c_char_p, I, C.c_char_p]),
         "mpsd_gs_solve_levels": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, I, D, C.POINTER(D), C.POINTER(D),
                                      C.POINTER(GsResult)]),
         "mpsd_gs_sweep_part": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, I, C.POINTER(C.c_char_p),
@@ -303,6 +306,11 @@ def gs_sweep_part(ctx, cfg, mps_in, pos0, lim, mps_out=None, proj=(), weights=()
                                          fmt, str(mps_in).encode(), n, paths, w, pos0, lim,
                                          None if mps_out is None else str(mps_out).encode(), C.byref(e)))
     return e.value
+
+
+def mps_truncate(ctx, mps_in, D, mps_out):
+    """MPS(const MPS&, D) on an exported state, written to mps_out."""
+    ctx.check(ctx.lib.mpsd_mps_truncate(ctx.handle, str(mps_in).encode(), D, str(mps_out).encode()))
 
 
 def exported_symbols():

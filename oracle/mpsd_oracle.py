@@ -37,6 +37,9 @@ Pinning (how each function is checked; tests/test_oracle.py):
   sweep_part, penalty_sweep_part, find_*_excited_state -- ED excited energies; alternating
                        partial sweeps reach the ED ground energy; penalised energy decreases.
   mps_write/mps_read -- a hand-assembled exportMPS file (known answer).
+  svd_keep, mps_truncate -- the truncated state equals the dense Schmidt truncation across the
+                       one bond that exceeds D (left-canonical start), and is exact when no
+                       bond exceeds D.
   mpo_write/mpo_read -- a hand-written text file and a hand-assembled binary file laid out
                        as the reference's save / savetext code writes them (known answers),
                        and against the device library's own writer.
@@ -61,6 +64,7 @@ __all__ = [
     "chain_mpo", "mps_write", "mps_read", "overlap_left", "overlap_right", "effective_bra",
     "proj_multi_product", "proj_multi_full", "sweep_part", "penalty_sweep_part",
     "find_ground_state_with_projector_penalty", "find_next_excited_state", "overlap",
+    "svd_keep", "mps_truncate",
 ]
 
 
@@ -886,4 +890,42 @@ def find_next_excited_state(mpo, mps, levels, convtol=1e-12, offset=0.0):
     if len(levels) == 1 or pen < 1e-6:
         pen = len(levels[0]) * 0.5
     return find_ground_state_with_projector_penalty(mpo, mps, [pen] * len(levels), levels, convtol, offset)
+
+
+# --------------------------------------------------------------------------- SVD truncation
+
+def svd_keep(M, cutD):
+    """wrapper::svd with tol = 0 and Xi = cutD (src/wrapper/svd.cpp:23-118): thin SVD, keeping
+    the leading singular values above zero among the first min(cutD, k).  Returns (U, S, Vh)."""
+    U, S, Vh = np.linalg.svd(M, full_matrices=False)
+    X = 0
+    while X < min(cutD, S.size) and S[X] > 0:
+        X += 1
+    if X == 0:
+        raise ValueError("no singular value above zero")
+    return U[:, :X], S[:X], Vh[:X, :]
+
+
+def mps_truncate(mps, D, normfact=1.0):
+    """MPS(const MPS&, D) (MPS.cpp:761-773) = gaugeCond('L', false, D) (MPS.cpp:306-364) with
+    Site::gaugeL's SVD branch (Site.cpp:210-238): from the right, M[a, s + d c] = U S V^H, the
+    site becomes V^H, U S goes into the left neighbour; the last 1x1 factor's phase stays in
+    site 0 and its magnitude multiplies the norm factor.  Returns (sites, normfact)."""
+    mps = [a.copy() for a in mps]
+    for pos in range(len(mps) - 1, -1, -1):
+        d, Dl, Dr = mps[pos].shape
+        M = reshape_f(permute(mps[pos], (2, 1, 3)), (Dl, d * Dr))
+        U, S, Vh = svd_keep(M, D)
+        X = S.size
+        lt = U * S[None, :]
+        site = permute(reshape_f(Vh, (X, d, Dr)), (2, 1, 3))
+        if X == 1 and Dl == 1:
+            site = site * (lt[0, 0] / abs(lt[0, 0]))
+            lt = np.abs(lt)
+        mps[pos] = site
+        if pos > 0:
+            mps[pos - 1] = _absorb_left(mps[pos - 1], lt)
+        else:
+            normfact = normfact * abs(lt[0, 0])
+    return mps, normfact
 
