@@ -68,6 +68,10 @@ int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx);
 #define MPSD_KERNEL_KINDS 4
 int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on);
 int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches);
+/* Restrict timing to the kinds whose bit (1 << MPSD_KERNEL_*) is set in mask (default: all).
+ * Each timed launch adds two event records to the stream, so timing only the kind being
+ * measured keeps the others' schedule as it is untimed. */
+int mpsd_ctx_set_timing_kinds(mpsd_ctx* ctx, unsigned mask);
 /* Force the BRGEMM tile (tm, tn in {32, 64}); 0, 0 restores the built-in choice. */
 int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
 /* Other launch options of the context (used by the tile tuner). */

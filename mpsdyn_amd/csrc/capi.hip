@@ -44,14 +44,17 @@ static hipEvent_t pool_event(mpsd_ctx* ctx) {
     ctx->event_pool.pop_back();
     return e;
   }
+  // timing-only events: no system-scope fence, so recording one neither writes back nor
+  // invalidates the caches between the kernels it brackets (elapsed times are read after a
+  // stream synchronisation)
   hipEvent_t e = nullptr;
-  (void)hipEventCreate(&e);
+  (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
   return e;
 }
 
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch) {
   ctx->cur_kind = kind;
-  if (!ctx->timing) return launch();
+  if (!ctx->timing || !((ctx->timing_mask >> kind) & 1u)) return launch();
   hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
   (void)hipEventRecord(a, ctx->stream);
   hipError_t e = launch();
@@ -269,6 +272,12 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
 int mpsd_ctx_sync(mpsd_ctx* ctx) {
   if (!ctx) return MPSD_ERR_ARG;
   return hip_err(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int mpsd_ctx_set_timing_kinds(mpsd_ctx* ctx, unsigned mask) {
+  if (!ctx) return MPSD_ERR_ARG;
+  ctx->timing_mask = mask;
+  return MPSD_OK;
 }
 
 int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on) {

@@ -156,6 +156,7 @@ struct mpsd_ctx {
   int fused_lanczos = 1;  // small local problems: one-workgroup eigensolver (1) or the general one (0)
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
+  unsigned timing_mask = ~0u;  // kernel kinds timed when timing is on
   struct Pending {
     int kind;
     hipEvent_t a, b;

@@ -124,6 +124,7 @@ def _optional(lib):
         "mpsd_server_run_trace": (I, [P, I, C.POINTER(GsConfig), C.POINTER(D), C.POINTER(ServeResult)]),
         "mpsd_server_stats": (I, [P, C.POINTER(ServeStats)]),
         "mpsd_ctx_set_timing": (I, [P, I]),
+        "mpsd_ctx_set_timing_kinds": (I, [P, C.c_uint]),
         "mpsd_ctx_timing": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
         "mpsd_ctx_set_tile": (I, [P, I, I]),
         "mpsd_ctx_set_option": (I, [P, I, I]),
@@ -394,6 +395,10 @@ class Context:
     # timing (HIP events around each launch, accumulated per kernel kind)
     def set_timing(self, on=True):
         self.check(self.lib.mpsd_ctx_set_timing(self._h, 1 if on else 0))
+
+    def set_timing_kinds(self, mask):
+        """Time only the kernel kinds whose bit (1 << kind) is set (default all)."""
+        self.check(self.lib.mpsd_ctx_set_timing_kinds(self._h, mask))
 
     def timing(self, kind):
         ms = C.c_double()
