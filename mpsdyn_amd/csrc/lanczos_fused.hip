@@ -4,13 +4,21 @@
 // kernel of a Lanczos step runs for a few microseconds and the launch latency dominates.  Here one
 // workgroup of 1024 threads does everything a site's eigensolve needs, in one launch:
 //
-// * the matvec y = (L W R) x (+ offset x) in three stages through global scratch T1, T2
-//   (the contraction order of TensorMultiplier::product, src/wrapper/multipliers/
-//   TensorMultiplier.cpp:61-93);
-// * two-pass classical Gram-Schmidt against the Krylov basis (one dot per wave, fixed order);
-// * alpha_j, beta_j as the stream and graph solvers form them (beta from the second pass);
+// * the matvec y = (L W R) x (+ offset x) in three stages through T1, T2 (the contraction
+//   order of TensorMultiplier::product, src/wrapper/multipliers/TensorMultiplier.cpp:61-93),
+//   the two GEMM stages on FP64 MFMA one 16 x 16 block per wave, the MPO mix on the vector
+//   units.
+//   L, R, W, the current basis vector and the new vector live in LDS (the operands loaded
+//   once per eigensolve), and T1, T2 too when they fit: the inner products of the three
+//   stages read LDS instead of waiting on L2 in dependent chains;
+// * two-pass classical Gram-Schmidt of the new vector (in LDS) against the Krylov basis (in
+//   global memory; one dot per wave, fixed order);
+// * alpha_j, beta_j as the stream and graph solvers form them (beta from the second pass),
+//   reduced across one wave;
 // * the tridiagonal eigenproblem every third step: the lowest eigenvalue by multisection
-//   (256 Sturm counts per round), its eigenvector by inverse iteration on one thread;
+//   (256 Sturm counts per round), its eigenvector by two steps of inverse iteration on one
+//   thread, the elimination carried in registers; both divide through a refined hardware
+//   reciprocal;
 // * restarts from the Ritz vector, and the final normalised Ritz vector in x.
 //
 // The host reads one small status block per eigensolve.
@@ -24,8 +32,23 @@ namespace {
 
 constexpr int kFT = 1024;
 constexpr int kFW = kFT / 64;
-constexpr int kFM = 64;  // largest Krylov basis (m + 1)
-constexpr int kMS = 256; // points per multisection round
+constexpr int kFM = 64;       // largest Krylov basis (m + 1)
+constexpr int kMS = 256;      // points per multisection round
+constexpr int kPool = 9472;   // complex elements of LDS for operands and vectors (148 KiB)
+
+// LDS footprint of a problem, in complex elements: the operands with the current and the new
+// basis vector, and the two intermediates.
+struct FusedLds {
+  int64_t ops, inter;
+};
+inline FusedLds fused_lds(const mpsd_heff* h) {
+  const int64_t n = (int64_t)h->d * h->Dk * h->Dkr;
+  FusedLds f;
+  f.ops = (int64_t)h->Db * h->wl * h->Dk + (int64_t)h->Dbr * h->wr * h->Dkr +
+          (int64_t)h->d * h->wl * h->d * h->wr + 2 * n;
+  f.inter = (int64_t)h->Db * h->wl * h->d * h->Dkr + (int64_t)h->Db * h->d * h->wr * h->Dkr;
+  return f;
+}
 
 struct FusedArgs {
   const double2* L;
@@ -53,87 +76,155 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
 
-// y = H x on the whole workgroup (x and y distinct, both n long).
-__device__ void fused_matvec(const FusedArgs& a, const double2* __restrict__ x, double2* __restrict__ y) {
+typedef double fd4 __attribute__((ext_vector_type(4)));
+
+// 1 / v: the hardware reciprocal estimate refined by two Newton-Raphson steps
+// (r <- r + r (1 - v r)), accurate to about an ulp; much shorter than an IEEE division in the
+// serial recurrences of the tridiagonal solver.
+__device__ __forceinline__ double recip(double v) {
+  double r = __builtin_amdgcn_rcp(v);
+  r = fma(r, fma(-v, r, 1.0), r);
+  r = fma(r, fma(-v, r, 1.0), r);
+  return r;
+}
+
+// One 16 x 16 block C(m0.., n0..) = sum_k A(m, k) B(k, n) of a small complex GEMM on one wave,
+// with v_mfma_f64_16x16x4f64 in the three-product form (P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2).  A(m, k) and B(k, n) are read
+// through the accessors (out-of-range indices read as zero); the result element (m0 + q + 4 r,
+// n0 + (lane & 15)), q = lane >> 4, is handed to store(m, n, value) for r = 0..3.
+template <class FA, class FB, class FS>
+__device__ __forceinline__ void wave_zgemm16(int M, int N, int K, int m0, int n0, FA A, FB B, FS store) {
+  const int lane = threadIdx.x & 63;
+  const int lc = lane & 15, q = lane >> 4;
+  fd4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+  const int m = m0 + lc, n = n0 + lc;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + q;
+    const double2 av = (m < M && k < K) ? A(m, k) : double2{0.0, 0.0};
+    const double2 bv = (n < N && k < K) ? B(k, n) : double2{0.0, 0.0};
+    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, p1, 0, 0, 0);
+    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.y, p2, 0, 0, 0);
+    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x + av.y, bv.x + bv.y, p3, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int mo = m0 + q + 4 * r;
+    if (mo < M && n < N) store(mo, n, double2{p1[r] - p2[r], p3[r] - (p1[r] + p2[r])});
+  }
+}
+
+// Where the matvec's operands live: L, R, W, the input X and the output Y in LDS; T1, T2 in
+// LDS or in global scratch.
+struct FusedOps {
+  const double2* L;
+  const double2* R;
+  const double2* W;
+  double2* X;
+  double2* Y;
+  double2* T1;
+  double2* T2;
+};
+
+// Y = H X on the whole workgroup.
+__device__ void fused_matvec(const FusedArgs& a, const FusedOps& o) {
   const int tid = threadIdx.x;
   const int d = a.d, Db = a.Db, wl = a.wl, Dk = a.Dk, Dbr = a.Dbr, wr = a.wr, Dkr = a.Dkr;
-  // stage 1: T1[b, al, s, ar] = sum_a L[b, al, a] x[s, a, ar]
-  const int64_t n1 = (int64_t)Db * wl * d * Dkr;
-  for (int64_t t = tid; t < n1; t += kFT) {
-    int64_t q = t;
-    const int b = (int)(q % Db);
-    q /= Db;
-    const int al = (int)(q % wl);
-    q /= wl;
-    const int s = (int)(q % d);
-    const int ar = (int)(q / d);
-    const double2* Lp = a.L + b + (int64_t)Db * al;
-    const double2* xp = x + s + (int64_t)d * Dk * ar;
-    double2 acc{0.0, 0.0};
-    for (int k = 0; k < Dk; ++k) acc = cfma(Lp[(int64_t)Db * wl * k], xp[(int64_t)d * k], acc);
-    a.T1[t] = acc;
+  const double2* x = o.X;
+  const int wave = tid >> 6;
+  // stage 1: T1[(b, al), (s, ar)] = sum_a L[(b, al), a] x[s, a, ar], one 16 x 16 block per wave
+  {
+    const int M = Db * wl, N = d * Dkr;
+    const int gm = (M + 15) / 16, gn = (N + 15) / 16;
+    const double2* Lm = o.L;
+    double2* T1 = o.T1;
+    for (int blk = wave; blk < gm * gn; blk += kFW)
+      wave_zgemm16(
+          M, N, Dk, 16 * (blk % gm), 16 * (blk / gm), [&](int mm, int k) { return Lm[mm + M * k]; },
+          [&](int k, int nn) { return x[(nn % d) + d * k + d * Dk * (nn / d)]; },
+          [&](int mm, int nn, double2 v) { T1[mm + M * nn] = v; });
   }
   __syncthreads();
   // stage 2: T2[b, s', be, ar] = sum_{al, s} W[s', al, s, be] T1[b, al, s, ar]
-  const int64_t n2 = (int64_t)Db * d * wr * Dkr;
-  for (int64_t t = tid; t < n2; t += kFT) {
-    int64_t q = t;
-    const int b = (int)(q % Db);
+  const int n2 = Db * d * wr * Dkr;
+  for (int t = tid; t < n2; t += kFT) {
+    int q = t;
+    const int b = q % Db;
     q /= Db;
-    const int sp = (int)(q % d);
+    const int sp = q % d;
     q /= d;
-    const int be = (int)(q % wr);
-    const int ar = (int)(q / wr);
+    const int be = q % wr;
+    const int ar = q / wr;
     double2 acc{0.0, 0.0};
     for (int s = 0; s < d; ++s)
       for (int al = 0; al < wl; ++al)
-        acc = cfma(a.W[sp + (int64_t)d * (al + (int64_t)wl * (s + (int64_t)d * be))],
-                   a.T1[b + (int64_t)Db * (al + (int64_t)wl * (s + (int64_t)d * ar))], acc);
-    a.T2[t] = acc;
+        acc = cfma(o.W[sp + d * (al + wl * (s + d * be))], o.T1[b + Db * (al + wl * (s + d * ar))], acc);
+    o.T2[t] = acc;
   }
   __syncthreads();
-  // stage 3: y[s', b, c] = sum_{be, ar} T2[b, s', be, ar] R[c, be, ar] (+ offset x)
-  const int64_t n3 = (int64_t)d * Db * Dbr;
-  for (int64_t t = tid; t < n3; t += kFT) {
-    int64_t q = t;
-    const int sp = (int)(q % d);
-    q /= d;
-    const int b = (int)(q % Db);
-    const int c = (int)(q / Db);
-    double2 acc{0.0, 0.0};
-    for (int ar = 0; ar < Dkr; ++ar)
-      for (int be = 0; be < wr; ++be)
-        acc = cfma(a.T2[b + (int64_t)Db * (sp + (int64_t)d * (be + (int64_t)wr * ar))],
-                   a.R[c + (int64_t)Dbr * (be + (int64_t)wr * ar)], acc);
-    if (a.offset != 0.0) {
-      const double2 xi = x[t];
-      acc.x = fma(a.offset, xi.x, acc.x);
-      acc.y = fma(a.offset, xi.y, acc.y);
-    }
-    y[t] = acc;
+  // stage 3: y[s', b, c] = sum_{(be, ar)} T2[(b, s'), (be, ar)] R[c, (be, ar)] (+ offset x)
+  {
+    const int M = Db * d, N = Dbr, K = wr * Dkr;
+    const int gm = (M + 15) / 16, gn = (N + 15) / 16;
+    const double2* T2 = o.T2;
+    const double2* Rm = o.R;
+    double2* Y = o.Y;
+    const double off = a.offset;
+    for (int blk = wave; blk < gm * gn; blk += kFW)
+      wave_zgemm16(
+          M, N, K, 16 * (blk % gm), 16 * (blk / gm), [&](int mm, int k) { return T2[mm + M * k]; },
+          [&](int k, int nn) { return Rm[nn + Dbr * k]; },
+          [&](int mm, int nn, double2 v) {
+            const int t = (mm / Db) + d * (mm % Db) + d * Db * nn;
+            if (off != 0.0) {
+              const double2 xi = x[t];
+              v.x = fma(off, xi.x, v.x);
+              v.y = fma(off, xi.y, v.y);
+            }
+            Y[t] = v;
+          });
   }
   __syncthreads();
 }
 
-// h[i] = V_i^H w for i < cnt (one dot per wave, fixed order); h is shared, valid after the call.
-__device__ void multi_dot(const double2* __restrict__ V, int64_t n, int cnt, const double2* __restrict__ w,
+// h[i] = V_i^H w for i < cnt, and h[cnt] = w^H w when self is set (one dot per wave, fixed
+// order); V in global memory, w in LDS; h is shared, valid after the call.
+__device__ void multi_dot(const double2* __restrict__ V, int64_t n, int cnt, bool self, const double2* w,
                           double2* h) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = wave; i < cnt; i += kFW) {
-    const double2* v = V + (int64_t)i * n;
+  const int tot = cnt + (self ? 1 : 0);
+  for (int i = wave; i < tot; i += kFW) {
     double2 acc{0.0, 0.0};
-    for (int64_t t = lane; t < n; t += 64) acc = cjfma(v[t], w[t], acc);
+    if (i < cnt) {
+      const double2* v = V + (int64_t)i * n;
+#pragma unroll 4
+      for (int64_t t = lane; t < n; t += 64) acc = cjfma(v[t], w[t], acc);
+    } else {
+#pragma unroll 4
+      for (int64_t t = lane; t < n; t += 64) acc = cjfma(w[t], w[t], acc);
+    }
     const double sx = wave_sum(acc.x), sy = wave_sum(acc.y);
     if (lane == 0) h[i] = double2{sx, sy};
   }
   __syncthreads();
 }
 
-// w -= sum_{i < cnt} V_i h[i]
-__device__ void gemv_sub(const double2* __restrict__ V, int64_t n, int cnt, const double2* h, double2* __restrict__ w) {
+// w -= sum_{i < cnt} V_i h[i]  (V global, w in LDS)
+__device__ void gemv_sub(const double2* __restrict__ V, int64_t n, int cnt, const double2* h, double2* w) {
   for (int64_t t = threadIdx.x; t < n; t += kFT) {
     double2 acc{0.0, 0.0};
+#pragma unroll 4
     for (int i = 0; i < cnt; ++i) acc = cfma(V[(int64_t)i * n + t], h[i], acc);
     const double2 wi = w[t];
     w[t] = double2{wi.x - acc.x, wi.y - acc.y};
@@ -141,118 +232,177 @@ __device__ void gemv_sub(const double2* __restrict__ V, int64_t n, int cnt, cons
   __syncthreads();
 }
 
-// Number of eigenvalues of the tridiagonal (a, b) below x (Sturm sequence).
-__device__ int sturm(const double* a, const double* b, int m, double x) {
+// Number of eigenvalues of the tridiagonal (diagonal a, squared off-diagonal b2) below x
+// (Sturm sequence of the LDL^T pivots).
+__device__ int sturm(const double* a, const double* b2, int m, double x) {
   const double tiny = 1e-300;
   int cnt = 0;
   double q = a[0] - x;
   if (q < 0) ++cnt;
   for (int i = 1; i < m; ++i) {
     if (fabs(q) < tiny) q = q < 0 ? -tiny : tiny;
-    q = a[i] - x - b[i - 1] * b[i - 1] / q;
+    q = a[i] - x - b2[i - 1] * recip(q);
     if (q < 0) ++cnt;
   }
   return cnt;
 }
 
-// (T - sigma) y = r in place on one thread (partial pivoting); dd, u, v, l: shared work
-// arrays of length >= m (m <= kFM).
+// (T - sigma) y = r in place on one thread: Gaussian elimination with partial pivoting of the
+// tridiagonal (diagonal a, off-diagonal b), m <= kFM.  The row being eliminated is carried in
+// registers; the finished pivot rows go to dd, u, v (shared work arrays) for the back
+// substitution, so no step waits on a value it has just stored.
 __device__ void tri_solve(const double* a, const double* b, int m, double sigma, double* r, double* dd, double* u,
-                          double* v, double* l) {
+                          double* v) {
+  const double tiny = 1e-300;
   if (m == 1) {
     double p = a[0] - sigma;
-    if (fabs(p) < 1e-300) p = 1e-300;
+    if (fabs(p) < tiny) p = tiny;
     r[0] /= p;
     return;
   }
-  for (int i = 0; i < m; ++i) {
-    dd[i] = a[i] - sigma;
-    u[i] = v[i] = l[i] = 0.0;
-  }
+  // current row i: columns i, i+1, i+2 and right-hand side
+  double c0 = a[0] - sigma, c1 = b[0], c2 = 0.0, cr = r[0];
   for (int i = 0; i + 1 < m; ++i) {
-    u[i] = b[i];
-    l[i + 1] = b[i];
-  }
-  for (int i = 0; i + 1 < m; ++i) {
-    if (fabs(l[i + 1]) > fabs(dd[i])) {
-      double t;
-      t = dd[i]; dd[i] = l[i + 1]; l[i + 1] = t;
-      t = u[i]; u[i] = dd[i + 1]; dd[i + 1] = t;
-      t = v[i]; v[i] = u[i + 1]; u[i + 1] = t;
-      t = r[i]; r[i] = r[i + 1]; r[i + 1] = t;
+    // row i+1 before elimination: columns i, i+1, i+2
+    const double n0 = b[i], n1 = a[i + 1] - sigma, n2 = (i + 2 < m) ? b[i + 1] : 0.0, nr = r[i + 1];
+    double p0, p1, p2, pr, o0, o1, o2, orr;
+    if (fabs(n0) > fabs(c0)) {
+      p0 = n0, p1 = n1, p2 = n2, pr = nr;
+      o0 = c0, o1 = c1, o2 = c2, orr = cr;
+    } else {
+      p0 = c0, p1 = c1, p2 = c2, pr = cr;
+      o0 = n0, o1 = n1, o2 = n2, orr = nr;
     }
-    double piv = dd[i];
-    if (fabs(piv) < 1e-300) piv = dd[i] = 1e-300;
-    const double f = l[i + 1] / piv;
-    dd[i + 1] -= f * u[i];
-    u[i + 1] -= f * v[i];
-    r[i + 1] -= f * r[i];
+    if (fabs(p0) < tiny) p0 = tiny;
+    const double ip = recip(p0);
+    const double f = o0 * ip;
+    dd[i] = ip;  // the back substitution multiplies by the reciprocal pivot
+    u[i] = p1;
+    v[i] = p2;
+    r[i] = pr;
+    c0 = o1 - f * p1;
+    c1 = o2 - f * p2;
+    c2 = 0.0;
+    cr = orr - f * pr;
   }
-  if (fabs(dd[m - 1]) < 1e-300) dd[m - 1] = 1e-300;
-  r[m - 1] /= dd[m - 1];
-  r[m - 2] = (r[m - 2] - u[m - 2] * r[m - 1]) / dd[m - 2];
-  for (int i = m - 3; i >= 0; --i) r[i] = (r[i] - u[i] * r[i + 1] - v[i] * r[i + 2]) / dd[i];
+  if (fabs(c0) < tiny) c0 = tiny;
+  double x1 = cr * recip(c0);  // y[m-1]
+  r[m - 1] = x1;
+  double x2 = 0.0;             // y[i+2]
+  for (int i = m - 2; i >= 0; --i) {
+    const double xi = (r[i] - u[i] * x1 - v[i] * x2) * dd[i];
+    r[i] = xi;
+    x2 = x1;
+    x1 = xi;
+  }
 }
 
+template <bool TL>
 __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
+  __shared__ double2 pool[kPool];
   __shared__ double2 h1[kFM + 1], h2[kFM + 1];
-  __shared__ double al[kFM], be[kFM], s[kFM];
-  __shared__ double wk[4][kFM];  // inverse-iteration work arrays (thread 0)
+  __shared__ double al[kFM], be[kFM], be2[kFM], s[kFM];
+  __shared__ double wk[3][kFM];  // inverse-iteration pivot rows (thread 0)
   __shared__ int cnt[kFT];
   __shared__ double lo_s, hi_s, inv_s, theta_s, res_s;
   __shared__ int stop_s, status_s;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int64_t n = (int64_t)a.d * a.Dk * a.Dkr;
   const int m = a.m;
   double2* x = a.x;
+  // operands into LDS once for the whole eigensolve
+  const int64_t nL = (int64_t)a.Db * a.wl * a.Dk, nR = (int64_t)a.Dbr * a.wr * a.Dkr,
+                nW = (int64_t)a.d * a.wl * a.d * a.wr, n1 = (int64_t)a.Db * a.wl * a.d * a.Dkr;
+  FusedOps o;
+  double2* Ls = pool;
+  double2* Rs = Ls + nL;
+  double2* Ws = Rs + nR;
+  double2* X = Ws + nW;  // current basis vector V_j
+  double2* Y = X + n;    // new vector
+  o.L = Ls;
+  o.R = Rs;
+  o.W = Ws;
+  o.X = X;
+  o.Y = Y;
+  if constexpr (TL) {
+    o.T1 = Y + n;
+    o.T2 = o.T1 + n1;
+  } else {
+    o.T1 = a.T1;
+    o.T2 = a.T2;
+  }
+  for (int64_t t = tid; t < nL; t += kFT) Ls[t] = a.L[t];
+  for (int64_t t = tid; t < nR; t += kFT) Rs[t] = a.R[t];
+  for (int64_t t = tid; t < nW; t += kFT) Ws[t] = a.W[t];
   int matvecs = 0, restarts = 0, converged = 0;
   if (tid == 0) status_s = 0;
+  __syncthreads();
   for (int restart = 0; restart <= a.max_restarts; ++restart) {
     restarts = restart;
     // V_0 = x / |x|
-    multi_dot(x, n, 1, x, h1);
+    for (int64_t t = tid; t < n; t += kFT) Y[t] = x[t];
+    __syncthreads();
+    multi_dot(nullptr, n, 0, true, Y, h1);
     const double nx = h1[0].x;
     if (!(nx > 0.0) || !isfinite(nx)) {
       if (tid == 0) status_s = 2;  // the host reseeds
       break;
     }
     const double ix = 1.0 / sqrt(nx);
-    for (int64_t t = tid; t < n; t += kFT) a.V[t] = double2{x[t].x * ix, x[t].y * ix};
+    for (int64_t t = tid; t < n; t += kFT) {
+      const double2 v = double2{Y[t].x * ix, Y[t].y * ix};
+      a.V[t] = v;
+      X[t] = v;
+    }
     if (tid == 0) stop_s = 0;
     __syncthreads();
     int K = 0;
     for (int j = 0; j < m; ++j) {
-      double2* w = a.V + (int64_t)(j + 1) * n;
-      fused_matvec(a, a.V + (int64_t)j * n, w);
+      fused_matvec(a, o);  // Y = H V_j
       ++matvecs;
-      multi_dot(a.V, n, j + 1, w, h1);
-      gemv_sub(a.V, n, j + 1, h1, w);
-      multi_dot(a.V, n, j + 2, w, h2);  // h2[j + 1] = |w|^2
-      gemv_sub(a.V, n, j + 1, h2, w);
-      if (tid == 0) {
-        const double alpha = h1[j].x + h2[j].x;
+      multi_dot(a.V, n, j + 1, false, Y, h1);
+      gemv_sub(a.V, n, j + 1, h1, Y);
+      multi_dot(a.V, n, j + 1, true, Y, h2);  // h2[j + 1] = |w|^2
+      gemv_sub(a.V, n, j + 1, h2, Y);
+      if (tid < 64) {
+        // alpha_j, beta_j and the Gershgorin interval of T_j across one wave
         double corr = 0.0;
-        for (int i = 0; i <= j; ++i) corr = fma(h2[i].x, h2[i].x, fma(h2[i].y, h2[i].y, corr));
+        for (int i = lane; i <= j; i += 64) corr = fma(h2[i].x, h2[i].x, fma(h2[i].y, h2[i].y, corr));
+        corr = wave_sum(corr);
+        const double alpha = h1[j].x + h2[j].x;
         const double beta = sqrt(fmax(h2[j + 1].x - corr, 0.0));
-        al[j] = alpha;
-        be[j] = beta;
-        inv_s = beta > 1e-300 ? 1.0 / beta : 0.0;
-        if (!isfinite(alpha) || !isfinite(beta)) status_s = 3;
-        // Gershgorin interval of T_j
-        double lo = al[0], hi = al[0];
-        for (int i = 0; i <= j; ++i) {
-          const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < j ? fabs(be[i]) : 0.0);
-          lo = fmin(lo, al[i] - rad);
-          hi = fmax(hi, al[i] + rad);
+        if (lane == 0) {
+          al[j] = alpha;
+          be[j] = beta;
+          be2[j] = beta * beta;
         }
-        lo_s = lo;
-        hi_s = hi;
+        double lo = DBL_MAX, hi = -DBL_MAX;
+        for (int i = lane; i <= j; i += 64) {
+          const double ai = i == j ? alpha : al[i], bi = i == j ? beta : be[i];
+          const double rad = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < j ? fabs(bi) : 0.0);
+          lo = fmin(lo, ai - rad);
+          hi = fmax(hi, ai + rad);
+        }
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        if (lane == 0) {
+          inv_s = beta > 1e-300 ? 1.0 / beta : 0.0;
+          if (!isfinite(alpha) || !isfinite(beta)) status_s = 3;
+          lo_s = lo;
+          hi_s = hi;
+        }
       }
       __syncthreads();
       if (status_s == 3) break;
       const double inv = inv_s;
-      for (int64_t t = tid; t < n; t += kFT) w[t] = double2{w[t].x * inv, w[t].y * inv};
-      __syncthreads();  // V_{j+1} complete before the next matvec reads it
+      // V_{j+1}: to the basis in global memory and, as the next matvec's input, to X
+      for (int64_t t = tid; t < n; t += kFT) {
+        const double2 v = double2{Y[t].x * inv, Y[t].y * inv};
+        a.V[(int64_t)(j + 1) * n + t] = v;
+        X[t] = v;
+      }
+      __syncthreads();
       K = j + 1;
       // the tridiagonal problem is solved every third step and at the end of the basis
       if ((j + 1) % 3 != 0 && j + 1 < m && (int64_t)(j + 1) < n) continue;
@@ -262,7 +412,7 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi))) break;
         const double step = (hi - lo) / (kMS + 1);
         const double xk = lo + step * (tid + 1);
-        if (tid < kMS) cnt[tid] = (xk > lo && xk < hi) ? (sturm(al, be, j + 1, xk) >= 1) : (xk >= hi ? 1 : 0);
+        if (tid < kMS) cnt[tid] = (xk > lo && xk < hi) ? (sturm(al, be2, j + 1, xk) >= 1) : (xk >= hi ? 1 : 0);
         __syncthreads();
         // the first point whose count reaches one bounds the eigenvalue from above
         if (tid < kMS && cnt[tid] && (tid == 0 || !cnt[tid - 1])) {
@@ -278,8 +428,8 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
         const int mm = j + 1;
         for (int i = 0; i < mm; ++i) s[i] = 1.0;
         const double shift = theta - 4.0 * DBL_EPSILON * fmax(1.0, fabs(theta));
-        for (int it = 0; it < 3; ++it) {
-          tri_solve(al, be, mm, shift, s, wk[0], wk[1], wk[2], wk[3]);
+        for (int it = 0; it < 2; ++it) {
+          tri_solve(al, be, mm, shift, s, wk[0], wk[1], wk[2]);
           double nrm = 0.0;
           for (int i = 0; i < mm; ++i) nrm += s[i] * s[i];
           nrm = sqrt(nrm);
@@ -287,7 +437,8 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
             for (int i = 0; i < mm; ++i) s[i] = i == 0 ? 1.0 : 0.0;
             break;
           }
-          for (int i = 0; i < mm; ++i) s[i] /= nrm;
+          const double inr = 1.0 / nrm;
+          for (int i = 0; i < mm; ++i) s[i] *= inr;
         }
         const double res = be[j] * fabs(s[j]);
         const double scale = fmax(1.0, fabs(theta));
@@ -307,12 +458,12 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
         acc.x = fma(v.x, s[i], acc.x);
         acc.y = fma(v.y, s[i], acc.y);
       }
-      x[t] = acc;
+      Y[t] = acc;
     }
     __syncthreads();
-    multi_dot(x, n, 1, x, h1);
+    multi_dot(nullptr, n, 0, true, Y, h1);
     const double ir = h1[0].x > 0.0 ? 1.0 / sqrt(h1[0].x) : 0.0;
-    for (int64_t t = tid; t < n; t += kFT) x[t] = double2{x[t].x * ir, x[t].y * ir};
+    for (int64_t t = tid; t < n; t += kFT) x[t] = double2{Y[t].x * ir, Y[t].y * ir};
     __syncthreads();
     if (stop_s) {
       converged = 1;
@@ -339,7 +490,7 @@ bool lanczos_fused_fits(const mpsd_heff* h, int m) {
   // one CU's FP64 rate (~0.25 TFLOP/s) against the many-CU path's per-step launch cost
   // (~60 us): the one-workgroup solver wins up to ~1.5 MFLOP per matvec (D <= 24 at w = 3)
   return m + 1 <= kFM && n <= 2048 && F <= 1.5e6 && (int64_t)h->d * h->Db * h->Dbr == n &&
-         h->precision == MPSD_PREC_FP64 && !h->herm;
+         h->precision == MPSD_PREC_FP64 && !h->herm && fused_lds(h).ops <= kPool;
 }
 
 int lanczos_fused(mpsd_heff* h, double* x, double tol, int m, int max_restarts, double* out6) {
@@ -370,7 +521,11 @@ int lanczos_fused(mpsd_heff* h, double* x, double tol, int m, int max_restarts, 
   a.max_restarts = max_restarts;
   a.tol = tol;
   a.out = (double*)(a.V + (m + 1) * n);
-  hipLaunchKernelGGL(lanczos_fused_kernel, dim3(1), dim3(kFT), 0, ctx->stream, a);
+  const FusedLds f = fused_lds(h);
+  if (f.ops + f.inter <= kPool)
+    hipLaunchKernelGGL(lanczos_fused_kernel<true>, dim3(1), dim3(kFT), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(lanczos_fused_kernel<false>, dim3(1), dim3(kFT), 0, ctx->stream, a);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out6, a.out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
