@@ -151,7 +151,7 @@ struct mpsd_ctx {
   std::vector<std::pair<int, mpsd::GemmKey>> tune_seen;
   int cur_kind = -1;  // kind of the launch being issued (set by timed)
   int split3 = -1;  // last matvec stage split over the MPO bond: -1 auto, 0 off, 1 on
-  bool graph = true;  // eigensolver steps as replayed hipGraphs where the Heff allows it
+  bool graph = false;  // eigensolver steps as replayed hipGraphs where the Heff allows it
   int gauge_qr = 0;   // gauge-step QR / LQ: 0 the library's Householder kernels, 1 rocSOLVER
   int fused_lanczos = 1;  // small local problems: one-workgroup eigensolver (1) or the general one (0)
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
