@@ -230,6 +230,11 @@ int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n,
  * (w = 0 when beta_j underflows). */
 int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev, int j, double* ab_dev,
                       double* w, int64_t n);
+/* mpsd_vec_gemv_dev(V, j + 1, h2, -1, w) followed by mpsd_lanczos_coef(h1, h2, j, ab, w) in
+ * one launch, with the same arithmetic: w = (w - V_{0..j} h2) / beta_j, alpha_j and beta_j at
+ * ab_dev[2j], ab_dev[2j+1]. */
+int mpsd_lanczos_coef_update(mpsd_ctx* ctx, const double* V, int64_t ldv, const double* h1_dev,
+                             const double* h2_dev, int j, double* ab_dev, double* w, int64_t n);
 /* Projector terms of TensorMultiplierProjMulti::product (src/wrapper/multipliers/
  * TensorMultiplierProjMulti.cpp:87-96): y -= sum_j weights[j] <phi_j|x> phi_j, phi_j at
  * Phi + 2*j*ldphi, j < m <= 32; weights is a host array of m reals; x and y distinct device

@@ -300,8 +300,7 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
         check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 1, w, n, h1), "dots");
         check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h1, -1.0, w, n), "gemv");
         check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 2, w, n, h2), "dots2");
-        check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h2, -1.0, w, n), "gemv2");
-        check(ctx_, mpsd_lanczos_coef(ctx_, h1, h2, jj, ab + 2, w, n), "coef");
+        check(ctx_, mpsd_lanczos_coef_update(ctx_, V, ldv, h1, h2, jj, ab + 2, w, n), "gemv2 + coef");
       }
       check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef download");
       if (j == 0) {

@@ -68,6 +68,26 @@ __global__ void __launch_bounds__(kT) proj_update_kernel(const double2* __restri
   y[i] = acc;
 }
 
+// w = (w - V_{0..j} h2) / beta_j: the second update and lanczos_coef_kernel in one pass, with
+// the same operations (gemv_elem's, then the scaling); block 0 stores alpha_j, beta_j.
+__global__ void __launch_bounds__(kT) coef_update_kernel(const double2* __restrict__ V, int64_t ldv,
+                                                         const double2* __restrict__ h1,
+                                                         const double2* __restrict__ h2, int j,
+                                                         double* __restrict__ ab, double2* __restrict__ w,
+                                                         int64_t n) {
+  double alpha, beta, inv;
+  lanczos_scalars(h1, h2, j, alpha, beta, inv);
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < n; t += (int64_t)gridDim.x * kT) {
+    gemv_elem(V, ldv, j + 1, h2, -1.0, w, t);
+    const double2 v = w[t];
+    w[t] = double2{v.x * inv, v.y * inv};
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ab[2 * j] = alpha;
+    ab[2 * j + 1] = beta;
+  }
+}
+
 // Lanczos coefficients of step j from the two Gram-Schmidt passes (h1 = V^H w before the
 // first pass, h2 = [V_0..V_j, w]^H w after it): alpha_j = Re(h1[j] + h2[j]),
 // beta_j = sqrt(|w|^2 - sum_i |h2[i]|^2) (the norm after the second pass), then
@@ -387,6 +407,16 @@ int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev,
   hipLaunchKernelGGL(lanczos_coef_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)h1_dev,
                      (const double2*)h2_dev, j, ab_dev, (double2*)w, n);
   return hip_err(ctx, hipGetLastError(), "lanczos_coef");
+}
+
+int mpsd_lanczos_coef_update(mpsd_ctx* ctx, const double* V, int64_t ldv, const double* h1_dev,
+                             const double* h2_dev, int j, double* ab_dev, double* w, int64_t n) {
+  if (!ctx || !V || !h1_dev || !h2_dev || !ab_dev || !w || j < 0 || n < 0 || (j > 0 && ldv < n))
+    return MPSD_ERR_ARG;
+  const unsigned blocks = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kT - 1) / kT), 1024);
+  hipLaunchKernelGGL(coef_update_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)h1_dev, (const double2*)h2_dev, j, ab_dev, (double2*)w, n);
+  return hip_err(ctx, hipGetLastError(), "lanczos_coef_update");
 }
 
 int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n, double* nrm2_dev) {
