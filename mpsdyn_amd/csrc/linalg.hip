@@ -470,15 +470,19 @@ int mpsd_site_gauge_r(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out) {
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int m = d * Dl, n = Dr, k = m < n ? m : n;
   if (ctx->gauge_qr == 0 && m <= house_qr_max_rows()) {
-    // library Householder QR: site (m x n) = Q R, Q (m x k) back into the site
-    int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * k);
+    // library Householder QR: site (m x n) = Q R, Q (m x k) back into the site (written there
+    // directly by the single-workgroup form)
+    const bool small = house_qr_small(m, n);
+    int rc = small ? 0 : ensure_scratch(ctx, 3, sizeof(double2) * (size_t)m * k);
     const int rd[2] = {k, n};
     if (!rc) rc = mpsd_tensor_reshape(R_out, 2, rd);
-    if (!rc) rc = house_qr(ctx, site->ptr, m, n, (double2*)ctx->scratch[3], R_out->ptr);
+    if (!rc) rc = house_qr(ctx, site->ptr, m, n, small ? site->ptr : (double2*)ctx->scratch[3], R_out->ptr);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(site->ptr, ctx->scratch[3], sizeof(double2) * (size_t)m * k,
-                                  hipMemcpyDeviceToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_err(ctx, e, "gauge_r copy");
+    if (!small) {
+      hipError_t e = hipMemcpyAsync(site->ptr, ctx->scratch[3], sizeof(double2) * (size_t)m * k,
+                                    hipMemcpyDeviceToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_err(ctx, e, "gauge_r copy");
+    }
     const int sd[3] = {d, Dl, k};
     rc = mpsd_tensor_reshape(site, 3, sd);
     if (rc) return rc;

@@ -208,8 +208,11 @@ void heff_drop_hermitian(mpsd_heff* h);
 hipError_t heff_full_dev(mpsd_heff* h, double2* out);
 // Householder QR of A (m x n, column-major, in place) on ctx's stream: Q (m x k) and R
 // (k x n), k = min(m, n), in the LAPACK zgeqrf / zungqr conventions.  m <= house_qr_max_rows().
+// Small matrices (house_qr_small) take one single-workgroup launch that leaves A as it was and
+// lets Q be A itself; larger ones go through panels and overwrite A.
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R);
 int house_qr_max_rows();
+bool house_qr_small(int m, int n);
 // One-workgroup restarted Lanczos for small Heffs (lanczos_fused.hip): whether h fits, and
 // the solve (out6: theta, residual, matvecs, restarts, converged, status 0 / 2 start vector
 // zero or not finite / 3 non-finite coefficients).

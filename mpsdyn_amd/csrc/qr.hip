@@ -289,6 +289,249 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
   }
 }
 
+// ---- whole QR of a small matrix in one workgroup ----------------------------------------------
+// A (m x n, m <= 64, n <= 64) is factorised by one workgroup in the zlarfg / zung2r conventions
+// of the panel path, Q (m x k) is formed backwards, and Q and R are written out: one launch where
+// the panel path takes a dozen for a gauge step of a small site.
+//   Layout: lane u of every wave holds column u, wave w the rows [w RPT, (w + 1) RPT), in
+// registers, nw = ceil(m / RPT) <= 16 waves; the matrix is mirrored in LDS (X, leading dimension
+// m | 1).  A column step is one wave-uniform instruction stream, two barriers apart: every thread
+// evaluates zlarfg from the column's partial squared norms and reads its rows x_r of column c
+// from X as broadcasts; the reflector v = scal x below the diagonal (and 1 = scal (alpha - beta)
+// on it, so the diagonal row takes x_c = alpha - beta and needs no branch) is never formed: a
+// wave's partial of v^H a_u is conj(scal) sum conj(x_r) a_r, and the update is a_r -= x_r
+// (scal g).  Every lane then writes its updated column and its partial squared norm below the
+// next diagonal back to LDS.  Q is formed backwards from the columns left in X with the stored
+// scal and alpha - beta.  A is read whole (through LDS, coalesced) before anything is written,
+// so Q may be A itself.
+constexpr int kSQW = 16;            // most waves (the partial sums below are 16-slot trees)
+constexpr int kSQStage = 65 * 64;   // X: leading dimension m | 1
+
+struct Reflector {
+  double2 tau, scal, dd;  // dd = alpha - beta = 1 / scal
+  double beta;
+  bool reflect;
+};
+
+// 1 / x from the hardware reciprocal and two Newton steps (within an ulp or two of the quotient)
+__device__ __forceinline__ double rcp2(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// sum of the kSQW slots of a column of partials, as a tree
+template <typename T>
+__device__ __forceinline__ T tree_sum(T (*v)[64], int u);
+template <>
+__device__ __forceinline__ double tree_sum<double>(double (*v)[64], int u) {
+  double t[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) t[q] = v[q][u];
+#pragma unroll
+  for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+    for (int q = 0; q < h; ++q) t[q] += t[q + h];
+  return t[0];
+}
+template <>
+__device__ __forceinline__ double2 tree_sum<double2>(double2 (*v)[64], int u) {
+  double2 t[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) t[q] = v[q][u];
+#pragma unroll
+  for (int h = 8; h > 0; h >>= 1)
+#pragma unroll
+    for (int q = 0; q < h; ++q) {
+      t[q].x += t[q + h].x;
+      t[q].y += t[q + h].y;
+    }
+  return t[0];
+}
+
+// zlarfg on alpha and the squared norm xs2 of the rest of the column.  Unless the column's squares
+// underflow (scaled: xs2 is then the norm of the column over S, its largest entry), |alpha - beta|^2
+// >= xs2 >= 1e-280 and two plain reciprocals suffice; otherwise every step is scaled.
+__device__ __forceinline__ Reflector zlarfg_of(double2 alpha, double xs2, double S, bool scaled) {
+  Reflector f{{0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}, alpha.x, false};
+  if (xs2 == 0.0 && alpha.y == 0.0) return f;
+  f.reflect = true;
+  if (!scaled) {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xs2);
+    f.beta = alpha.x >= 0.0 ? -nrm : nrm;
+    const double rb = rcp2(f.beta);
+    f.tau = double2{(f.beta - alpha.x) * rb, -alpha.y * rb};
+    f.dd = double2{alpha.x - f.beta, alpha.y};
+    const double rd = rcp2(f.dd.x * f.dd.x + f.dd.y * f.dd.y);
+    f.scal = double2{f.dd.x * rd, -f.dd.y * rd};
+    return f;
+  }
+  const double ar = alpha.x / S, ai = alpha.y / S;
+  const double nrm = S * sqrt(ar * ar + ai * ai + xs2);
+  f.beta = alpha.x >= 0.0 ? -nrm : nrm;
+  f.tau = double2{(f.beta - alpha.x) / f.beta, -alpha.y / f.beta};
+  f.dd = double2{alpha.x - f.beta, alpha.y};
+  const double sd = fmax(fabs(f.dd.x), fabs(f.dd.y));
+  const double2 ds{f.dd.x / sd, f.dd.y / sd};
+  const double den = (ds.x * ds.x + ds.y * ds.y) * sd;
+  f.scal = double2{ds.x / den, -ds.y / den};
+  return f;
+}
+
+// acc += conj(x) a and a -= x g, in fused multiply-adds
+__device__ __forceinline__ void cj_fma(double2& acc, double2 x, double2 a) {
+  acc.x = fma(x.x, a.x, acc.x);
+  acc.x = fma(x.y, a.y, acc.x);
+  acc.y = fma(x.x, a.y, acc.y);
+  acc.y = fma(-x.y, a.x, acc.y);
+}
+__device__ __forceinline__ void sub_mul(double2& a, double2 x, double2 g) {
+  a.x = fma(-x.x, g.x, a.x);
+  a.x = fma(x.y, g.y, a.x);
+  a.y = fma(-x.x, g.y, a.y);
+  a.y = fma(-x.y, g.x, a.y);
+}
+
+template <int RPT>
+__global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, int m, int n, double2* Qg,
+                                                             double2* __restrict__ Rg) {
+  __shared__ double2 X[kSQStage];
+  __shared__ double2 part[kSQW][64];
+  __shared__ double nrm_s[kSQW][64];
+  __shared__ double2 tau_s[64], scal_s[64], dd_s[64];
+  __shared__ double beta_s[64];
+  const int tid = threadIdx.x, u = tid & 63, nt = blockDim.x, nw = nt >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), r0 = w * RPT;
+  const int k = min(m, n), ld = m | 1;
+  for (int e = tid; e < m * n; e += nt) X[e % m + ld * (e / m)] = Ag[e];
+  // slots of absent waves read as zeros
+  for (int e = nw * 64 + tid; e < kSQW * 64; e += nt) {
+    (&part[0][0])[e] = double2{0.0, 0.0};
+    (&nrm_s[0][0])[e] = 0.0;
+  }
+  __syncthreads();
+  double2 a[RPT], xr[RPT];
+  {
+    double s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + i;
+      a[i] = (u < n && r < m) ? X[r + ld * u] : double2{0.0, 0.0};
+      if (r > 0) s2 = fma(a[i].x, a[i].x, fma(a[i].y, a[i].y, s2));
+    }
+    nrm_s[w][u] = s2;
+  }
+  for (int c = 0; c < k; ++c) {
+    __syncthreads();
+    double xs2 = tree_sum<double>(nrm_s, c);
+    const double2* xc = X + ld * c;
+    const double2 alpha = xc[c];
+    double S = 1.0;
+    const bool scaled = xs2 < 1e-280;
+    if (scaled) {
+      // squares that underflow: rescale by the column's largest entry (zlarfg's safmin loop)
+      double mx = 0.0;
+      for (int r = c; r < m; ++r) mx = fmax(mx, fmax(fabs(xc[r].x), fabs(xc[r].y)));
+      double s2 = 0.0;
+      if (mx > 0.0)
+        for (int r = c + 1; r < m; ++r) {
+          const double p = xc[r].x / mx, q = xc[r].y / mx;
+          s2 += p * p + q * q;
+        }
+      xs2 = s2;
+      S = mx > 0.0 ? mx : 1.0;
+    }
+    const Reflector f = zlarfg_of(alpha, xs2, S, scaled);
+    if (tid == 0) {
+      tau_s[c] = f.tau;
+      scal_s[c] = f.scal;
+      dd_s[c] = f.dd;
+      beta_s[c] = f.beta;
+    }
+    // (H_c = I when !reflect: tau = scal = 0 and the step below only refreshes the partial norms)
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + i;
+      const double2 x = xc[r];
+      xr[i] = r > c && r < m ? x : (r == c ? f.dd : double2{0.0, 0.0});
+    }
+    // this wave's partial of v^H a_u
+    const bool mine = u > c && u < n;
+    {
+      double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], a[i]);
+      if (mine) part[w][u] = cjmul(f.scal, double2{acc0.x + acc1.x, acc0.y + acc1.y});
+    }
+    __syncthreads();
+    // H_c^H on the trailing columns: a_u -= conj(tau) v (v^H a_u), then back to LDS
+    if (mine) {
+      const double2 g2 = cmul(f.scal, cjmul(f.tau, tree_sum<double2>(part, u)));
+      double s2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int r = r0 + i;
+        sub_mul(a[i], xr[i], g2);
+        if (r < m) X[r + ld * u] = a[i];
+        if (r > c + 1) s2 = fma(a[i].x, a[i].x, fma(a[i].y, a[i].y, s2));
+      }
+      nrm_s[w][u] = s2;
+    }
+  }
+  __syncthreads();
+  // R (k x n): rows of X above the diagonal, beta on it
+  for (int e = tid; e < k * n; e += nt) {
+    const int i = e % k, j = e / k;
+    Rg[e] = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
+  }
+  // Q = H_0 ... H_{k-1} [I; 0], accumulated backwards: Q <- Q - tau v (v^H Q), columns u >= c
+  double2 qv[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) qv[i] = double2{(r0 + i == u && u < k) ? 1.0 : 0.0, 0.0};
+  for (int c = k - 1; c >= 0; --c) {
+    const double2 t = tau_s[c];
+    if (t.x == 0.0 && t.y == 0.0) continue;
+    const double2 sc = scal_s[c], dd = dd_s[c];
+    const double2* xc = X + ld * c;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = r0 + i;
+      const double2 x = xc[r];
+      xr[i] = r > c && r < m ? x : (r == c ? dd : double2{0.0, 0.0});
+    }
+    const bool mine = u >= c && u < k;
+    {
+      double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], qv[i]);
+      if (mine) part[w][u] = cjmul(sc, double2{acc0.x + acc1.x, acc0.y + acc1.y});
+    }
+    __syncthreads();
+    if (mine) {
+      const double2 g2 = cmul(sc, cmul(t, tree_sum<double2>(part, u)));
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) sub_mul(qv[i], xr[i], g2);
+    }
+    __syncthreads();
+  }
+  __syncthreads();  // R's reads of X are done wherever no Q step ran
+  if (u < k) {
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+      if (r0 + i < m) X[r0 + i + ld * u] = qv[i];
+  }
+  __syncthreads();
+  for (int e = tid; e < m * k; e += nt) Qg[e] = X[e % m + ld * (e / m)];
+}
+
+template <int RPT>
+void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R) {
+  const int nw = (m + RPT - 1) / RPT;
+  hipLaunchKernelGGL(qr_small_kernel<RPT>, dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R);
+}
+
 // Q = I (m x k)
 __global__ void __launch_bounds__(256) eye_kernel(double2* __restrict__ Q, int m, int k) {
   const int64_t total = (int64_t)m * k;
@@ -395,8 +638,23 @@ hipError_t plus_times(const double2* P, int ldp, int rows, int pb, const double2
 
 int house_qr_max_rows() { return 1024; }
 
+// m <= 64: beyond that (d = 2, D = 64) the panel path is as fast; the kernel's step is two barriers
+// and a zlarfg whatever the size, so it pays where the panels' dozen launches dominate
+bool house_qr_small(int m, int n) { return m <= 4 * kSQW && n <= 64; }
+
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   if (m < 1 || n < 1 || m > house_qr_max_rows()) return set_err(ctx, MPSD_ERR_ARG, "house_qr: unsupported size");
+  if (house_qr_small(m, n)) {
+    // up to 16 waves of 1, 2 or 4 rows
+    if (m <= 16)
+      launch_qr_small<1>(ctx->stream, A, m, n, Q, R);
+    else if (m <= 32)
+      launch_qr_small<2>(ctx->stream, A, m, n, Q, R);
+    else
+      launch_qr_small<4>(ctx->stream, A, m, n, Q, R);
+    return hip_err(ctx, hipGetLastError(), "house_qr small");
+  }
+  if (Q == A) return set_err(ctx, MPSD_ERR_ARG, "house_qr: Q may alias A only for small matrices");
   const int k = std::min(m, n);
   const int64_t mk = (int64_t)m * k;
   const int64_t wn = std::max(n, k), nch = (m + kVC - 1) / kVC;

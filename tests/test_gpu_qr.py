@@ -44,9 +44,12 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-# (d, Dl, Dr): panel edges (16), one- to four-row-per-thread panels (m up to 1024), wide factors
-SHAPES = [(2, 8, 16), (2, 20, 20), (2, 33, 17), (2, 64, 64), (2, 128, 128), (2, 200, 150), (2, 256, 256),
-          (2, 300, 512), (2, 512, 512), (3, 10, 40)]
+# (d, Dl, Dr): the single-workgroup form (m = d Dl <= 64, n = Dr <= 64: single rows and columns,
+# one, two and four rows per thread, wide and the largest cases), panel edges (16), one- to
+# four-row-per-thread panels (m up to 1024), wide factors
+SHAPES = [(2, 1, 1), (2, 1, 2), (2, 8, 1), (2, 8, 16), (2, 20, 20), (2, 32, 32), (2, 32, 64), (2, 16, 64),
+          (3, 20, 20), (2, 33, 17), (2, 50, 64), (2, 64, 64), (2, 65, 64), (2, 64, 65), (2, 128, 16),
+          (2, 128, 128), (2, 200, 150), (2, 200, 4), (2, 256, 256), (2, 300, 512), (2, 512, 512), (3, 10, 40)]
 
 
 @pytest.mark.parametrize("d,Dl,Dr", SHAPES)
@@ -66,7 +69,8 @@ def test_gauge_r_householder(ctx, d, Dl, Dr):
     assert _rel(R, R2) <= 1e-12 and _rel(Q, Q2) <= 1e-12
 
 
-@pytest.mark.parametrize("d,Dl,Dr", [(2, 16, 8), (2, 40, 20), (2, 64, 64), (2, 256, 200), (2, 512, 512), (3, 40, 10)])
+@pytest.mark.parametrize("d,Dl,Dr", [(2, 1, 1), (2, 16, 8), (2, 40, 20), (2, 64, 64), (2, 64, 32), (2, 256, 200),
+                                     (2, 512, 512), (3, 40, 10)])
 def test_gauge_l_householder(ctx, d, Dl, Dr):
     print(f"gauge_l d={d} Dl={Dl} Dr={Dr}", flush=True)
     A = orc.complex_gaussian(200 + Dl + Dr, (d, Dl, Dr))
@@ -81,10 +85,10 @@ def test_gauge_l_householder(ctx, d, Dl, Dr):
     assert _rel(Lm, L2) <= 1e-12 and _rel(Q, Q2) <= 1e-12
 
 
-def test_rank_deficient_and_zero_columns(ctx):
+@pytest.mark.parametrize("d,Dl,Dr", [(2, 30, 48), (2, 100, 96)])
+def test_rank_deficient_and_zero_columns(ctx, d, Dl, Dr):
     """Exactly zero and linearly dependent columns (tau = 0 reflectors, tiny pivots) still give
-    an isometry that reconstructs A."""
-    d, Dl, Dr = 2, 40, 48
+    an isometry that reconstructs A, in the single-workgroup and the panel form."""
     A = orc.complex_gaussian(7, (d, Dl, Dr))
     Am = orc.reshape_f(A, (d * Dl, Dr)).copy()
     Am[:, 5] = 0.0
