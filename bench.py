@@ -278,7 +278,7 @@ def main():
         sys.exit(2)
 
     from mpsdyn_amd import device as dev
-    from oracle.mpsd_oracle import complex_gaussian  # input generator only
+    from mpsdyn_amd.synth import complex_gaussian
 
     if args.traffic_probe:
         traffic_probe(dev, complex_gaussian, args.D, args.w)
