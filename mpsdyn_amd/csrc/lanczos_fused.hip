@@ -17,7 +17,8 @@
 //   reduced across one wave;
 // * the tridiagonal eigenproblem every third step: the lowest eigenvalue by multisection
 //   (256 Sturm counts per round), its eigenvector by two steps of inverse iteration on one
-//   thread, the elimination carried in registers; both divide through a refined hardware
+//   wave, row i of the tridiagonal in lane i's registers (taken by readlane, no LDS round
+//   trip in the elimination's dependent chain); both divide through a refined hardware
 //   reciprocal;
 // * restarts from the Ritz vector, and the final normalised Ritz vector in x.
 //
@@ -224,7 +225,7 @@ __device__ void multi_dot(const double2* __restrict__ V, int64_t n, int cnt, boo
 __device__ void gemv_sub(const double2* __restrict__ V, int64_t n, int cnt, const double2* h, double2* w) {
   for (int64_t t = threadIdx.x; t < n; t += kFT) {
     double2 acc{0.0, 0.0};
-#pragma unroll 4
+#pragma unroll 8
     for (int i = 0; i < cnt; ++i) acc = cfma(V[(int64_t)i * n + t], h[i], acc);
     const double2 wi = w[t];
     w[t] = double2{wi.x - acc.x, wi.y - acc.y};
@@ -247,24 +248,31 @@ __device__ int sturm(const double* a, const double* b2, int m, double x) {
   return cnt;
 }
 
-// (T - sigma) y = r in place on one thread: Gaussian elimination with partial pivoting of the
-// tridiagonal (diagonal a, off-diagonal b), m <= kFM.  The row being eliminated is carried in
-// registers; the finished pivot rows go to dd, u, v (shared work arrays) for the back
-// substitution, so no step waits on a value it has just stored.
-__device__ void tri_solve(const double* a, const double* b, int m, double sigma, double* r, double* dd, double* u,
-                          double* v) {
+// lane l's value of x (l uniform), and x with lane l's value replaced by v (v uniform)
+__device__ __forceinline__ double rl(double x, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double wl(double v, int l, double x) { return (int)(threadIdx.x & 63) == l ? v : x; }
+
+// tri_solve on one whole wave with row i in lane i (a_l, b_l, r_l: the diagonal, the
+// off-diagonal and the right-hand side; m <= 64): the same elimination and back substitution,
+// each row's operands taken by readlane and its pivot row left in lane i by a select.  Returns the
+// solution, lane i holding y[i].
+__device__ double tri_solve_wave(double a_l, double b_l, int m, double sigma, double r_l) {
   const double tiny = 1e-300;
   if (m == 1) {
-    double p = a[0] - sigma;
+    double p = rl(a_l, 0) - sigma;
     if (fabs(p) < tiny) p = tiny;
-    r[0] /= p;
-    return;
+    return wl(rl(r_l, 0) / p, 0, r_l);
   }
-  // current row i: columns i, i+1, i+2 and right-hand side
-  double c0 = a[0] - sigma, c1 = b[0], c2 = 0.0, cr = r[0];
+  double dd_l = 0.0, u_l = 0.0, v_l = 0.0;
+  double c0 = rl(a_l, 0) - sigma, c1 = rl(b_l, 0), c2 = 0.0, cr = rl(r_l, 0);
   for (int i = 0; i + 1 < m; ++i) {
-    // row i+1 before elimination: columns i, i+1, i+2
-    const double n0 = b[i], n1 = a[i + 1] - sigma, n2 = (i + 2 < m) ? b[i + 1] : 0.0, nr = r[i + 1];
+    const double n0 = rl(b_l, i), n1 = rl(a_l, i + 1) - sigma, n2 = (i + 2 < m) ? rl(b_l, i + 1) : 0.0,
+                 nr = rl(r_l, i + 1);
     double p0, p1, p2, pr, o0, o1, o2, orr;
     if (fabs(n0) > fabs(c0)) {
       p0 = n0, p1 = n1, p2 = n2, pr = nr;
@@ -276,25 +284,26 @@ __device__ void tri_solve(const double* a, const double* b, int m, double sigma,
     if (fabs(p0) < tiny) p0 = tiny;
     const double ip = recip(p0);
     const double f = o0 * ip;
-    dd[i] = ip;  // the back substitution multiplies by the reciprocal pivot
-    u[i] = p1;
-    v[i] = p2;
-    r[i] = pr;
+    dd_l = wl(ip, i, dd_l);
+    u_l = wl(p1, i, u_l);
+    v_l = wl(p2, i, v_l);
+    r_l = wl(pr, i, r_l);
     c0 = o1 - f * p1;
     c1 = o2 - f * p2;
     c2 = 0.0;
     cr = orr - f * pr;
   }
   if (fabs(c0) < tiny) c0 = tiny;
-  double x1 = cr * recip(c0);  // y[m-1]
-  r[m - 1] = x1;
-  double x2 = 0.0;             // y[i+2]
+  double x1 = cr * recip(c0);
+  r_l = wl(x1, m - 1, r_l);
+  double x2 = 0.0;
   for (int i = m - 2; i >= 0; --i) {
-    const double xi = (r[i] - u[i] * x1 - v[i] * x2) * dd[i];
-    r[i] = xi;
+    const double xi = (rl(r_l, i) - rl(u_l, i) * x1 - rl(v_l, i) * x2) * rl(dd_l, i);
+    r_l = wl(xi, i, r_l);
     x2 = x1;
     x1 = xi;
   }
+  return r_l;
 }
 
 template <bool TL>
@@ -302,7 +311,6 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
   __shared__ double2 pool[kPool];
   __shared__ double2 h1[kFM + 1], h2[kFM + 1];
   __shared__ double al[kFM], be[kFM], be2[kFM], s[kFM];
-  __shared__ double wk[3][kFM];  // inverse-iteration pivot rows (thread 0)
   __shared__ int cnt[kFT];
   __shared__ double lo_s, hi_s, inv_s, theta_s, res_s;
   __shared__ int stop_s, status_s;
@@ -423,28 +431,31 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
         if (tid == 0 && !cnt[kMS - 1]) lo_s = lo + step * kMS;  // above the last point
         __syncthreads();
       }
-      if (tid == 0) {
+      if (tid < 64) {
+        // the eigenvector by inverse iteration on wave 0, row i of T_j in lane i
         const double theta = 0.5 * (lo_s + hi_s);
         const int mm = j + 1;
-        for (int i = 0; i < mm; ++i) s[i] = 1.0;
+        const double a_l = lane < mm ? al[lane] : 0.0, b_l = lane < mm ? be[lane] : 0.0;
+        double s_l = lane < mm ? 1.0 : 0.0;
         const double shift = theta - 4.0 * DBL_EPSILON * fmax(1.0, fabs(theta));
         for (int it = 0; it < 2; ++it) {
-          tri_solve(al, be, mm, shift, s, wk[0], wk[1], wk[2]);
-          double nrm = 0.0;
-          for (int i = 0; i < mm; ++i) nrm += s[i] * s[i];
-          nrm = sqrt(nrm);
+          s_l = tri_solve_wave(a_l, b_l, mm, shift, s_l);
+          if (lane >= mm) s_l = 0.0;
+          const double nrm = sqrt(wave_sum(s_l * s_l));
           if (!(nrm > 0.0) || !isfinite(nrm)) {
-            for (int i = 0; i < mm; ++i) s[i] = i == 0 ? 1.0 : 0.0;
+            s_l = lane == 0 ? 1.0 : 0.0;
             break;
           }
-          const double inr = 1.0 / nrm;
-          for (int i = 0; i < mm; ++i) s[i] *= inr;
+          s_l *= 1.0 / nrm;
         }
-        const double res = be[j] * fabs(s[j]);
+        if (lane < mm) s[lane] = s_l;
+        const double res = rl(b_l, j) * fabs(rl(s_l, j));
         const double scale = fmax(1.0, fabs(theta));
-        theta_s = theta;
-        res_s = res;
-        stop_s = (res <= a.tol * scale || be[j] <= 1e-14 * scale || (int64_t)(j + 1) >= n) ? 1 : 0;
+        if (lane == 0) {
+          theta_s = theta;
+          res_s = res;
+          stop_s = (res <= a.tol * scale || be[j] <= 1e-14 * scale || (int64_t)(j + 1) >= n) ? 1 : 0;
+        }
       }
       __syncthreads();
       if (stop_s) break;

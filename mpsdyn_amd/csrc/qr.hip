@@ -645,9 +645,10 @@ bool house_qr_small(int m, int n) { return m <= 4 * kSQW && n <= 64; }
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   if (m < 1 || n < 1 || m > house_qr_max_rows()) return set_err(ctx, MPSD_ERR_ARG, "house_qr: unsupported size");
   if (house_qr_small(m, n)) {
-    // up to 16 waves of 1, 2 or 4 rows
+    // up to four waves of four rows for m <= 16 (fewer waves per SIMD beat shorter row loops
+    // there), else up to 16 waves of 2 or 4 rows
     if (m <= 16)
-      launch_qr_small<1>(ctx->stream, A, m, n, Q, R);
+      launch_qr_small<4>(ctx->stream, A, m, n, Q, R);
     else if (m <= 32)
       launch_qr_small<2>(ctx->stream, A, m, n, Q, R);
     else
