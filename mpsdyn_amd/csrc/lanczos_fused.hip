@@ -111,6 +111,8 @@ __device__ __forceinline__ void wave_zgemm16(int M, int N, int K, int m0, int n0
   const int lc = lane & 15, q = lane >> 4;
   fd4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
   const int m = m0 + lc, n = n0 + lc;
+  // unrolled so that the LDS reads of later k-steps are issued ahead of this step's MFMAs
+#pragma unroll 4
   for (int k0 = 0; k0 < K; k0 += 4) {
     const int k = k0 + q;
     const double2 av = (m < M && k < K) ? A(m, k) : double2{0.0, 0.0};
@@ -138,6 +140,14 @@ struct FusedOps {
   double2* T2;
 };
 
+// t / D as (t mgc) >> 32 with mgc = ceil(2^32 / D) (33 bits for D = 1): exact for D <= 2^10
+// and t < 2^20, since t (mgc D - 2^32) < 2^32 keeps the error term below 1 / D
+struct FastDiv {
+  unsigned long long mgc;
+  __device__ explicit FastDiv(unsigned d) : mgc((0x100000000ull + d - 1) / d) {}
+  __device__ __forceinline__ unsigned div(unsigned t) const { return (unsigned)(((unsigned long long)t * mgc) >> 32); }
+};
+
 // Y = H X on the whole workgroup.
 __device__ void fused_matvec(const FusedArgs& a, const FusedOps& o) {
   const int tid = threadIdx.x;
@@ -159,14 +169,18 @@ __device__ void fused_matvec(const FusedArgs& a, const FusedOps& o) {
   __syncthreads();
   // stage 2: T2[b, s', be, ar] = sum_{al, s} W[s', al, s, be] T1[b, al, s, ar]
   const int n2 = Db * d * wr * Dkr;
+  const FastDiv fDb(Db), fd(d), fwr(wr);
   for (int t = tid; t < n2; t += kFT) {
-    int q = t;
-    const int b = q % Db;
-    q /= Db;
-    const int sp = q % d;
-    q /= d;
-    const int be = q % wr;
-    const int ar = q / wr;
+    unsigned q = (unsigned)t;
+    unsigned q1 = fDb.div(q);
+    const int b = (int)(q - q1 * (unsigned)Db);
+    q = q1;
+    q1 = fd.div(q);
+    const int sp = (int)(q - q1 * (unsigned)d);
+    q = q1;
+    q1 = fwr.div(q);
+    const int be = (int)(q - q1 * (unsigned)wr);
+    const int ar = (int)q1;
     double2 acc{0.0, 0.0};
     for (int s = 0; s < d; ++s)
       for (int al = 0; al < wl; ++al)
