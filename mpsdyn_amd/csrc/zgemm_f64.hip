@@ -56,9 +56,14 @@ struct Pol<1> {  // FP32: v_mfma_f32_16x16x4f32, element (m, n) in lane n + 16*(
 __device__ __forceinline__ double2 cload(const double2* p) { return *p; }
 
 // Workgroup -> tile mapping.  The hardware hands linear workgroup ids to the 8 XCDs round
-// robin; the ids are first regrouped so each XCD owns a contiguous range, and that range is
-// walked in groups of kGroupM row-tiles so the tiles resident in one XCD's L2 share operand
-// panels.  z (the batch group) is slowest.
+// robin; the ids are first regrouped so each XCD owns a contiguous range.  z (the batch
+// group) is slowest.  When the tile grid of a z splits evenly into px x py = 8 blocks, the
+// ranges are those blocks, with (px, py) chosen to minimise the row and column tiles each XCD
+// touches: one XCD's L2 then fetches (gm/px) A panels and (gn/py) B panels instead of a
+// strip of A rows and nearly all of B (the D = 512 env x site stage: 4 x 2 blocks fetch
+// about 57 MB instead of 77 MB).  Each block (or the whole grid) is walked in groups of
+// kGroupM row-tiles so the tiles resident together share operand panels.  The mapping only
+// decides which workgroup computes a tile, never the order of its sums.
 constexpr int kGroupM = 8;
 
 __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, int& z) {
@@ -67,14 +72,36 @@ __device__ __forceinline__ void tile_of(int gm, int gn, int& tm_i, int& tn_i, in
   if ((total & 7) == 0) id = (id & 7) * (total >> 3) + (id >> 3);
   const int per_z = gm * gn;
   z = id / per_z;
-  const int r = id - z * per_z;
-  const int in_group = kGroupM * gn;
+  int r = id - z * per_z;
+  int px = 0, py = 0;
+  if ((total & 7) == 0 && (per_z & 7) == 0) {
+    int best = 1 << 30;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int qx = 8 >> f, qy = 1 << f;
+      if (gm % qx == 0 && gn % qy == 0 && gm / qx + gn / qy < best) {
+        best = gm / qx + gn / qy;
+        px = qx;
+        py = qy;
+      }
+    }
+  }
+  int bm = gm, bn = gn, om = 0, on = 0;
+  if (px) {
+    bm = gm / px;
+    bn = gn / py;
+    const int blk = r / (bm * bn);
+    r -= blk * (bm * bn);
+    om = (blk % px) * bm;
+    on = (blk / px) * bn;
+  }
+  const int in_group = kGroupM * bn;
   const int g = r / in_group;
   const int first_m = g * kGroupM;
-  const int gsize = min(gm - first_m, kGroupM);
+  const int gsize = min(bm - first_m, kGroupM);
   const int rr = r - g * in_group;
-  tm_i = first_m + rr % gsize;
-  tn_i = rr / gsize;
+  tm_i = om + first_m + rr % gsize;
+  tn_i = on + rr / gsize;
 }
 
 // LDS operand tiles of one workgroup, [k][m] and [k][n].  Each row carries one element of
