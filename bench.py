@@ -11,7 +11,7 @@ short pass before it, with every launch bracketed),
 the environment-update time at the same shape, and a CPU baseline (the numpy oracle with
 the reference's contraction order, OpenBLAS ZGEMM on the host cores) on a bounded sample.
 
-Ground-state part of the metric: --gs-solves N independent solves of the config-1 chain
+Ground-state part of the metric: --gs-solves N independent solves of the TFIM chain (L=20, D=20)
 (TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20, convtol 1e-12, seeds 1..N) run back to back;
 reported as solves/s, p50/p99 latency and the worst energy error against the reference's
 own value for that chain.
@@ -113,9 +113,9 @@ def cpu_baseline(L, W, R, x, flops, seconds):
 
 
 def ground_states(ctx, dev, n):
-    """n config-1 solves back to back: solves/s, latency percentiles, energy error."""
+    """n TFIM L=20, D=20 solves back to back: solves/s, latency percentiles, energy error."""
     with open(os.path.join(ROOT, "tests", "golden", "reference_values.json")) as f:
-        ref = json.load(f)["config1_tfim_L20_D20"]["reference_energy"]
+        ref = json.load(f)["tfim_L20_D20"]["reference_energy"]
     lat, errs, mv = [], [], 0
     dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12, seed=99))
     t0 = time.perf_counter()
@@ -189,7 +189,7 @@ def traffic_probe(dev, gaussian, D, w):
 
 
 def batched_form(dev, n):
-    """Config 4: n TFIM instances of uneven L, g, h, D submitted at once."""
+    """Batched form: n TFIM instances of uneven L, g, h, D submitted at once."""
     rng = np.random.default_rng(4)
     cfgs = []
     for k in range(n):
@@ -209,7 +209,7 @@ def batched_form(dev, n):
 
 
 def served_form(dev, n, rate, full):
-    """Config 5: Poisson arrivals of a model mix; FP64 and mixed precision."""
+    """Served form: Poisson arrivals of a model mix; FP64 and mixed precision."""
     rng = np.random.default_rng(5)
     Lmax, Smax, Ds = (200, 120, [32, 64, 128, 256, 512]) if full else (60, 60, [32, 64, 128])
     cfgs, t = [], 0.0

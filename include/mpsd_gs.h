@@ -27,7 +27,7 @@ typedef struct mpsd_gs_config {
   int D;              /* MPS bond dimension */
   double p[4];        /* model parameters, see MPSD_MODEL_* */
   double offset;      /* findGroundState offset argument */
-  double convtol;     /* Contractor::setConvTol (the configs use 1e-12 or 1e-10) */
+  double convtol;     /* Contractor::setConvTol (1e-12 for the L=20 reference value, 1e-10 elsewhere) */
   double eigtol;      /* residual tolerance of each local eigensolve (0: 1e-10) */
   int max_halfsweeps; /* <= 0: until converged */
   int precision;      /* MPSD_PREC_* of the matvecs */

@@ -43,7 +43,7 @@ Pinning (how each function is checked; tests/test_oracle.py):
   mpo_write/mpo_read -- a hand-written text file and a hand-assembled binary file laid out
                        as the reference's save / savetext code writes them (known answers),
                        and against the device library's own writer.
-  find_ground_state -- reference energies recorded in tests/golden (config 1 value from
+  find_ground_state -- reference energies recorded in tests/golden (the TFIM L=20, D=20 value from
                        the compiled reference), exact diagonalisation and the open-chain
                        free-fermion closed form.
   permute/reshape   -- the known answers of the reference's own unit test

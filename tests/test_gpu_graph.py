@@ -96,7 +96,7 @@ def test_fused_solver_sweeps_match():
             c.close()
         print(f"general={e[0]:.15f} ({t[0]:.3f}s) fused={e[1]:.15f} ({t[1]:.3f}s)", flush=True)
         assert abs(e[0] - e[1]) <= 1e-10 * abs(e[0])
-    ref = json.load(open(GOLDEN))["config1_tfim_L20_D20"]["reference_energy"]
+    ref = json.load(open(GOLDEN))["tfim_L20_D20"]["reference_energy"]
     c = dev.Context(0)
     r = dev.gs_solve(c, cfgs[0])
     c.close()

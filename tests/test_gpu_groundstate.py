@@ -38,9 +38,9 @@ def test_tfim_L8_reference_value(ctx):
 
 
 def test_config1_reference_value(ctx):
-    """Config 1: TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20, convtol 1e-12."""
+    """TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20, convtol 1e-12."""
     from mpsdyn_amd import device as dev
-    g = json.load(open(GOLDEN))["config1_tfim_L20_D20"]
+    g = json.load(open(GOLDEN))["tfim_L20_D20"]
     r = _solve(ctx, dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12)
     assert abs(r.energy - g["reference_energy"]) < 1e-10 * abs(g["reference_energy"])
     assert r.halfsweeps >= 2 and r.matvecs > 0

@@ -33,7 +33,7 @@ def _operands(Db, Dbr, wl, wr, d=2, seed=0, Dk=None, Dkr=None):
 
 
 SHAPES = [
-    # (Db, Dbr, wl, wr): edges, config-1 shape, ragged tiles, real DMRG shapes
+    # (Db, Dbr, wl, wr): edges, the L=20 D=20 shape, ragged tiles, real DMRG shapes
     (1, 2, 1, 3), (2, 1, 3, 1), (1, 1, 1, 1), (20, 20, 3, 3), (2, 4, 3, 3), (37, 50, 5, 5),
     (64, 64, 3, 3), (128, 128, 3, 3), (100, 128, 5, 5), (16, 33, 5, 3),
 ]
@@ -121,7 +121,7 @@ def test_shape_errors_are_reported(ctx):
 
 @pytest.mark.parametrize("D,w", [(256, 3), (512, 3), (512, 5)])
 def test_heff_full_size(ctx, D, w):
-    """Real DMRG shapes (config 2: D=256/512 Ising w=3, Heisenberg w=5) against the oracle,
+    """Real DMRG shapes (D=256/512 Ising w=3, Heisenberg w=5) against the oracle,
     plus Hermiticity <u, H v> = <H u, v> for a Hermitian-structured operator."""
     from mpsdyn_amd import device as dev
     print(f"full-size D={D} w={w}", flush=True)

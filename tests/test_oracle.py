@@ -295,8 +295,8 @@ def test_gs_schwinger_ed():
 
 @pytest.mark.slow
 def test_gs_config1_reference_value():
-    """Config 1: TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20 -> reference value."""
-    g = _golden()["config1_tfim_L20_D20"]
+    """TFIM -sum ZZ - sum X - 0.5 sum Z, L=20, D=20 -> reference value."""
+    g = _golden()["tfim_L20_D20"]
     mpo = orc.ising_mpo(20, -1.0, -1.0, -0.5)
     e, _, _, _ = orc.find_ground_state(mpo, orc.random_mps(20, 20, seed=1), convtol=1e-12)
     assert abs(e - g["reference_energy"]) < 1e-10 * abs(g["reference_energy"])
