@@ -522,6 +522,20 @@ int mpsd_site_gauge_l(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out) {
   if (!ctx || !site || !L_out || site == L_out) return MPSD_ERR_ARG;
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int m = Dl, n = d * Dr, k = m < n ? m : n;
+  if (ctx->gauge_qr == 0 && house_qr_small(n, m)) {
+    // the same LQ as below with the adjoint taken inside the single-workgroup QR
+    const int ld[2] = {m, k};
+    int rc = mpsd_tensor_reshape(L_out, 2, ld);
+    if (!rc) rc = house_lq_site_small(ctx, site->ptr, d, Dl, Dr, site->ptr, L_out->ptr);
+    const int sd[3] = {d, k, Dr};
+    if (!rc) rc = mpsd_tensor_reshape(site, 3, sd);
+    if (rc) return rc;
+    if (k == 1 && m == 1)
+      hipLaunchKernelGGL(edge_phase_kernel, dim3(1), dim3(kT), 0, ctx->stream, site->ptr, site->n, L_out->ptr);
+    ++site->version;
+    ++L_out->version;
+    return hip_err(ctx, hipGetLastError(), "gauge_l");
+  }
   if (ctx->gauge_qr == 0 && n <= house_qr_max_rows()) {
     // LQ of M[a, s + d c] = site[s, a, c] as the QR of B = M^H (n x m): M = R'^H Q'^H, so
     // L = R'^H (m x k) and the site becomes Q'^H reshaped to [d, k, Dr]

@@ -213,6 +213,9 @@ hipError_t heff_full_dev(mpsd_heff* h, double2* out);
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R);
 int house_qr_max_rows();
 bool house_qr_small(int m, int n);
+// LQ of a site tensor [d, Dl, Dr] in one launch (house_qr_small(d Dr, Dl)): L (Dl x k) and the
+// new site [d, k, Dr], k = min(Dl, d Dr), as Site::gaugeL defines them; site_out may be site.
+int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L);
 // One-workgroup restarted Lanczos for small Heffs (lanczos_fused.hip): whether h fits, and
 // the solve (out6: theta, residual, matvecs, restarts, converged, status 0 / 2 start vector
 // zero or not finite / 3 non-finite coefficients).

@@ -304,6 +304,9 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
 // next diagonal back to LDS.  Q is formed backwards from the columns left in X with the stored
 // scal and alpha - beta.  A is read whole (through LDS, coalesced) before anything is written,
 // so Q may be A itself.
+//   With lqd = d > 0 the kernel does the LQ of a site instead (Site::gaugeL): A is the site
+// tensor [d, n, m / d] and the matrix factorised is B[s + d c, a] = conj(site[s, a, c])
+// (m x n); Rg receives L = R^H (n x k) and Qg the new site [d, k, m / d] = Q^H reshaped.
 constexpr int kSQW = 16;            // most waves (the partial sums below are 16-slot trees)
 constexpr int kSQStage = 65 * 64;   // X: leading dimension m | 1
 
@@ -396,7 +399,7 @@ __device__ __forceinline__ void sub_mul(double2& a, double2 x, double2 g) {
 
 template <int RPT>
 __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, int m, int n, double2* Qg,
-                                                             double2* __restrict__ Rg) {
+                                                             double2* __restrict__ Rg, int lqd) {
   __shared__ double2 X[kSQStage];
   __shared__ double2 part[kSQW][64];
   __shared__ double nrm_s[kSQW][64];
@@ -405,7 +408,16 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
   const int tid = threadIdx.x, u = tid & 63, nt = blockDim.x, nw = nt >> 6;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), r0 = w * RPT;
   const int k = min(m, n), ld = m | 1;
-  for (int e = tid; e < m * n; e += nt) X[e % m + ld * (e / m)] = Ag[e];
+  if (lqd > 0) {
+    // site[s, a, c] (s + d (a + n c), read in order) -> B[s + d c, a] = conj
+    for (int e = tid; e < m * n; e += nt) {
+      const int s = e % lqd, a = (e / lqd) % n, c = e / (lqd * n);
+      const double2 v = Ag[e];
+      X[s + lqd * c + ld * a] = double2{v.x, -v.y};
+    }
+  } else {
+    for (int e = tid; e < m * n; e += nt) X[e % m + ld * (e / m)] = Ag[e];
+  }
   // slots of absent waves read as zeros
   for (int e = nw * 64 + tid; e < kSQW * 64; e += nt) {
     (&part[0][0])[e] = double2{0.0, 0.0};
@@ -481,10 +493,11 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     }
   }
   __syncthreads();
-  // R (k x n): rows of X above the diagonal, beta on it
+  // R (k x n): rows of X above the diagonal, beta on it; or L = R^H (n x k)
   for (int e = tid; e < k * n; e += nt) {
-    const int i = e % k, j = e / k;
-    Rg[e] = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
+    const int i = lqd > 0 ? e / n : e % k, j = lqd > 0 ? e % n : e / k;
+    const double2 r = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
+    Rg[e] = lqd > 0 ? double2{r.x, -r.y} : r;
   }
   // Q = H_0 ... H_{k-1} [I; 0], accumulated backwards: Q <- Q - tau v (v^H Q), columns u >= c
   double2 qv[RPT];
@@ -523,13 +536,33 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       if (r0 + i < m) X[r0 + i + ld * u] = qv[i];
   }
   __syncthreads();
-  for (int e = tid; e < m * k; e += nt) Qg[e] = X[e % m + ld * (e / m)];
+  if (lqd > 0) {
+    // new site[s, j, c] = conj(Q[s + d c, j]), written in order
+    for (int e = tid; e < m * k; e += nt) {
+      const int s = e % lqd, j = (e / lqd) % k, c = e / (lqd * k);
+      const double2 q = X[s + lqd * c + ld * j];
+      Qg[e] = double2{q.x, -q.y};
+    }
+  } else {
+    for (int e = tid; e < m * k; e += nt) Qg[e] = X[e % m + ld * (e / m)];
+  }
 }
 
 template <int RPT>
-void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R) {
+void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd) {
   const int nw = (m + RPT - 1) / RPT;
-  hipLaunchKernelGGL(qr_small_kernel<RPT>, dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R);
+  hipLaunchKernelGGL(qr_small_kernel<RPT>, dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
+}
+
+// rows per wave: up to four waves of four rows for m <= 16 (fewer waves per SIMD beat shorter
+// row loops there), else up to 16 waves of 2 or 4 rows
+void launch_qr_small_any(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd) {
+  if (m <= 16)
+    launch_qr_small<4>(s, A, m, n, Q, R, lqd);
+  else if (m <= 32)
+    launch_qr_small<2>(s, A, m, n, Q, R, lqd);
+  else
+    launch_qr_small<4>(s, A, m, n, Q, R, lqd);
 }
 
 // Q = I (m x k)
@@ -642,17 +675,17 @@ int house_qr_max_rows() { return 1024; }
 // and a zlarfg whatever the size, so it pays where the panels' dozen launches dominate
 bool house_qr_small(int m, int n) { return m <= 4 * kSQW && n <= 64; }
 
+int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L) {
+  if (d < 1 || Dl < 1 || Dr < 1 || !house_qr_small(d * Dr, Dl))
+    return set_err(ctx, MPSD_ERR_ARG, "house_lq_site_small: unsupported size");
+  launch_qr_small_any(ctx->stream, site, d * Dr, Dl, site_out, L, d);
+  return hip_err(ctx, hipGetLastError(), "house_lq_site_small");
+}
+
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   if (m < 1 || n < 1 || m > house_qr_max_rows()) return set_err(ctx, MPSD_ERR_ARG, "house_qr: unsupported size");
   if (house_qr_small(m, n)) {
-    // up to four waves of four rows for m <= 16 (fewer waves per SIMD beat shorter row loops
-    // there), else up to 16 waves of 2 or 4 rows
-    if (m <= 16)
-      launch_qr_small<4>(ctx->stream, A, m, n, Q, R);
-    else if (m <= 32)
-      launch_qr_small<2>(ctx->stream, A, m, n, Q, R);
-    else
-      launch_qr_small<4>(ctx->stream, A, m, n, Q, R);
+    launch_qr_small_any(ctx->stream, A, m, n, Q, R, 0);
     return hip_err(ctx, hipGetLastError(), "house_qr small");
   }
   if (Q == A) return set_err(ctx, MPSD_ERR_ARG, "house_qr: Q may alias A only for small matrices");
