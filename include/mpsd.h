@@ -81,6 +81,8 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
                                    steps, 4-14% faster per step for D = 32..512) or 1 */
 #define MPSD_OPT_GAUGE_QR 5     /* gauge-step QR / LQ: 0 library Householder kernels (default), 1 rocSOLVER */
 #define MPSD_OPT_FUSED_LANCZOS 6 /* small local problems on the one-workgroup eigensolver: 1 (default) or 0 */
+#define MPSD_OPT_ENV_CHAIN 7    /* small environment updates as one single-workgroup launch: 1 (default)
+                                   or 0 (three launches); the results are bit-identical */
 #define MPSD_OPT_BATCH_GROUP 4  /* batches per BRGEMM workgroup: 0 built-in (default), 1, or 2 when an
                                    operand is shared by the batches; results are the same for all */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);

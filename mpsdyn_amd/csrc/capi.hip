@@ -117,8 +117,12 @@ static int arith_of(int precision) {
   return precision == MPSD_PREC_FP32 ? 1 : precision == MPSD_PREC_BF16X3 ? 2 : 0;
 }
 
+// An environment update small enough for one workgroup (FP64 flops of its three stages; the
+// one-workgroup eigensolver's bound): one launch instead of three, same bits.
+static bool env_chain_fits(const mpsd_ctx* ctx, double flops) { return ctx->env_chain && flops <= 1.5e6; }
+
 static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, const double2* X,
-                       int d, int Dkr, const double2* mix, int wr, int arith = 0) {
+                       int d, int Dkr, const double2* mix, int wr, int arith = 0, Chain* chain = nullptr) {
   const size_t t1 = sizeof(double2) * (size_t)Db * wl * d * Dkr;
   const size_t t2 = sizeof(double2) * (size_t)Db * d * wr * Dkr;
   int rc = ensure_scratch(ctx, 0, t1);
@@ -145,6 +149,22 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
   g.a_mfast = 1;
   g.b_kfast = 1;
   g.arith = arith;
+  if (chain) {
+    // the first two steps of a chain: stage 1 and the mix, as launched below
+    chain->step[0].kind = 0;
+    chain->step[0].g = g;
+    ChainStep& mx = chain->step[1];
+    mx.kind = 1;
+    mx.in = (const double2*)ctx->scratch[0];
+    mx.out = (double2*)ctx->scratch[1];
+    mx.mix = mix;
+    mx.Bx = Db;
+    mx.nin = wl * d;
+    mx.nout = d * wr;
+    mx.Ax = Dkr;
+    chain->nsteps = 2;
+    return MPSD_OK;
+  }
   hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
                        [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
   if (e != hipSuccess) return hip_err(ctx, e, "stage1 launch");
@@ -155,45 +175,50 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
   return hip_err(ctx, e, "mix launch");
 }
 
-// Mix matrix of a left-type contraction: M[o = s' + d*be, i = al + wl*s] = W[s', al, s, be].
-static void mix_left(const std::vector<double2>& W, int d, int wl, int wr, std::vector<double2>& m) {
-  const int nout = d * wr, nin = wl * d;
-  m.assign((size_t)nout * nin, double2{0.0, 0.0});
-  for (int sp = 0; sp < d; ++sp)
-    for (int al = 0; al < wl; ++al)
-      for (int s = 0; s < d; ++s)
-        for (int be = 0; be < wr; ++be)
-          m[(sp + d * be) + (size_t)nout * (al + wl * s)] =
-              W[sp + d * al + (size_t)d * wl * s + (size_t)d * wl * d * be];
+// at most this many bytes of freed buffers are kept per context
+constexpr int64_t kPoolMaxBytes = int64_t(1) << 31;
+
+hipError_t pool_alloc(mpsd_ctx* ctx, int64_t n, double2** p, int64_t* cap) {
+  // the smallest pooled buffer that holds n elements without wasting more than half of it
+  auto it = ctx->pool.lower_bound(n);
+  if (it != ctx->pool.end() && it->first <= 2 * n + 256) {
+    *p = it->second;
+    *cap = it->first;
+    ctx->pool_bytes -= (int64_t)sizeof(double2) * it->first;
+    ctx->pool.erase(it);
+    return hipSuccess;
+  }
+  *cap = n;
+  return hipMalloc((void**)p, sizeof(double2) * (size_t)std::max<int64_t>(n, 1));
 }
 
-// Mix matrix of a right-type contraction: M[o = s' + d*al, i = be + wr*s] = W[s', al, s, be].
-static void mix_right(const std::vector<double2>& W, int d, int wl, int wr, std::vector<double2>& m) {
-  const int nout = d * wl, nin = wr * d;
-  m.assign((size_t)nout * nin, double2{0.0, 0.0});
-  for (int sp = 0; sp < d; ++sp)
-    for (int al = 0; al < wl; ++al)
-      for (int s = 0; s < d; ++s)
-        for (int be = 0; be < wr; ++be)
-          m[(sp + d * al) + (size_t)nout * (be + wr * s)] =
-              W[sp + d * al + (size_t)d * wl * s + (size_t)d * wl * d * be];
+void pool_free(mpsd_ctx* ctx, double2* p, int64_t cap) {
+  if (!p) return;
+  const int64_t bytes = (int64_t)sizeof(double2) * cap;
+  if (ctx->pool_bytes + bytes > kPoolMaxBytes) {
+    (void)hipFree(p);
+    return;
+  }
+  ctx->pool.emplace(cap, p);
+  ctx->pool_bytes += bytes;
 }
 
-static int fetch_w(mpsd_ctx* ctx, const mpsd_tensor* W, std::vector<double2>& h) {
-  h.resize(W->n);
-  hipError_t e = hipMemcpyAsync(h.data(), W->ptr, sizeof(double2) * W->n, hipMemcpyDeviceToHost,
-                                ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  return hip_err(ctx, e, "fetch W");
+namespace {
+// The mix matrix of W, a permutation of its elements formed on the device (no read-back of W,
+// no upload, nothing the host waits for):
+//   left-type contraction (kind 0):  M[o = s' + d be, i = al + wl s] = W[s', al, s, be],
+//   right-type contraction (kind 1): M[o = s' + d al, i = be + wr s] = W[s', al, s, be].
+__global__ void __launch_bounds__(256) mix_perm_kernel(const double2* __restrict__ W, int d, int wl, int wr,
+                                                       int kind, double2* __restrict__ m) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= d * wl * d * wr) return;
+  const int sp = t % d, al = (t / d) % wl, s = (t / (d * wl)) % d, be = t / (d * wl * d);
+  if (kind == 0)
+    m[(sp + d * be) + (int64_t)d * wr * (al + wl * s)] = W[t];
+  else
+    m[(sp + d * al) + (int64_t)d * wl * (be + wr * s)] = W[t];
 }
-
-static int upload_mix(mpsd_ctx* ctx, const std::vector<double2>& m, double2** dst) {
-  hipError_t e = hipMalloc((void**)dst, sizeof(double2) * m.size());
-  if (e != hipSuccess) return hip_err(ctx, e, "mix alloc");
-  e = hipMemcpyAsync(*dst, m.data(), sizeof(double2) * m.size(), hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  return hip_err(ctx, e, "mix upload");
-}
+}  // namespace
 
 }  // namespace mpsd
 
@@ -207,19 +232,21 @@ int get_mix(mpsd_ctx* ctx, const mpsd_tensor* W, int kind, int d, int wl, int wr
     *out = W->mix[kind];
     return MPSD_OK;
   }
-  std::vector<double2> hw, hm;
-  int rc = fetch_w(ctx, W, hw);
-  if (rc) return rc;
-  if (kind == 0)
-    mix_left(hw, d, wl, wr, hm);
-  else
-    mix_right(hw, d, wl, wr, hm);
-  if (W->mix[kind]) {
-    (void)hipFree(W->mix[kind]);
+  const int64_t n = (int64_t)d * wl * d * wr;
+  if (n != W->n) return set_err(ctx, MPSD_ERR_SHAPE, "mix: W size does not match its dimensions");
+  if (W->mix[kind] && W->mix_cap[kind] < n) {
+    (void)hipStreamSynchronize(ctx->stream);
+    pool_free(ctx, W->mix[kind], W->mix_cap[kind]);
     W->mix[kind] = nullptr;
   }
-  rc = upload_mix(ctx, hm, &W->mix[kind]);
-  if (rc) return rc;
+  if (!W->mix[kind]) {
+    hipError_t e = pool_alloc(ctx, n, &W->mix[kind], &W->mix_cap[kind]);
+    if (e != hipSuccess) return hip_err(ctx, e, "mix alloc");
+  }
+  hipLaunchKernelGGL(mix_perm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, W->ptr, d, wl,
+                     wr, kind, W->mix[kind]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_err(ctx, e, "mix build");
   W->mix_version[kind] = W->version;
   *out = W->mix[kind];
   return MPSD_OK;
@@ -261,6 +288,8 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   if (ctx->gtab_ev) (void)hipEventDestroy(ctx->gtab_ev);
   if (ctx->coef) (void)hipFree(ctx->coef);
   if (ctx->coef_host) (void)hipHostFree(ctx->coef_host);
+  for (auto& kv : ctx->pool) (void)hipFree(kv.second);
+  ctx->pool.clear();
   for (hipEvent_t e : ctx->coef_ev)
     if (e) (void)hipEventDestroy(e);
   destroy_blas(ctx);
@@ -323,6 +352,7 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_BATCH_GROUP: return ctx->tile_override.nbw;
     case MPSD_OPT_GAUGE_QR: return ctx->gauge_qr;
     case MPSD_OPT_FUSED_LANCZOS: return ctx->fused_lanczos;
+    case MPSD_OPT_ENV_CHAIN: return ctx->env_chain;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -349,6 +379,9 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_FUSED_LANCZOS:
       ctx->fused_lanczos = value != 0;
       return MPSD_OK;
+    case MPSD_OPT_ENV_CHAIN:
+      ctx->env_chain = value != 0;
+      return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");
       ctx->tile_override.nbw = value;
@@ -365,7 +398,7 @@ void* mpsd_ctx_stream(mpsd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr
 
 int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx) {
   if (!ctx) return 0;
-  int64_t b = ctx->tensor_bytes + (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1] +
+  int64_t b = ctx->tensor_bytes + ctx->pool_bytes + (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1] +
               (int64_t)ctx->dots_bytes;
   for (int i = 0; i < mpsd_ctx::kScratch; ++i) b += (int64_t)ctx->scratch_bytes[i];
   return b;
@@ -383,15 +416,14 @@ int mpsd_tensor_create(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** o
   t->rank = rank;
   for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
   t->n = n;
-  t->cap = n;
-  hipError_t e = hipMalloc((void**)&t->ptr, sizeof(double2) * (size_t)n);
+  hipError_t e = pool_alloc(ctx, n, &t->ptr, &t->cap);
   if (e == hipSuccess) e = hipMemsetAsync(t->ptr, 0, sizeof(double2) * (size_t)n, ctx->stream);
   if (e != hipSuccess) {
-    if (t->ptr) (void)hipFree(t->ptr);
+    if (t->ptr) pool_free(ctx, t->ptr, t->cap);
     delete t;
     return hip_err(ctx, e, "tensor_create");
   }
-  ctx->tensor_bytes += (int64_t)sizeof(double2) * n;
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * t->cap;
   *out = t;
   return MPSD_OK;
 }
@@ -399,9 +431,8 @@ int mpsd_tensor_create(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** o
 int mpsd_tensor_destroy(mpsd_tensor* t) {
   if (!t) return MPSD_ERR_ARG;
   (void)hipStreamSynchronize(t->ctx->stream);
-  if (t->ptr) (void)hipFree(t->ptr);
-  for (int k = 0; k < 2; ++k)
-    if (t->mix[k]) (void)hipFree(t->mix[k]);
+  pool_free(t->ctx, t->ptr, t->cap);
+  for (int k = 0; k < 2; ++k) pool_free(t->ctx, t->mix[k], t->mix_cap[k]);
   t->ctx->tensor_bytes -= (int64_t)sizeof(double2) * t->cap;
   delete t;
   return MPSD_OK;
@@ -428,14 +459,13 @@ int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims) {
   if (n > t->cap) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_err(ctx, e, "reshape sync");
-    (void)hipFree(t->ptr);
+    pool_free(ctx, t->ptr, t->cap);
     ctx->tensor_bytes -= (int64_t)sizeof(double2) * t->cap;
     t->ptr = nullptr;
     t->cap = 0;
-    e = hipMalloc((void**)&t->ptr, sizeof(double2) * (size_t)n);
+    e = pool_alloc(ctx, n, &t->ptr, &t->cap);
     if (e != hipSuccess) return hip_err(ctx, e, "reshape alloc");
-    t->cap = n;
-    ctx->tensor_bytes += (int64_t)sizeof(double2) * n;
+    ctx->tensor_bytes += (int64_t)sizeof(double2) * t->cap;
   }
   t->rank = rank;
   for (int i = 0; i < 8; ++i) t->dims[i] = i < rank ? dims[i] : 0;
@@ -496,7 +526,11 @@ int mpsd_env_left(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* ket
   const int odims[3] = {Dbr, wr, Dkr};
   rc = mpsd_tensor_reshape(out, 3, odims);
   if (rc) return rc;
-  rc = left_stages(ctx, envL->ptr, Db, wl, Dk, ket->ptr, d, Dkr, mix, wr);
+  const double flops = 8.0 * ((double)d * wl * Db * Dk * Dkr + (double)d * d * wl * wr * Db * Dkr +
+                              (double)d * wr * Db * Dbr * Dkr);
+  Chain chain;
+  const bool one = env_chain_fits(ctx, flops);
+  rc = left_stages(ctx, envL->ptr, Db, wl, Dk, ket->ptr, d, Dkr, mix, wr, 0, one ? &chain : nullptr);
   if (rc) return rc;
   // L'(c, n = be + wr*ar) = sum_{s'} sum_b conj(bra[s', b, c]) T2[b + Db*s' + Db*d*n]
   ZgemmDesc g;
@@ -519,6 +553,13 @@ int mpsd_env_left(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* ket
   g.conjA = 1;
   g.a_mfast = 0;
   g.b_kfast = 1;
+  if (one) {
+    chain.step[2].kind = 0;
+    chain.step[2].g = g;
+    chain.nsteps = 3;
+    return hip_err(ctx, timed(ctx, MPSD_KERNEL_ENV_STAGE3, [&] { return launch_zchain_f64(chain, ctx->stream); }),
+                   "env_left chain");
+  }
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_ENV_STAGE3,
                        [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); }),
@@ -573,15 +614,34 @@ int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ke
   g.sCb = (int64_t)Dbr * wr;
   g.a_mfast = 1;
   g.b_kfast = 0;
-  hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
-                       [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
-  if (e != hipSuccess) return hip_err(ctx, e, "env_right stage1");
-  // U2[c, s', al, a] = sum_{be,s} W[s', al, s, be] U1[c, be, s, a]
-  e = timed(ctx, MPSD_KERNEL_MIX, [&] {
-    return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Dbr,
-                           wr * d, d * wl, Dkl, ctx->stream);
-  });
-  if (e != hipSuccess) return hip_err(ctx, e, "env_right mix");
+  const double flops = 8.0 * ((double)d * wr * Dbr * Dkr * Dkl + (double)d * d * wl * wr * Dbr * Dkl +
+                              (double)d * wl * Dbr * Dbl * Dkl);
+  const bool one = env_chain_fits(ctx, flops);
+  Chain chain;
+  if (one) {
+    // stage 1 and the mix as the first two steps of one launch
+    chain.step[0].kind = 0;
+    chain.step[0].g = g;
+    ChainStep& mx = chain.step[1];
+    mx.kind = 1;
+    mx.in = (const double2*)ctx->scratch[0];
+    mx.out = (double2*)ctx->scratch[1];
+    mx.mix = mix;
+    mx.Bx = Dbr;
+    mx.nin = wr * d;
+    mx.nout = d * wl;
+    mx.Ax = Dkl;
+  } else {
+    hipError_t e = timed(ctx, MPSD_KERNEL_STAGE1,
+                         [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
+    if (e != hipSuccess) return hip_err(ctx, e, "env_right stage1");
+    // U2[c, s', al, a] = sum_{be,s} W[s', al, s, be] U1[c, be, s, a]
+    e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+      return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Dbr,
+                             wr * d, d * wl, Dkl, ctx->stream);
+    });
+    if (e != hipSuccess) return hip_err(ctx, e, "env_right mix");
+  }
   // R'(b, n = al + wl*a) = sum_{s'} sum_c conj(bra[s', b, c]) U2[c + Dbr*s' + Dbr*d*n]
   ZgemmDesc h;
   h.A = bra->ptr;
@@ -603,6 +663,13 @@ int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ke
   h.conjA = 1;
   h.a_mfast = 1;
   h.b_kfast = 1;
+  if (one) {
+    chain.step[2].kind = 0;
+    chain.step[2].g = h;
+    chain.nsteps = 3;
+    return hip_err(ctx, timed(ctx, MPSD_KERNEL_ENV_STAGE3, [&] { return launch_zchain_f64(chain, ctx->stream); }),
+                   "env_right chain");
+  }
   return hip_err(ctx,
                  timed(ctx, MPSD_KERNEL_ENV_STAGE3,
                        [&] { return launch_zbrgemm_f64(h, tile_for(ctx, h), ctx->stream); }),

@@ -103,6 +103,25 @@ hipError_t launch_zsum_parts_f64(const double2* P, int np, int64_t stride, const
 
 // Apply the MPO "mix" of the middle contraction stage:
 //   out[bx + Bx*o + Bx*nout*ax] = sum_i mix[o + nout*i] * in[bx + Bx*i + Bx*nin*ax]
+// A short chain of small FP64 contractions in one workgroup: up to three steps, each a BRGEMM
+// (ZgemmDesc, batches b < NB at the s*b strides, no groups) or an MPO mix (the arguments of
+// launch_zmix_f64), run in order with a barrier between them.  Every output element is
+// accumulated exactly as launch_zbrgemm_f64 / launch_zmix_f64 accumulate it, so the results
+// are bit-identical to the separate launches; only the launch count changes.
+struct ChainStep {
+  int kind = 0;  // 0: BRGEMM, 1: MPO mix
+  ZgemmDesc g;
+  const double2* in = nullptr;
+  double2* out = nullptr;
+  const double2* mix = nullptr;
+  int Bx = 0, nin = 0, nout = 0;
+  int64_t Ax = 0;
+};
+struct Chain {
+  ChainStep step[3];
+  int nsteps = 0;
+};
+hipError_t launch_zchain_f64(const Chain& c, hipStream_t s);
 hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, int Bx, int nin,
                            int nout, int64_t Ax, hipStream_t s);
 
@@ -142,6 +161,13 @@ struct mpsd_ctx {
   void* stage[2] = {nullptr, nullptr};
   size_t stage_bytes[2] = {0, 0};
   int64_t tensor_bytes = 0;
+  // Device buffers of destroyed tensors (and mix matrices), by capacity in complex elements,
+  // for the next tensors of this context: a solve's setup creates and frees some hundred
+  // tensors, and hipMalloc / hipFree cost more than the kernels that fill them.  All of a
+  // context's work is on its stream, and a tensor is destroyed after that stream is drained,
+  // so a pooled buffer is never still in use when it is handed out again.
+  std::multimap<int64_t, double2*> pool;
+  int64_t pool_bytes = 0;
   mpsd::TileCfg tile_override{0, 0, 16};
   bool tile_user = false;  // tile_override set away from the built-in choice
   // Autotuner: fastest configuration per BRGEMM shape class (mpsd_tune_*), and, while a
@@ -154,6 +180,7 @@ struct mpsd_ctx {
   bool graph = false;  // eigensolver steps as replayed hipGraphs where the Heff allows it
   int gauge_qr = 0;   // gauge-step QR / LQ: 0 the library's Householder kernels, 1 rocSOLVER
   int fused_lanczos = 1;  // small local problems: one-workgroup eigensolver (1) or the general one (0)
+  int env_chain = 1;      // small environment updates: one single-workgroup launch (1) or three (0)
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   unsigned timing_mask = ~0u;  // kernel kinds timed when timing is on
@@ -167,6 +194,13 @@ struct mpsd_ctx {
   int64_t launches[MPSD_KERNEL_KINDS] = {0};
 };
 
+// A device buffer of at least n complex elements from the context's pool (capacity in *cap),
+// or a new one; and a buffer back to the pool (freed when the pool would exceed its bound).
+namespace mpsd {
+hipError_t pool_alloc(mpsd_ctx* ctx, int64_t n, double2** p, int64_t* cap);
+void pool_free(mpsd_ctx* ctx, double2* p, int64_t cap);
+}  // namespace mpsd
+
 struct mpsd_tensor {
   mpsd_ctx* ctx = nullptr;
   int rank = 0;
@@ -178,6 +212,7 @@ struct mpsd_tensor {
   // mix matrices derived from it when the tensor is used as an MPO tensor W.
   uint64_t version = 0;
   mutable double2* mix[2] = {nullptr, nullptr};
+  mutable int64_t mix_cap[2] = {0, 0};  // capacity of each mix buffer, complex elements
   mutable uint64_t mix_version[2] = {0, 0};
 };
 

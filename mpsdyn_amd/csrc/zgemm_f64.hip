@@ -694,6 +694,106 @@ hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, 
   return hipGetLastError();
 }
 
+namespace {
+
+constexpr int kChainT = 1024;
+
+// One BRGEMM of a chain: 16 x 16 output blocks (per batch) over the waves, each block the
+// sequence of v_mfma_f64_16x16x4f64 the tiled kernel issues for it (batches at s*b, r outer,
+// k ascending in steps of 4 with lane group q on k0 + q, zero beyond M, N, K; the conjugate
+// and the three-product operands formed as its LDS staging forms them) and the same epilogue.
+__device__ void chain_gemm(const ZgemmDesc& p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 15, q = lane >> 4;
+  const int gm = (p.M + 15) / 16, gn = (p.N + 15) / 16, per_b = gm * gn;
+  const double csign = p.conjA ? -1.0 : 1.0;
+  for (int blk = wave; blk < per_b * p.NB; blk += kChainT / 64) {
+    const int b = blk / per_b, rem = blk - b * per_b;
+    const int m = (rem % gm) * 16 + lc, n = (rem / gm) * 16 + lc;
+    const double2* Ab = p.A + (int64_t)b * p.sAb;
+    const double2* Bb = p.B + (int64_t)b * p.sBb;
+    d4 p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+    for (int r = 0; r < p.R; ++r) {
+#pragma unroll 4
+      for (int k0 = 0; k0 < p.K; k0 += 4) {
+        const int k = k0 + q;
+        const double2 av = (m < p.M && k < p.K) ? Ab[m * p.sAm + k * p.sAk + r * p.sAr] : double2{0.0, 0.0};
+        const double2 bv = (k < p.K && n < p.N) ? Bb[k * p.sBk + n * p.sBn + r * p.sBr] : double2{0.0, 0.0};
+        const double ar = av.x, ai = csign * av.y;
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bv.x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bv.y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, bv.x + bv.y, p3, 0, 0, 0);
+      }
+    }
+    double2* Cb = p.C + (int64_t)b * p.sCb;
+    const double2* Db = p.Dadd ? p.Dadd + (int64_t)b * p.sCb : nullptr;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int mo = (rem % gm) * 16 + q + 4 * rr;
+      if (mo < p.M && n < p.N) {
+        const int64_t off = mo * p.sCm + n * p.sCn;
+        double2 v{p1[rr] - p2[rr], p3[rr] - (p1[rr] + p2[rr])};
+        if (Db) {
+          const double2 x = Db[off];
+          v.x = fma(p.beta, x.x, v.x);
+          v.y = fma(p.beta, x.y, v.y);
+        }
+        Cb[off] = v;
+      }
+    }
+  }
+}
+
+// One MPO mix of a chain, element by element as zmix_f64_kernel sums it.
+__device__ void chain_mix(const ChainStep& c) {
+  const int64_t total = (int64_t)c.Bx * c.Ax;
+  for (int64_t gid = threadIdx.x; gid < total; gid += kChainT) {
+    const int bx = (int)(gid % c.Bx);
+    const int64_t ax = gid / c.Bx;
+    const double2* src = c.in + bx + (int64_t)c.Bx * c.nin * ax;
+    double2* dst = c.out + bx + (int64_t)c.Bx * c.nout * ax;
+    for (int o0 = 0; o0 < c.nout; o0 += kMixMax) {
+      double2 acc[kMixMax];
+#pragma unroll
+      for (int o = 0; o < kMixMax; ++o) acc[o] = double2{0.0, 0.0};
+      for (int i = 0; i < c.nin; ++i) {
+        const double2 v = src[(int64_t)c.Bx * i];
+#pragma unroll
+        for (int o = 0; o < kMixMax; ++o) {
+          if (o0 + o < c.nout) {
+            const double2 w = c.mix[o0 + o + c.nout * i];
+            acc[o].x = fma(w.x, v.x, fma(-w.y, v.y, acc[o].x));
+            acc[o].y = fma(w.x, v.y, fma(w.y, v.x, acc[o].y));
+          }
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kMixMax; ++o)
+        if (o0 + o < c.nout) dst[(int64_t)c.Bx * (o0 + o)] = acc[o];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kChainT) zchain_kernel(const Chain c) {
+  for (int i = 0; i < c.nsteps; ++i) {
+    if (c.step[i].kind == 0)
+      chain_gemm(c.step[i].g);
+    else
+      chain_mix(c.step[i]);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+hipError_t launch_zchain_f64(const Chain& c, hipStream_t s) {
+  if (c.nsteps < 1 || c.nsteps > 3) return hipErrorInvalidValue;
+  for (int i = 0; i < c.nsteps; ++i)
+    if (c.step[i].kind == 0 && (c.step[i].g.arith != 0 || c.step[i].g.group != 0)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(zchain_kernel, dim3(1), dim3(kChainT), 0, s, c);
+  return hipGetLastError();
+}
+
 hipError_t launch_heff_full_f64(const double2* L, const double2* W, const double2* R, double2* H,
                                 int d, int Db, int wl, int Dk, int Dbr, int wr, int Dkr,
                                 double offset, hipStream_t s) {

@@ -29,6 +29,7 @@ OPT_GRAPH = 3
 OPT_BATCH_GROUP = 4
 OPT_GAUGE_QR = 5
 OPT_FUSED_LANCZOS = 6
+OPT_ENV_CHAIN = 7
 
 _lib = None
 

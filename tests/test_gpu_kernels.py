@@ -138,3 +138,32 @@ def test_heff_full_size(ctx, D, w):
     Hu, Hx = h.apply(u), h.apply(x)
     assert abs(np.vdot(u, Hx) - np.vdot(Hu, x)) <= 1e-11 * np.linalg.norm(Hu) * np.linalg.norm(x)
     h.close()
+
+
+@pytest.mark.parametrize("d,Dl,Dr,wl,wr", [(2, 1, 2, 1, 3), (2, 20, 20, 3, 3), (2, 2, 1, 3, 1), (3, 9, 14, 5, 5),
+                                           (2, 17, 23, 4, 2), (2, 24, 24, 3, 3), (2, 8, 8, 9, 9)])
+def test_env_chain_bit_identical(ctx, d, Dl, Dr, wl, wr):
+    """Small environment updates run as one single-workgroup launch (MPSD_OPT_ENV_CHAIN): the
+    same bits as the three separate launches, and the oracle's result to 1e-12."""
+    from mpsdyn_amd import device as dev
+    print(f"env chain d={d} Dl={Dl} Dr={Dr} wl={wl} wr={wr}", flush=True)
+    L = orc.complex_gaussian(11, (Dl, wl, Dl))
+    R = orc.complex_gaussian(12, (Dr, wr, Dr))
+    W = orc.complex_gaussian(13, (d, wl, d, wr))
+    ket = orc.complex_gaussian(14, (d, Dl, Dr))
+    bra = orc.complex_gaussian(15, (d, Dl, Dr))
+    tL, tR, tk, tb, tW = ctx.tensor(L), ctx.tensor(R), ctx.tensor(ket), ctx.tensor(bra), ctx.tensor(W)
+    got = {}
+    try:
+        for chain in (1, 0):
+            ctx.set_option(dev.OPT_ENV_CHAIN, chain)
+            out = ctx.tensor(dims=(1,))
+            dev.env_left(ctx, tL, tk, tW, tb, out)
+            left = out.download()
+            dev.env_right(ctx, tR, tk, tW, tb, out)
+            got[chain] = (left, out.download())
+    finally:
+        ctx.set_option(dev.OPT_ENV_CHAIN, 1)
+    assert np.array_equal(got[1][0], got[0][0]) and np.array_equal(got[1][1], got[0][1])
+    assert _rel(got[1][0], orc.contract_left(L, ket, W, bra)) <= TOL
+    assert _rel(got[1][1], orc.contract_right(R, ket, W, bra)) <= TOL
