@@ -100,11 +100,18 @@ int64_t mpsd_tensor_size(const mpsd_tensor* t); /* number of complex elements */
 /* Change the logical shape; reallocates only if the new size exceeds the capacity.
  * Replaces mwArray::reshape/resize of TmpManager slots (TmpManager.cpp:111-211). */
 int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims);
+/* Host data of the whole tensor to the device, stream-ordered: the host buffer may be reused
+ * as soon as the call returns (the data goes through a pinned staging buffer of the context;
+ * tensors larger than it are copied synchronously). */
 int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex);
+/* mpsd_tensor_create followed by mpsd_tensor_upload of n = prod(dims) complex numbers, without
+ * the zero fill. */
+int mpsd_tensor_create_from(mpsd_ctx* ctx, int rank, const int* dims, const double* host, mpsd_tensor** out);
 int mpsd_tensor_download(const mpsd_tensor* t, double* host, int64_t n_complex);
 int mpsd_tensor_copy(mpsd_tensor* dst, const mpsd_tensor* src); /* same size, stream-ordered */
 /* Device address of the buffer (complex128 interleaved), for callers that keep vectors
- * on the device (mpsd_heff_apply with on_device = 1). */
+ * on the device (mpsd_heff_apply with on_device = 1).  Valid until a call that changes the
+ * tensor's shape (reshape past its capacity, gauge and reduce-dim steps). */
 void* mpsd_tensor_devptr(mpsd_tensor* t);
 
 /* ---- environment updates ---------------------------------------------------------------

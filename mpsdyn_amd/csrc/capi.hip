@@ -290,6 +290,8 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   if (ctx->coef_host) (void)hipHostFree(ctx->coef_host);
   for (auto& kv : ctx->pool) (void)hipFree(kv.second);
   ctx->pool.clear();
+  if (ctx->up_host) (void)hipHostFree(ctx->up_host);
+  if (ctx->status_host) (void)hipHostFree(ctx->status_host);
   for (hipEvent_t e : ctx->coef_ev)
     if (e) (void)hipEventDestroy(e);
   destroy_blas(ctx);
@@ -476,11 +478,56 @@ int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims) {
 int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex) {
   if (!t || !host) return MPSD_ERR_ARG;
   if (n_complex != t->n) return set_err(t->ctx, MPSD_ERR_SHAPE, "tensor_upload: size mismatch");
-  hipError_t e = hipMemcpyAsync(t->ptr, host, sizeof(double2) * (size_t)t->n, hipMemcpyHostToDevice,
-                                t->ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+  mpsd_ctx* ctx = t->ctx;
+  const size_t bytes = sizeof(double2) * (size_t)t->n;
+  hipError_t e = hipSuccess;
+  if (bytes <= mpsd_ctx::kUpBytes / 4) {
+    // through the pinned staging buffer: no wait for the copy (or anything before it)
+    if (!ctx->up_host) e = hipHostMalloc((void**)&ctx->up_host, mpsd_ctx::kUpBytes, hipHostMallocDefault);
+    if (e == hipSuccess && ctx->up_off + bytes > mpsd_ctx::kUpBytes) {
+      e = hipStreamSynchronize(ctx->stream);  // every staged copy has landed: start over
+      ctx->up_off = 0;
+    }
+    if (e == hipSuccess) {
+      char* stage = ctx->up_host + ctx->up_off;
+      std::memcpy(stage, host, bytes);
+      ctx->up_off += (bytes + 255) & ~size_t(255);
+      e = hipMemcpyAsync(t->ptr, stage, bytes, hipMemcpyHostToDevice, ctx->stream);
+    }
+  } else {
+    e = hipMemcpyAsync(t->ptr, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
   ++t->version;
-  return hip_err(t->ctx, e, "tensor_upload");
+  return hip_err(ctx, e, "tensor_upload");
+}
+
+int mpsd_tensor_create_from(mpsd_ctx* ctx, int rank, const int* dims, const double* host, mpsd_tensor** out) {
+  if (!ctx || !out || !host || rank < 0 || rank > 8 || (rank > 0 && !dims))
+    return set_err(ctx, MPSD_ERR_ARG, "tensor_create_from: bad arguments");
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] <= 0) return set_err(ctx, MPSD_ERR_ARG, "tensor_create_from: non-positive dimension");
+    n *= dims[i];
+  }
+  mpsd_tensor* t = new mpsd_tensor();
+  t->ctx = ctx;
+  t->rank = rank;
+  for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
+  t->n = n;
+  hipError_t e = pool_alloc(ctx, n, &t->ptr, &t->cap);
+  if (e != hipSuccess) {
+    delete t;
+    return hip_err(ctx, e, "tensor_create_from");
+  }
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * t->cap;
+  const int rc = mpsd_tensor_upload(t, host, n);
+  if (rc) {
+    (void)mpsd_tensor_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return MPSD_OK;
 }
 
 int mpsd_tensor_download(const mpsd_tensor* t, double* host, int64_t n_complex) {

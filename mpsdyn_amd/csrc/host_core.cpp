@@ -156,7 +156,19 @@ DevTensor::DevTensor(mpsd_ctx* c, const std::vector<int>& dims) : ctx(c) {
   check(ctx, mpsd_tensor_create(ctx, (int)d.size(), d.data(), &t), "tensor_create");
 }
 
-DevTensor::DevTensor(mpsd_ctx* c, const mwArray& a) : DevTensor(c, a.getDimensions()) { upload(a); }
+DevTensor::DevTensor(mpsd_ctx* c, const mwArray& a) : ctx(c) {
+  const std::vector<int>& d = a.getDimensions();
+  int64_t n = 1;
+  for (int x : d) n *= x;
+  if (d.empty() || n != a.getNrComponents()) {
+    // shapes the upload rejects: the plain create and upload report it
+    const std::vector<int> d1 = d.empty() ? std::vector<int>{1} : d;
+    check(ctx, mpsd_tensor_create(ctx, (int)d1.size(), d1.data(), &t), "tensor_create");
+    upload(a);
+    return;
+  }
+  check(ctx, mpsd_tensor_create_from(ctx, (int)d.size(), d.data(), a.getComponents(), &t), "tensor_create_from");
+}
 
 DevTensor::~DevTensor() {
   if (t) mpsd_tensor_destroy(t);

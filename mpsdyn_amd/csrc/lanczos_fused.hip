@@ -552,8 +552,12 @@ int lanczos_fused(mpsd_heff* h, double* x, double tol, int m, int max_restarts, 
   else
     hipLaunchKernelGGL(lanczos_fused_kernel<false>, dim3(1), dim3(kFT), 0, ctx->stream, a);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(out6, a.out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  // the status block through pinned memory: an asynchronous copy, then one wait
+  if (e == hipSuccess && !ctx->status_host) e = hipHostMalloc((void**)&ctx->status_host, 64 * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(ctx->status_host, a.out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess)
+    for (int i = 0; i < 6; ++i) out6[i] = ctx->status_host[i];
   return hip_err(ctx, e, "lanczos_fused");
 }
 

@@ -692,23 +692,43 @@ int mpsd_site_gauge_l_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, 
   return hip_err(ctx, hipGetLastError(), "gauge_l_svd");
 }
 
+// The site's result goes to a fresh pooled buffer of n elements with shape [d, Dl, Dr], and the
+// old buffer (returned in *old / *oldcap) is read by the launch that fills the new one, then
+// pooled after it is enqueued: reuse is stream-ordered, so no copy of the old contents is needed.
+static int take_new_buffer(mpsd_ctx* ctx, mpsd_tensor* site, int d, int Dl, int Dr, double2** old,
+                           int64_t* oldcap) {
+  const int64_t n = (int64_t)d * Dl * Dr;
+  double2* nb = nullptr;
+  int64_t ncap = 0;
+  hipError_t e = pool_alloc(ctx, n, &nb, &ncap);
+  if (e != hipSuccess) return hip_err(ctx, e, "site buffer");
+  *old = site->ptr;
+  *oldcap = site->cap;
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * (ncap - site->cap);
+  site->ptr = nb;
+  site->cap = ncap;
+  site->rank = 3;
+  for (int i = 0; i < 8; ++i) site->dims[i] = 0;
+  site->dims[0] = d;
+  site->dims[1] = Dl;
+  site->dims[2] = Dr;
+  site->n = n;
+  return MPSD_OK;
+}
+
 int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R) {
   // site[s, i, c] = sum_j R[i, j] site_old[s, j, c]   (Site::reduceDimR, Site.cpp:240-261)
   if (!ctx || !site || !R || site == R) return MPSD_ERR_ARG;
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int Rm = R->dims[0], Rn = R->rank > 1 ? R->dims[1] : 1;
   if (Rn != Dl) return set_err(ctx, MPSD_ERR_SHAPE, "reduce_dim_r: factor does not match the left bond");
-  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)site->n);
-  if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(ctx->scratch[3], site->ptr, sizeof(double2) * site->n, hipMemcpyDeviceToDevice,
-                                ctx->stream);
-  if (e != hipSuccess) return hip_err(ctx, e, "reduce_dim_r copy");
-  const int sd[3] = {d, Rm, Dr};
-  rc = mpsd_tensor_reshape(site, 3, sd);
+  double2* old = nullptr;
+  int64_t oldcap = 0;
+  int rc = take_new_buffer(ctx, site, d, Rm, Dr, &old, &oldcap);
   if (rc) return rc;
   ZgemmDesc g;
   g.A = R->ptr;
-  g.B = (const double2*)ctx->scratch[3];
+  g.B = old;
   g.C = site->ptr;
   g.M = Rm;
   g.N = Dr;
@@ -726,7 +746,9 @@ int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* 
   g.a_mfast = 1;
   g.b_kfast = 1;
   ++site->version;
-  return hip_err(ctx, launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream), "reduce_dim_r");
+  const hipError_t e = launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream);
+  pool_free(ctx, old, oldcap);
+  return hip_err(ctx, e, "reduce_dim_r");
 }
 
 int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L) {
@@ -735,16 +757,12 @@ int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* 
   const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
   const int Lm = L->dims[0], Ln = L->rank > 1 ? L->dims[1] : 1;
   if (Lm != Dr) return set_err(ctx, MPSD_ERR_SHAPE, "reduce_dim_l: factor does not match the right bond");
-  int rc = ensure_scratch(ctx, 3, sizeof(double2) * (size_t)site->n);
-  if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(ctx->scratch[3], site->ptr, sizeof(double2) * site->n, hipMemcpyDeviceToDevice,
-                                ctx->stream);
-  if (e != hipSuccess) return hip_err(ctx, e, "reduce_dim_l copy");
-  const int sd[3] = {d, Dl, Ln};
-  rc = mpsd_tensor_reshape(site, 3, sd);
+  double2* old = nullptr;
+  int64_t oldcap = 0;
+  int rc = take_new_buffer(ctx, site, d, Dl, Ln, &old, &oldcap);
   if (rc) return rc;
   ZgemmDesc g;
-  g.A = (const double2*)ctx->scratch[3];
+  g.A = old;
   g.B = L->ptr;
   g.C = site->ptr;
   g.M = d * Dl;
@@ -760,7 +778,9 @@ int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* 
   g.a_mfast = 1;
   g.b_kfast = 1;
   ++site->version;
-  return hip_err(ctx, launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream), "reduce_dim_l");
+  const hipError_t e = launch_zbrgemm_f64(g, TileCfg{32, 32}, ctx->stream);
+  pool_free(ctx, old, oldcap);
+  return hip_err(ctx, e, "reduce_dim_l");
 }
 
 }  // extern "C"

@@ -168,6 +168,13 @@ struct mpsd_ctx {
   // so a pooled buffer is never still in use when it is handed out again.
   std::multimap<int64_t, double2*> pool;
   int64_t pool_bytes = 0;
+  // Pinned staging of host uploads: data is copied in at up_off and DMA'd on the stream; the
+  // stream is drained only when the buffer wraps.  status_host: pinned landing place of the
+  // one-workgroup eigensolver's status block.
+  char* up_host = nullptr;
+  size_t up_off = 0;
+  static constexpr size_t kUpBytes = size_t(8) << 20;
+  double* status_host = nullptr;
   mpsd::TileCfg tile_override{0, 0, 16};
   bool tile_user = false;  // tile_override set away from the built-in choice
   // Autotuner: fastest configuration per BRGEMM shape class (mpsd_tune_*), and, while a
