@@ -107,6 +107,9 @@ int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex);
 /* mpsd_tensor_create followed by mpsd_tensor_upload of n = prod(dims) complex numbers, without
  * the zero fill. */
 int mpsd_tensor_create_from(mpsd_ctx* ctx, int rank, const int* dims, const double* host, mpsd_tensor** out);
+/* mpsd_tensor_create without the zero fill: the contents are undefined until written (for
+ * slots that a contraction or copy overwrites, like the environment cache's). */
+int mpsd_tensor_create_empty(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** out);
 int mpsd_tensor_download(const mpsd_tensor* t, double* host, int64_t n_complex);
 int mpsd_tensor_copy(mpsd_tensor* dst, const mpsd_tensor* src); /* same size, stream-ordered */
 /* Device address of the buffer (complex128 interleaved), for callers that keep vectors

@@ -83,6 +83,8 @@ struct DevTensor {
   mpsd_ctx* ctx = nullptr;
   DevTensor() {}
   DevTensor(mpsd_ctx* c, const std::vector<int>& dims);
+  // zero = false: contents undefined until written (mpsd_tensor_create_empty)
+  DevTensor(mpsd_ctx* c, const std::vector<int>& dims, bool zero);
   DevTensor(mpsd_ctx* c, const mwArray& a);
   ~DevTensor();
   DevTensor(const DevTensor&) = delete;

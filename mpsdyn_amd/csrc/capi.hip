@@ -502,6 +502,29 @@ int mpsd_tensor_upload(mpsd_tensor* t, const double* host, int64_t n_complex) {
   return hip_err(ctx, e, "tensor_upload");
 }
 
+int mpsd_tensor_create_empty(mpsd_ctx* ctx, int rank, const int* dims, mpsd_tensor** out) {
+  if (!ctx || !out || rank < 0 || rank > 8 || (rank > 0 && !dims))
+    return set_err(ctx, MPSD_ERR_ARG, "tensor_create_empty: bad arguments");
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] <= 0) return set_err(ctx, MPSD_ERR_ARG, "tensor_create_empty: non-positive dimension");
+    n *= dims[i];
+  }
+  mpsd_tensor* t = new mpsd_tensor();
+  t->ctx = ctx;
+  t->rank = rank;
+  for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
+  t->n = n;
+  hipError_t e = pool_alloc(ctx, n, &t->ptr, &t->cap);
+  if (e != hipSuccess) {
+    delete t;
+    return hip_err(ctx, e, "tensor_create_empty");
+  }
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * t->cap;
+  *out = t;
+  return MPSD_OK;
+}
+
 int mpsd_tensor_create_from(mpsd_ctx* ctx, int rank, const int* dims, const double* host, mpsd_tensor** out) {
   if (!ctx || !out || !host || rank < 0 || rank > 8 || (rank > 0 && !dims))
     return set_err(ctx, MPSD_ERR_ARG, "tensor_create_from: bad arguments");

@@ -156,6 +156,14 @@ DevTensor::DevTensor(mpsd_ctx* c, const std::vector<int>& dims) : ctx(c) {
   check(ctx, mpsd_tensor_create(ctx, (int)d.size(), d.data(), &t), "tensor_create");
 }
 
+DevTensor::DevTensor(mpsd_ctx* c, const std::vector<int>& dims, bool zero) : ctx(c) {
+  std::vector<int> d = dims.empty() ? std::vector<int>{1} : dims;
+  if (zero)
+    check(ctx, mpsd_tensor_create(ctx, (int)d.size(), d.data(), &t), "tensor_create");
+  else
+    check(ctx, mpsd_tensor_create_empty(ctx, (int)d.size(), d.data(), &t), "tensor_create_empty");
+}
+
 DevTensor::DevTensor(mpsd_ctx* c, const mwArray& a) : ctx(c) {
   const std::vector<int>& d = a.getDimensions();
   int64_t n = 1;
@@ -465,9 +473,10 @@ void MPS::importMPS(const char* filename) {
 // ------------------------------------------------------------------------------ TmpManager
 TmpManager::TmpManager(int length, mpsd_ctx* ctx) : upL_(length, false), upR_(length, false) {
   const complex_t one(1.0, 0.0);
+  // every slot but the two edges is written by an environment update before it is read
   for (int k = 0; k < length; ++k) {
-    L_.emplace_back(ctx, std::vector<int>{1, 1, 1});
-    R_.emplace_back(ctx, std::vector<int>{1, 1, 1});
+    L_.emplace_back(ctx, std::vector<int>{1, 1, 1}, false);
+    R_.emplace_back(ctx, std::vector<int>{1, 1, 1}, false);
   }
   mwArray edge(std::vector<int>{1, 1, 1}, &one);
   L_[0].upload(edge);
