@@ -459,8 +459,9 @@ int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims) {
     n *= dims[i];
   }
   if (n > t->cap) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_err(ctx, e, "reshape sync");
+    // the old buffer back to the pool and a larger one from it: both reuses are ordered on the
+    // context's stream behind every launch that still reads the old one, so no wait
+    hipError_t e = hipSuccess;
     pool_free(ctx, t->ptr, t->cap);
     ctx->tensor_bytes -= (int64_t)sizeof(double2) * t->cap;
     t->ptr = nullptr;
