@@ -247,6 +247,14 @@ int mpsd_lanczos_coef(mpsd_ctx* ctx, const double* h1_dev, const double* h2_dev,
  * ab_dev[2j], ab_dev[2j+1]. */
 int mpsd_lanczos_coef_update(mpsd_ctx* ctx, const double* V, int64_t ldv, const double* h1_dev,
                              const double* h2_dev, int j, double* ab_dev, double* w, int64_t n);
+/* The two Gram-Schmidt passes of a stream-launched Lanczos step, each in two launches when the
+ * vectors are short enough (n <= 32768): gs1 = mpsd_vec_dots_dev(V, m, w, h) then
+ * mpsd_vec_gemv_dev(V, m, h, -1, w); gs2 = mpsd_vec_dots_dev(V, j + 2, w, h2) then
+ * mpsd_lanczos_coef_update(V, h1, h2, j, ab, w).  The dots' last reduction runs inside the
+ * consuming kernel, with the same bits. */
+int mpsd_lanczos_gs1_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, double* w, int64_t n, double* h_dev);
+int mpsd_lanczos_gs2_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, const double* h1_dev, double* h2_dev,
+                         double* ab_dev, double* w, int64_t n);
 /* Projector terms of TensorMultiplierProjMulti::product (src/wrapper/multipliers/
  * TensorMultiplierProjMulti.cpp:87-96): y -= sum_j weights[j] <phi_j|x> phi_j, phi_j at
  * Phi + 2*j*ldphi, j < m <= 32; weights is a host array of m reals; x and y distinct device

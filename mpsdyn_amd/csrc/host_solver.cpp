@@ -297,10 +297,8 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
         double* w = vec(jj + 1);
         H.product_device(vec(jj), w);
         ++st.matvecs;
-        check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 1, w, n, h1), "dots");
-        check(ctx_, mpsd_vec_gemv_dev(ctx_, V, ldv, jj + 1, h1, -1.0, w, n), "gemv");
-        check(ctx_, mpsd_vec_dots_dev(ctx_, V, ldv, jj + 2, w, n, h2), "dots2");
-        check(ctx_, mpsd_lanczos_coef_update(ctx_, V, ldv, h1, h2, jj, ab + 2, w, n), "gemv2 + coef");
+        check(ctx_, mpsd_lanczos_gs1_dev(ctx_, V, ldv, jj + 1, w, n, h1), "Gram-Schmidt pass 1");
+        check(ctx_, mpsd_lanczos_gs2_dev(ctx_, V, ldv, jj, h1, h2, ab + 2, w, n), "Gram-Schmidt pass 2 + coef");
       }
       check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef download");
       if (j == 0) {

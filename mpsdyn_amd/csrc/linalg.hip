@@ -88,6 +88,46 @@ __global__ void __launch_bounds__(kT) coef_update_kernel(const double2* __restri
   }
 }
 
+// The two Gram-Schmidt passes of a Lanczos step with the dots' last reduction done by the
+// consumer (nb <= kDotFoldMax partial sums per dot): every block reduces the partials with
+// dot_final_one, which returns dot_final's bits, so the step is the same arithmetic as the
+// separate dots_final launches, one launch shorter per pass.  Block 0 stores the dots.
+constexpr int kFoldM = 1024;
+__global__ void __launch_bounds__(kT) gemv_fold_kernel(const double2* __restrict__ V, int64_t ldv, int m,
+                                                       const double2* __restrict__ partial, int nb, double sign,
+                                                       double2* __restrict__ w, int64_t n, double2* __restrict__ h) {
+  __shared__ double2 c[kFoldM];
+  for (int i = threadIdx.x; i < m; i += kT) c[i] = dot_final_one(partial + (int64_t)i * kDotBlocks, nb);
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < m; i += kT) h[i] = c[i];
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t < n) gemv_elem(V, ldv, m, c, sign, w, t);
+}
+
+__global__ void __launch_bounds__(kT) coef_fold_kernel(const double2* __restrict__ V, int64_t ldv,
+                                                       const double2* __restrict__ h1,
+                                                       const double2* __restrict__ partial, int nb, int j,
+                                                       double* __restrict__ ab, double2* __restrict__ w,
+                                                       int64_t n, double2* __restrict__ h2) {
+  __shared__ double2 c[kFoldM];
+  for (int i = threadIdx.x; i < j + 2; i += kT) c[i] = dot_final_one(partial + (int64_t)i * kDotBlocks, nb);
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < j + 2; i += kT) h2[i] = c[i];
+  double alpha, beta, inv;
+  lanczos_scalars(h1, c, j, alpha, beta, inv);
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < n; t += (int64_t)gridDim.x * kT) {
+    gemv_elem(V, ldv, j + 1, c, -1.0, w, t);
+    const double2 v = w[t];
+    w[t] = double2{v.x * inv, v.y * inv};
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ab[2 * j] = alpha;
+    ab[2 * j + 1] = beta;
+  }
+}
+
 // Lanczos coefficients of step j from the two Gram-Schmidt passes (h1 = V^H w before the
 // first pass, h2 = [V_0..V_j, w]^H w after it): alpha_j = Re(h1[j] + h2[j]),
 // beta_j = sqrt(|w|^2 - sum_i |h2[i]|^2) (the norm after the second pass), then
@@ -417,6 +457,43 @@ int mpsd_lanczos_coef_update(mpsd_ctx* ctx, const double* V, int64_t ldv, const 
   hipLaunchKernelGGL(coef_update_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
                      (const double2*)h1_dev, (const double2*)h2_dev, j, ab_dev, (double2*)w, n);
   return hip_err(ctx, hipGetLastError(), "lanczos_coef_update");
+}
+
+int mpsd_lanczos_gs1_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, double* w, int64_t n, double* h_dev) {
+  if (!ctx || !V || !w || !h_dev || m < 1 || n < 0 || (m > 1 && ldv < n)) return MPSD_ERR_ARG;
+  const int nb = dot_blocks(n);
+  if (nb > kDotFoldMax || m > kFoldM || n == 0) {
+    int rc = mpsd_vec_dots_dev(ctx, V, ldv, m, w, n, h_dev);
+    return rc ? rc : mpsd_vec_gemv_dev(ctx, V, ldv, m, h_dev, -1.0, w, n);
+  }
+  int rc = ensure_dots(ctx, m);
+  if (rc) return rc;
+  double2* partial = (double2*)ctx->dots;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, m), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)w, n, nb, partial);
+  hipLaunchKernelGGL(gemv_fold_kernel, dim3(nblk(n)), dim3(kT), 0, ctx->stream, (const double2*)V, ldv, m,
+                     (const double2*)partial, nb, -1.0, (double2*)w, n, (double2*)h_dev);
+  return hip_err(ctx, hipGetLastError(), "lanczos_gs1");
+}
+
+int mpsd_lanczos_gs2_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, const double* h1_dev, double* h2_dev,
+                         double* ab_dev, double* w, int64_t n) {
+  if (!ctx || !V || !w || !h1_dev || !h2_dev || !ab_dev || j < 0 || n < 0 || ldv < n) return MPSD_ERR_ARG;
+  const int nb = dot_blocks(n);
+  if (nb > kDotFoldMax || j + 2 > kFoldM || n == 0) {
+    int rc = mpsd_vec_dots_dev(ctx, V, ldv, j + 2, w, n, h2_dev);
+    return rc ? rc : mpsd_lanczos_coef_update(ctx, V, ldv, h1_dev, h2_dev, j, ab_dev, w, n);
+  }
+  int rc = ensure_dots(ctx, j + 2);
+  if (rc) return rc;
+  double2* partial = (double2*)ctx->dots;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb, j + 2), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)w, n, nb, partial);
+  const unsigned blocks = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kT - 1) / kT), 1024);
+  hipLaunchKernelGGL(coef_fold_kernel, dim3(blocks), dim3(kT), 0, ctx->stream, (const double2*)V, ldv,
+                     (const double2*)h1_dev, (const double2*)partial, nb, j, ab_dev, (double2*)w, n,
+                     (double2*)h2_dev);
+  return hip_err(ctx, hipGetLastError(), "lanczos_gs2");
 }
 
 int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n, double* nrm2_dev) {

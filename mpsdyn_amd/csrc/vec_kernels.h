@@ -41,6 +41,36 @@ __device__ inline double2 dot_block(const double2* __restrict__ v, const double2
   return red[0];
 }
 
+// dot_final's result computed by one thread, for nb <= kDotFoldMax: the same halving tree over
+// kDotMaxBlocks slots.  Slots at and beyond nb hold zeros, so the levels wider than P (the
+// next power of two >= nb) only add +0.0 to each live slot, which is done once (it is
+// idempotent), and the levels below P are the tree itself.  Bit for bit dot_final's value.
+constexpr int kDotFoldMax = 16;
+__device__ inline double2 dot_final_one(const double2* __restrict__ partial, int nb) {
+  double2 r[kDotFoldMax];
+  int P = 1;
+  while (P < nb) P <<= 1;
+#pragma unroll
+  for (int i = 0; i < kDotFoldMax; ++i) r[i] = i < nb ? partial[i] : double2{0.0, 0.0};
+  if (P < kDotMaxBlocks) {
+#pragma unroll
+    for (int i = 0; i < kDotFoldMax; ++i) {
+      r[i].x += 0.0;
+      r[i].y += 0.0;
+    }
+  }
+#pragma unroll
+  for (int s = kDotFoldMax / 2; s > 0; s >>= 1)
+    if (s < P) {
+#pragma unroll
+      for (int i = 0; i < s; ++i) {
+        r[i].x += r[i + s].x;
+        r[i].y += r[i + s].y;
+      }
+    }
+  return r[0];
+}
+
 // Sum of the nb partials of one dot (launched with kDotMaxBlocks threads); valid in thread 0.
 __device__ inline double2 dot_final(const double2* __restrict__ partial, int nb) {
   __shared__ double2 red[kDotMaxBlocks];

@@ -105,6 +105,11 @@ def _optional(lib):
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),
         "mpsd_mpo_save": (I, [P, C.POINTER(GsConfig), C.c_char_p, I]),
         "mpsd_vec_proj_update": (I, [P, P, I64, I, C.POINTER(D), P, P, I64]),
+        "mpsd_vec_dots_dev": (I, [P, P, I64, I, P, I64, P]),
+        "mpsd_vec_gemv_dev": (I, [P, P, I64, I, P, D, P, I64]),
+        "mpsd_lanczos_coef_update": (I, [P, P, I64, P, P, I, P, P, I64]),
+        "mpsd_lanczos_gs1_dev": (I, [P, P, I64, I, P, I64, P]),
+        "mpsd_lanczos_gs2_dev": (I, [P, P, I64, I, P, P, P, P, I64]),
         "mpsd_site_gauge_r_svd": (I, [P, P, P, I]),
         "mpsd_site_gauge_l_svd": (I, [P, P, P, I]),
         "mpsd_mps_truncate": (I, [P, C.c_char_p, I, C.c_char_p]),

@@ -54,6 +54,38 @@ def test_dots_are_bit_reproducible(ctx):
     assert all(np.array_equal(outs[0], o) for o in outs)
 
 
+@pytest.mark.parametrize("n,j", [(1, 0), (300, 2), (2049, 5), (32768, 9), (8000, 40), (32769, 3), (70000, 12)])
+def test_gram_schmidt_passes_bit_identical(ctx, n, j):
+    """The two Gram-Schmidt passes of a Lanczos step in their folded form (the dots' last
+    reduction inside the consumer, n <= 32768) leave the same bits as dots + gemv and
+    dots + coef_update; w is column j+1 of V, as in the solver.  Above n = 32768 the entries
+    take the unfolded launches."""
+    print(f"gram-schmidt n={n} j={j}", flush=True)
+    V = orc.complex_gaussian(7 + j, (n, j + 2))
+    V[:, : j + 1] /= np.sqrt(n)
+    runs = []
+    for folded in (False, True):
+        tV = ctx.tensor(V)
+        h1, h2, ab = (ctx.tensor(np.zeros(k, dtype=np.complex128)) for k in (j + 1, j + 2, j + 2))
+        base = tV.devptr()
+        w = base + (j + 1) * n * 16
+        if folded:
+            ctx.check(ctx.lib.mpsd_lanczos_gs1_dev(ctx.handle, base, n, j + 1, w, n, h1.devptr()))
+            ctx.check(ctx.lib.mpsd_lanczos_gs2_dev(ctx.handle, base, n, j, h1.devptr(), h2.devptr(), ab.devptr(),
+                                                   w, n))
+        else:
+            ctx.check(ctx.lib.mpsd_vec_dots_dev(ctx.handle, base, n, j + 1, w, n, h1.devptr()))
+            ctx.check(ctx.lib.mpsd_vec_gemv_dev(ctx.handle, base, n, j + 1, h1.devptr(), -1.0, w, n))
+            ctx.check(ctx.lib.mpsd_vec_dots_dev(ctx.handle, base, n, j + 2, w, n, h2.devptr()))
+            ctx.check(ctx.lib.mpsd_lanczos_coef_update(ctx.handle, base, n, h1.devptr(), h2.devptr(), j,
+                                                       ab.devptr(), w, n))
+        runs.append([t.download() for t in (tV, h1, h2, ab)])
+    for a, b in zip(*runs):
+        assert np.array_equal(np.ravel(a, order="F").view(np.float64), np.ravel(b, order="F").view(np.float64))
+    h1d = runs[1][1]
+    assert np.allclose(h1d, V[:, : j + 1].conj().T @ V[:, j + 1], rtol=1e-12, atol=1e-12 * np.sqrt(n))
+
+
 @pytest.mark.parametrize("d,Dl,Dr", [(2, 8, 16), (2, 16, 8), (2, 1, 2), (2, 64, 64), (2, 4, 1), (3, 5, 7)])
 def test_gauge_r_and_reduce(ctx, d, Dl, Dr):
     A = orc.complex_gaussian(7 + Dl, (d, Dl, Dr))
