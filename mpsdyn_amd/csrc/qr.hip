@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
 
 // ---- whole QR of a small matrix in one workgroup ----------------------------------------------
 // A (m x n, m <= 64, n <= 64) is factorised by one workgroup in the zlarfg / zung2r conventions
-// of the panel path, Q (m x k) is formed backwards, and Q and R are written out: one launch where
+// of the panel path, Q (m x k) is formed, and Q and R are written out: one launch where
 // the panel path takes a dozen for a gauge step of a small site.
 //   Layout: lane u of every wave holds column u, wave w the rows [w RPT, (w + 1) RPT), in
 // registers, nw = ceil(m / RPT) <= 16 waves; the matrix is mirrored in LDS (X, leading dimension
@@ -301,9 +301,12 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
 // on it, so the diagonal row takes x_c = alpha - beta and needs no branch) is never formed: a
 // wave's partial of v^H a_u is conj(scal) sum conj(x_r) a_r, and the update is a_r -= x_r
 // (scal g).  Every lane then writes its updated column and its partial squared norm below the
-// next diagonal back to LDS.  Q is formed backwards from the columns left in X with the stored
-// scal and alpha - beta.  A is read whole (through LDS, coalesced) before anything is written,
-// so Q may be A itself.
+// next diagonal back to LDS.  When n + m <= 64 the spare lanes n .. n + m - 1 start as the
+// columns of I_m and take every reflection with the matrix (they are never a reflector column,
+// so they stay in registers): they end as Q_full^H and Q is read off them, in the k steps of the
+// factorisation.  Otherwise Q is formed backwards from the columns left in X with the stored
+// scal and alpha - beta, k more steps.  A is read whole (through LDS, coalesced) before anything
+// is written, so Q may be A itself.
 //   With lqd = d > 0 the kernel does the LQ of a site instead (Site::gaugeL): A is the site
 // tensor [d, n, m / d] and the matrix factorised is B[s + d c, a] = conj(site[s, a, c])
 // (m x n); Rg receives L = R^H (n x k) and Qg the new site [d, k, m / d] = Q^H reshaped.
@@ -424,6 +427,9 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     (&nrm_s[0][0])[e] = 0.0;
   }
   __syncthreads();
+  // With n + m <= 64 the lanes n .. n + m - 1 hold the columns of I_m and take every reflection
+  // too, so they end as Q_full^H = H_{k-1}^H ... H_0^H and Q needs no backward pass.
+  const bool fq = n + m <= 64;
   double2 a[RPT], xr[RPT];
   {
     double s2 = 0.0;
@@ -432,6 +438,7 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       const int r = r0 + i;
       a[i] = (u < n && r < m) ? X[r + ld * u] : double2{0.0, 0.0};
       if (r > 0) s2 = fma(a[i].x, a[i].x, fma(a[i].y, a[i].y, s2));
+      if (fq && u >= n && r < m && r == u - n) a[i] = double2{1.0, 0.0};
     }
     nrm_s[w][u] = s2;
   }
@@ -470,7 +477,7 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       xr[i] = r > c && r < m ? x : (r == c ? f.dd : double2{0.0, 0.0});
     }
     // this wave's partial of v^H a_u
-    const bool mine = u > c && u < n;
+    const bool mine = (u > c && u < n) || (fq && u >= n && u < n + m);
     {
       double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
 #pragma unroll
@@ -486,10 +493,10 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       for (int i = 0; i < RPT; ++i) {
         const int r = r0 + i;
         sub_mul(a[i], xr[i], g2);
-        if (r < m) X[r + ld * u] = a[i];
+        if (u < n && r < m) X[r + ld * u] = a[i];
         if (r > c + 1) s2 = fma(a[i].x, a[i].x, fma(a[i].y, a[i].y, s2));
       }
-      nrm_s[w][u] = s2;
+      if (u < n) nrm_s[w][u] = s2;
     }
   }
   __syncthreads();
@@ -498,6 +505,22 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     const int i = lqd > 0 ? e / n : e % k, j = lqd > 0 ? e % n : e / k;
     const double2 r = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
     Rg[e] = lqd > 0 ? double2{r.x, -r.y} : r;
+  }
+  if (fq) {
+    // Q[q, j] = conj(Q^H[j, q]): row j of lane n + q, for j < k
+    if (u >= n && u < n + m) {
+      const int q = u - n;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int j = r0 + i;
+        if (j >= k) continue;
+        if (lqd > 0)
+          Qg[q % lqd + lqd * (j + k * (q / lqd))] = a[i];  // new site[s, j, c] = conj(Q[s + d c, j])
+        else
+          Qg[q + m * j] = double2{a[i].x, -a[i].y};
+      }
+    }
+    return;
   }
   // Q = H_0 ... H_{k-1} [I; 0], accumulated backwards: Q <- Q - tau v (v^H Q), columns u >= c
   double2 qv[RPT];

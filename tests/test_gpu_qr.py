@@ -45,11 +45,13 @@ def _rel(a, b):
 
 
 # (d, Dl, Dr): the single-workgroup form (m = d Dl <= 64, n = Dr <= 64: single rows and columns,
-# one, two and four rows per thread, wide and the largest cases), panel edges (16), one- to
+# one, two and four rows per thread, wide and the largest cases; with m + n <= 64 Q is accumulated
+# in the spare lanes, up to the boundary m + n = 64), panel edges (16), one- to
 # four-row-per-thread panels (m up to 1024), wide factors
 SHAPES = [(2, 1, 1), (2, 1, 2), (2, 8, 1), (2, 8, 16), (2, 20, 20), (2, 32, 32), (2, 32, 64), (2, 16, 64),
           (3, 20, 20), (2, 33, 17), (2, 50, 64), (2, 64, 64), (2, 65, 64), (2, 64, 65), (2, 128, 16),
-          (2, 128, 128), (2, 200, 150), (2, 200, 4), (2, 256, 256), (2, 300, 512), (2, 512, 512), (3, 10, 40)]
+          (2, 128, 128), (2, 200, 150), (2, 200, 4), (2, 256, 256), (2, 300, 512), (2, 512, 512), (3, 10, 40),
+          (3, 12, 20), (2, 30, 4), (2, 16, 33)]
 
 
 @pytest.mark.parametrize("d,Dl,Dr", SHAPES)
@@ -70,7 +72,7 @@ def test_gauge_r_householder(ctx, d, Dl, Dr):
 
 
 @pytest.mark.parametrize("d,Dl,Dr", [(2, 1, 1), (2, 16, 8), (2, 40, 20), (2, 64, 64), (2, 64, 32), (2, 256, 200),
-                                     (2, 512, 512), (3, 40, 10)])
+                                     (2, 512, 512), (3, 40, 10), (2, 20, 20), (3, 8, 12), (2, 32, 16)])
 def test_gauge_l_householder(ctx, d, Dl, Dr):
     print(f"gauge_l d={d} Dl={Dl} Dr={Dr}", flush=True)
     A = orc.complex_gaussian(200 + Dl + Dr, (d, Dl, Dr))
@@ -95,6 +97,24 @@ def test_rank_deficient_and_zero_columns(ctx, d, Dl, Dr):
     Am[:, 17] = 0.0
     Am[:, 30] = 2.0 * Am[:, 3] - 1j * Am[:, 9]
     Am[:, 40] = 1e-200 * Am[:, 2]
+    A = orc.reshape_f(Am, (d, Dl, Dr))
+    Q, R = _gauge_r(ctx, A, 0)
+    k = min(d * Dl, Dr)
+    Qm = orc.reshape_f(Q, (d * Dl, k))
+    assert np.all(np.isfinite(Qm)) and np.all(np.isfinite(R))
+    assert np.linalg.norm(Qm.conj().T @ Qm - np.eye(k)) <= 1e-12
+    assert _rel(Qm @ R, Am) <= 1e-13 * np.sqrt(d * Dl)
+
+
+def test_rank_deficient_fused_q(ctx):
+    """The same degenerate columns where Q is accumulated in the spare lanes (m + n <= 64)."""
+    d, Dl, Dr = 2, 12, 30
+    A = orc.complex_gaussian(8, (d, Dl, Dr))
+    Am = orc.reshape_f(A, (d * Dl, Dr)).copy()
+    Am[:, 4] = 0.0
+    Am[:, 11] = 0.0
+    Am[:, 20] = 2.0 * Am[:, 3] - 1j * Am[:, 9]
+    Am[:, 25] = 1e-200 * Am[:, 2]
     A = orc.reshape_f(Am, (d, Dl, Dr))
     Q, R = _gauge_r(ctx, A, 0)
     k = min(d * Dl, Dr)
