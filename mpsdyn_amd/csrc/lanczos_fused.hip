@@ -27,6 +27,7 @@
 #include <cfloat>
 
 #include "mpsd_internal.h"
+#include "wave_xor.h"
 
 namespace mpsd {
 namespace {
@@ -72,21 +73,9 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {  // c
 __device__ __forceinline__ double2 cjfma(double2 a, double2 b, double2 c) {  // c + conj(a) b
   return double2{fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y))};
 }
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_x(v); }
+__device__ __forceinline__ double wave_min(double v) { return wave_min_x(v); }
+__device__ __forceinline__ double wave_max(double v) { return wave_max_x(v); }
 
 typedef double fd4 __attribute__((ext_vector_type(4)));
 

@@ -17,6 +17,7 @@
 // column, which is what the small and medium gauge steps of a sweep are bound by.
 
 #include "mpsd_internal.h"
+#include "wave_xor.h"
 
 namespace mpsd {
 namespace {
@@ -36,12 +37,7 @@ __device__ __forceinline__ void block_sum(const double2 (&v)[NV], double2 (*red)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int u = 0; u < NV; ++u) {
-    double x = v[u].x, y = v[u].y;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      x += __shfl_xor(x, o);
-      y += __shfl_xor(y, o);
-    }
+    const double x = wave_sum_x(v[u].x), y = wave_sum_x(v[u].y);
     if (lane == 0) red[wave][u] = double2{x, y};
   }
   __syncthreads();
@@ -61,7 +57,8 @@ __device__ __forceinline__ void block_sum(const double2 (&v)[NV], double2 (*red)
 // threads.  Within a wave the values are reduced and scattered at once over lane bits 5..2
 // (8 + 4 + 2 + 1 exchanges instead of 16 x 4), then over bits 1..0; lane 4u ends with the
 // wave's sum of value u.
-__device__ __forceinline__ double2 shfl2(double2 v, int o) { return double2{__shfl_xor(v.x, o), __shfl_xor(v.y, o)}; }
+template <int O>
+__device__ __forceinline__ double2 shfl2(double2 v) { return xor_lane<O>(v); }
 __device__ __forceinline__ double2 add2(double2 a, double2 b) { return double2{a.x + b.x, a.y + b.y}; }
 
 template <int NW>
@@ -70,14 +67,14 @@ __device__ __forceinline__ void block_sum16(const double2 (&v)[kPB], double2 (*r
   const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
   double2 w8[8], w4[4], w2[2];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) w8[u] = add2(b5 ? v[u + 8] : v[u], shfl2(b5 ? v[u] : v[u + 8], 32));
+  for (int u = 0; u < 8; ++u) w8[u] = add2(b5 ? v[u + 8] : v[u], shfl2<32>(b5 ? v[u] : v[u + 8]));
 #pragma unroll
-  for (int u = 0; u < 4; ++u) w4[u] = add2(b4 ? w8[u + 4] : w8[u], shfl2(b4 ? w8[u] : w8[u + 4], 16));
+  for (int u = 0; u < 4; ++u) w4[u] = add2(b4 ? w8[u + 4] : w8[u], shfl2<16>(b4 ? w8[u] : w8[u + 4]));
 #pragma unroll
-  for (int u = 0; u < 2; ++u) w2[u] = add2(b3 ? w4[u + 2] : w4[u], shfl2(b3 ? w4[u] : w4[u + 2], 8));
-  double2 w1 = add2(b2 ? w2[1] : w2[0], shfl2(b2 ? w2[0] : w2[1], 4));
-  w1 = add2(w1, shfl2(w1, 2));
-  w1 = add2(w1, shfl2(w1, 1));
+  for (int u = 0; u < 2; ++u) w2[u] = add2(b3 ? w4[u + 2] : w4[u], shfl2<8>(b3 ? w4[u] : w4[u + 2]));
+  double2 w1 = add2(b2 ? w2[1] : w2[0], shfl2<4>(b2 ? w2[0] : w2[1]));
+  w1 = add2(w1, shfl2<2>(w1));
+  w1 = add2(w1, shfl2<1>(w1));
   // this lane now holds value u = 8 b5 + 4 b4 + 2 b3 + b2
   if ((lane & 3) == 0) red[wave][lane >> 2] = w1;
   __syncthreads();
@@ -94,8 +91,7 @@ __device__ __forceinline__ void block_sum16(const double2 (&v)[kPB], double2 (*r
 template <int NW>
 __device__ __forceinline__ double block_max(double v, double* red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  v = wave_max_x(v);
   if (lane == 0) red[wave] = v;
   __syncthreads();
   double r = red[0];
