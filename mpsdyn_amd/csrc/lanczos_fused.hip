@@ -36,6 +36,7 @@ constexpr int kFT = 1024;
 constexpr int kFW = kFT / 64;
 constexpr int kFM = 64;       // largest Krylov basis (m + 1)
 constexpr int kMS = 256;      // points per multisection round
+constexpr double kGeoLog2 = 18.0 * 3.321928094887362 / kMS;  // log2(1/g), g^kMS = 1e-18
 constexpr int kPool = 9472;   // complex elements of LDS for operands and vectors (148 KiB)
 
 // LDS footprint of a problem, in complex elements: the operands with the current and the new
@@ -317,6 +318,8 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
   __shared__ int cnt[kFT];
   __shared__ double lo_s, hi_s, inv_s, theta_s, res_s;
   __shared__ int stop_s, status_s;
+  __shared__ double warm_hi_s;     // upper end of the last bracket of this cycle
+  __shared__ int warm_s, geo_s;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int64_t n = (int64_t)a.d * a.Dk * a.Dkr;
@@ -366,7 +369,10 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
       a.V[t] = v;
       X[t] = v;
     }
-    if (tid == 0) stop_s = 0;
+    if (tid == 0) {
+      stop_s = 0;
+      warm_s = 0;
+    }
     __syncthreads();
     int K = 0;
     for (int j = 0; j < m; ++j) {
@@ -400,8 +406,12 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
         if (lane == 0) {
           inv_s = beta > 1e-300 ? 1.0 / beta : 0.0;
           if (!isfinite(alpha) || !isfinite(beta)) status_s = 3;
+          // Within a cycle T_j grows by rows, so by interlacing its lowest eigenvalue lies at or
+          // below the one bracketed at the last check: that bracket's upper end caps the search.
+          const bool g = warm_s && warm_hi_s > lo && warm_hi_s < hi;
           lo_s = lo;
-          hi_s = hi;
+          hi_s = g ? warm_hi_s : hi;
+          geo_s = g;
         }
       }
       __syncthreads();
@@ -418,21 +428,32 @@ __global__ void __launch_bounds__(kFT) lanczos_fused_kernel(FusedArgs a) {
       // the tridiagonal problem is solved every third step and at the end of the basis
       if ((j + 1) % 3 != 0 && j + 1 < m && (int64_t)(j + 1) < n) continue;
       // lowest eigenvalue of T_j by multisection: kMS threads count at their own points
+      // With a capped bracket the first round's points close in on the cap geometrically,
+      // hi - (hi - lo) g^k with g^kMS = 1e-18, so an eigenvalue that moved little since the
+      // last check is bracketed to a small fraction of that move in one round.
       for (int round = 0; round < 10; ++round) {
         const double lo = lo_s, hi = hi_s;
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi))) break;
+        const bool geo = round == 0 && geo_s;
         const double step = (hi - lo) / (kMS + 1);
-        const double xk = lo + step * (tid + 1);
+        auto pt = [&](int k) {
+          return geo ? (k == 0 ? lo : hi - (hi - lo) * exp2(-kGeoLog2 * k)) : lo + step * k;
+        };
+        const double xk = pt(tid + 1);
         if (tid < kMS) cnt[tid] = (xk > lo && xk < hi) ? (sturm(al, be2, j + 1, xk) >= 1) : (xk >= hi ? 1 : 0);
         __syncthreads();
         // the first point whose count reaches one bounds the eigenvalue from above
         if (tid < kMS && cnt[tid] && (tid == 0 || !cnt[tid - 1])) {
-          lo_s = tid == 0 ? lo : lo + step * tid;
+          lo_s = pt(tid);
           hi_s = xk;
         }
         __syncthreads();
-        if (tid == 0 && !cnt[kMS - 1]) lo_s = lo + step * kMS;  // above the last point
+        if (tid == 0 && !cnt[kMS - 1]) lo_s = pt(kMS);  // above the last point
         __syncthreads();
+      }
+      if (tid == 0) {
+        warm_hi_s = hi_s;
+        warm_s = 1;
       }
       if (tid < 64) {
         // the eigenvector by inverse iteration on wave 0, row i of T_j in lane i
