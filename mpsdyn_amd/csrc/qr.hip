@@ -297,12 +297,12 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
 // on it, so the diagonal row takes x_c = alpha - beta and needs no branch) is never formed: a
 // wave's partial of v^H a_u is conj(scal) sum conj(x_r) a_r, and the update is a_r -= x_r
 // (scal g).  Every lane then writes its updated column and its partial squared norm below the
-// next diagonal back to LDS.  When n + m <= 64 the spare lanes n .. n + m - 1 start as the
-// columns of I_m and take every reflection with the matrix (they are never a reflector column,
-// so they stay in registers): they end as Q_full^H and Q is read off them, in the k steps of the
-// factorisation.  Otherwise Q is formed backwards from the columns left in X with the stored
-// scal and alpha - beta, k more steps.  A is read whole (through LDS, coalesced) before anything
-// is written, so Q may be A itself.
+// next diagonal back to LDS.  Q is accumulated in the same k steps: the columns of I_m take
+// every reflection with the matrix and end as Q_full^H, whose first k rows give Q.  They are
+// never a reflector column, so they stay in registers: in the spare lanes n .. n + m - 1 when
+// n + m <= 64 (at no extra cost per step), else in a second register slot of lane u with its
+// own partial sums.  A is read whole (through LDS, coalesced) before anything is written, so Q
+// may be A itself.
 //   With lqd = d > 0 the kernel does the LQ of a site instead (Site::gaugeL): A is the site
 // tensor [d, n, m / d] and the matrix factorised is B[s + d c, a] = conj(site[s, a, c])
 // (m x n); Rg receives L = R^H (n x k) and Qg the new site [d, k, m / d] = Q^H reshaped.
@@ -396,13 +396,12 @@ __device__ __forceinline__ void sub_mul(double2& a, double2 x, double2 g) {
   a.y = fma(-x.y, g.x, a.y);
 }
 
-template <int RPT>
+template <int RPT, bool Q2>
 __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, int m, int n, double2* Qg,
                                                              double2* __restrict__ Rg, int lqd) {
   __shared__ double2 X[kSQStage];
-  __shared__ double2 part[kSQW][64];
+  __shared__ double2 part[kSQW][64], partB[kSQW][64];
   __shared__ double nrm_s[kSQW][64];
-  __shared__ double2 tau_s[64], scal_s[64], dd_s[64];
   __shared__ double beta_s[64];
   const int tid = threadIdx.x, u = tid & 63, nt = blockDim.x, nw = nt >> 6;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), r0 = w * RPT;
@@ -420,13 +419,19 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
   // slots of absent waves read as zeros
   for (int e = nw * 64 + tid; e < kSQW * 64; e += nt) {
     (&part[0][0])[e] = double2{0.0, 0.0};
+    if constexpr (Q2) (&partB[0][0])[e] = double2{0.0, 0.0};
     (&nrm_s[0][0])[e] = 0.0;
   }
   __syncthreads();
   // With n + m <= 64 the lanes n .. n + m - 1 hold the columns of I_m and take every reflection
   // too, so they end as Q_full^H = H_{k-1}^H ... H_0^H and Q needs no backward pass.
-  const bool fq = n + m <= 64;
-  double2 a[RPT], xr[RPT];
+  // Otherwise a second register slot per lane, b, holds column u of I_m (u < m) and takes the
+  // same reflections, with its own partial sums.
+  // (Q2 is set exactly when n + m > 64.)
+  constexpr bool q2 = Q2, fq = !Q2;
+  double2 a[RPT], xr[RPT], b[RPT];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) b[i] = double2{(q2 && r0 + i == u && u < m) ? 1.0 : 0.0, 0.0};
   {
     double s2 = 0.0;
 #pragma unroll
@@ -460,9 +465,6 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     }
     const Reflector f = zlarfg_of(alpha, xs2, S, scaled);
     if (tid == 0) {
-      tau_s[c] = f.tau;
-      scal_s[c] = f.scal;
-      dd_s[c] = f.dd;
       beta_s[c] = f.beta;
     }
     // (H_c = I when !reflect: tau = scal = 0 and the step below only refreshes the partial norms)
@@ -474,13 +476,25 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     }
     // this wave's partial of v^H a_u
     const bool mine = (u > c && u < n) || (fq && u >= n && u < n + m);
+    const bool mineB = q2 && u < m;
     {
       double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], a[i]);
       if (mine) part[w][u] = cjmul(f.scal, double2{acc0.x + acc1.x, acc0.y + acc1.y});
     }
+    if constexpr (q2) {
+      double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], b[i]);
+      if (mineB) partB[w][u] = cjmul(f.scal, double2{acc0.x + acc1.x, acc0.y + acc1.y});
+    }
     __syncthreads();
+    if (mineB) {
+      const double2 g2 = cmul(f.scal, cjmul(f.tau, tree_sum<double2>(partB, u)));
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) sub_mul(b[i], xr[i], g2);
+    }
     // H_c^H on the trailing columns: a_u -= conj(tau) v (v^H a_u), then back to LDS
     if (mine) {
       const double2 g2 = cmul(f.scal, cjmul(f.tau, tree_sum<double2>(part, u)));
@@ -502,75 +516,31 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     const double2 r = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
     Rg[e] = lqd > 0 ? double2{r.x, -r.y} : r;
   }
-  if (fq) {
-    // Q[q, j] = conj(Q^H[j, q]): row j of lane n + q, for j < k
-    if (u >= n && u < n + m) {
-      const int q = u - n;
+  {
+    // Q[q, j] = conj(Q^H[j, q]): row j of lane n + q (fq) or of slot b of lane q, for j < k
+    const int q = fq ? u - n : u;
+    if (q >= 0 && q < m) {
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
         const int j = r0 + i;
         if (j >= k) continue;
+        const double2 h = fq ? a[i] : b[i];
         if (lqd > 0)
-          Qg[q % lqd + lqd * (j + k * (q / lqd))] = a[i];  // new site[s, j, c] = conj(Q[s + d c, j])
+          Qg[q % lqd + lqd * (j + k * (q / lqd))] = h;  // new site[s, j, c] = conj(Q[s + d c, j])
         else
-          Qg[q + m * j] = double2{a[i].x, -a[i].y};
+          Qg[q + m * j] = double2{h.x, -h.y};
       }
     }
-    return;
-  }
-  // Q = H_0 ... H_{k-1} [I; 0], accumulated backwards: Q <- Q - tau v (v^H Q), columns u >= c
-  double2 qv[RPT];
-#pragma unroll
-  for (int i = 0; i < RPT; ++i) qv[i] = double2{(r0 + i == u && u < k) ? 1.0 : 0.0, 0.0};
-  for (int c = k - 1; c >= 0; --c) {
-    const double2 t = tau_s[c];
-    if (t.x == 0.0 && t.y == 0.0) continue;
-    const double2 sc = scal_s[c], dd = dd_s[c];
-    const double2* xc = X + ld * c;
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = r0 + i;
-      const double2 x = xc[r];
-      xr[i] = r > c && r < m ? x : (r == c ? dd : double2{0.0, 0.0});
-    }
-    const bool mine = u >= c && u < k;
-    {
-      double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
-#pragma unroll
-      for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], qv[i]);
-      if (mine) part[w][u] = cjmul(sc, double2{acc0.x + acc1.x, acc0.y + acc1.y});
-    }
-    __syncthreads();
-    if (mine) {
-      const double2 g2 = cmul(sc, cmul(t, tree_sum<double2>(part, u)));
-#pragma unroll
-      for (int i = 0; i < RPT; ++i) sub_mul(qv[i], xr[i], g2);
-    }
-    __syncthreads();
-  }
-  __syncthreads();  // R's reads of X are done wherever no Q step ran
-  if (u < k) {
-#pragma unroll
-    for (int i = 0; i < RPT; ++i)
-      if (r0 + i < m) X[r0 + i + ld * u] = qv[i];
-  }
-  __syncthreads();
-  if (lqd > 0) {
-    // new site[s, j, c] = conj(Q[s + d c, j]), written in order
-    for (int e = tid; e < m * k; e += nt) {
-      const int s = e % lqd, j = (e / lqd) % k, c = e / (lqd * k);
-      const double2 q = X[s + lqd * c + ld * j];
-      Qg[e] = double2{q.x, -q.y};
-    }
-  } else {
-    for (int e = tid; e < m * k; e += nt) Qg[e] = X[e % m + ld * (e / m)];
   }
 }
 
 template <int RPT>
 void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd) {
   const int nw = (m + RPT - 1) / RPT;
-  hipLaunchKernelGGL(qr_small_kernel<RPT>, dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
+  if (n + m <= 64)
+    hipLaunchKernelGGL((qr_small_kernel<RPT, false>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
+  else
+    hipLaunchKernelGGL((qr_small_kernel<RPT, true>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
 }
 
 // rows per wave: up to four waves of four rows for m <= 16 (fewer waves per SIMD beat shorter
