@@ -7,11 +7,12 @@
 //
 // * One workgroup factorises a panel with the panel held in registers (a thread owns up to
 //   RPT rows); each column costs two workgroup reductions.  It also forms the panel's
-//   compact-WY factor T (zlarft, forward / columnwise) from the Gram products it reduces
-//   alongside, and writes V, Y = V (-T)^H and Z = V (-T).
-// * The trailing columns take C <- (I - V T^H V^H) C = C + Y (V^H C), and Q is accumulated
-//   backwards, Q <- (I - V T V^H) Q = Q + Z (V^H Q).  V^H C (16 rows, long K) is a chunked
-//   product with an ordered chunk sum; the rank-16 update C += Y W is a BRGEMM launch.
+//   compact-WY factor T (zlarft, forward / columnwise) column by column from the Gram products
+//   it reduces alongside, and writes V and -T.
+// * The trailing columns take C <- (I - V T^H V^H) C = C + V ((-T)^H (V^H C)), and Q is
+//   accumulated backwards, Q <- (I - V T V^H) Q = Q + V ((-T) (V^H Q)).  V^H C (16 rows, long K)
+//   is a chunked product whose ordered chunk sum also applies the 16 x 16 factor; the rank-16
+//   update is a BRGEMM launch.
 //
 // A panel step is a handful of launches where rocSOLVER's blocked QR issues several per
 // column, which is what the small and medium gauge steps of a sweep are bound by.
@@ -107,15 +108,13 @@ __device__ __forceinline__ double2 vval(int r, int c, double2 a) {
 }
 
 // Factorise columns [j0, j0+pb) of A (rows j0..m-1), pb <= kPB, m - j0 <= RPT * NT.
-// Writes R / v in place, V[:, j0+u] and Z[:, j0+u] (ld = m, rows >= j0) and Y[:, u] (ld = m).
+// Writes R / v in place, V[:, j0+u] (ld = m, rows >= j0) and -T (kPB x kPB, row-major) to Tg.
 template <int NT, int RPT>
 __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, int m, int j0, int pb,
-                                                       double2* __restrict__ V, double2* __restrict__ Y,
-                                                       double2* __restrict__ Z) {
+                                                       double2* __restrict__ V, double2* __restrict__ Tg) {
   constexpr int NW = NT / 64;
   __shared__ double2 red[kQWmax][kPB];
   __shared__ double2 sums[kPB];
-  __shared__ double2 G[kPB][kPB];   // G[a][b] = V[:,a]^H V[:,b] for a < b
   __shared__ double2 Tn[kPB][kPB];  // -T, upper triangular
   __shared__ double2 tau_s[kPB];
   __shared__ double2 scal_s;
@@ -211,7 +210,25 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
         }
       }
       block_sum16<NW>(part, red, sums);
-      if (tid < t) G[tid][t] = sums[tid];
+      // column t of -T (zlarft, forward columnwise): Tn[t][t] = -tau_t and, with
+      // G[j][t] = V[:, j]^H v = sums[j] (j < t), Tn[r][t] = -tau_t sum_{r <= j < t} Tn[r][j] G[j][t]
+      if (tid < kPB) {
+        const int r = tid;
+        double2 v{0.0, 0.0};
+        if (r == t) {
+          v = double2{-tau.x, -tau.y};
+        } else if (r < t) {
+          double2 acc{0.0, 0.0};
+          for (int j = r; j < t; ++j) {
+            const double2 p = cmul(Tn[r][j], sums[j]);
+            acc.x += p.x;
+            acc.y += p.y;
+          }
+          const double2 q = cmul(tau, acc);
+          v = double2{-q.x, -q.y};
+        }
+        Tn[r][t] = v;
+      }
       // H^H on the remaining panel columns: a_u -= conj(tau) v w_u
       const double2 ctau{tau.x, -tau.y};
 #pragma unroll
@@ -231,56 +248,21 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
     }
   }
 
-  // -T of the panel (zlarft, forward columnwise): T[t][t] = tau_t,
-  // T[0:t, t] = -tau_t T[0:t, 0:t] G[0:t, t]; with Tn = -T: Tn[r][t] = -tau_t sum_j Tn[r][j] G[j][t]
-  for (int t = 0; t < kPB; ++t) {
-    if (tid < kPB) {
-      const int r = tid;
-      double2 v{0.0, 0.0};
-      if (t < pb) {
-        if (r == t) {
-          v = double2{-tau_s[t].x, -tau_s[t].y};
-        } else if (r < t) {
-          double2 acc{0.0, 0.0};
-          for (int j = r; j < t; ++j) {
-            const double2 p = cmul(Tn[r][j], G[j][t]);
-            acc.x += p.x;
-            acc.y += p.y;
-          }
-          const double2 q = cmul(tau_s[t], acc);
-          v = double2{-q.x, -q.y};
-        }
-      }
-      Tn[r][t] = v;
-    }
-    __syncthreads();
+  // columns t >= pb of -T are zero
+  for (int e = tid; e < kPB * kPB; e += NT) {
+    const int r = e / kPB, t = e % kPB;
+    Tg[e] = t < pb ? Tn[r][t] : double2{0.0, 0.0};
   }
-
-  // outputs: A (R / v), V, Y = V Tn^H, Z = V Tn
+  // outputs: A (R / v) and V
 #pragma unroll
   for (int i = 0; i < RPT; ++i) {
     if (row[i] >= m) continue;
-    double2 vr[kPB];
-#pragma unroll
-    for (int u = 0; u < kPB; ++u) vr[u] = u < pb ? vval(row[i], j0 + u, a[i][u]) : double2{0.0, 0.0};
 #pragma unroll
     for (int u = 0; u < kPB; ++u) {
       if (u >= pb) continue;
       const int64_t col = (int64_t)m * (j0 + u);
       A[row[i] + col] = a[i][u];
-      V[row[i] + col] = vr[u];
-      double2 y{0.0, 0.0}, z{0.0, 0.0};
-#pragma unroll
-      for (int q = 0; q < kPB; ++q) {
-        const double2 py = cjmul(Tn[u][q], vr[q]);  // vr[q] * conj(Tn[u][q])
-        y.x += py.x;
-        y.y += py.y;
-        const double2 pz = cmul(vr[q], Tn[q][u]);
-        z.x += pz.x;
-        z.y += pz.y;
-      }
-      Y[row[i] + (int64_t)m * u] = y;
-      Z[row[i] + col] = z;
+      V[row[i] + col] = vval(row[i], j0 + u, a[i][u]);
     }
   }
 }
@@ -608,27 +590,43 @@ __global__ void __launch_bounds__(256) vh_partial_kernel(const double2* __restri
   if (a < pb && jb + j < cols) P[((int64_t)blockIdx.y * cols + jb + j) * kPB + a] = acc;
 }
 
+// W = op(Tn) (V^H C): the chunk partials of V^H C summed in chunk order, then multiplied by the
+// panel's -T (op = conjugate transpose for the trailing columns, none for Q).  One workgroup per
+// 16 columns of W; thread (a, jj) owns W[a][j0 + jj].
 __global__ void __launch_bounds__(256) vh_reduce_kernel(const double2* __restrict__ P, int nch, int pb, int cols,
+                                                        const double2* __restrict__ Tg, int conjT,
                                                         double2* __restrict__ W) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)pb * cols) return;
-  const int a = (int)(t % pb), j = (int)(t / pb);
+  __shared__ double2 Ws[kPB][kVJ + 1];
+  __shared__ double2 Ts[kPB][kPB + 1];
+  const int tid = threadIdx.x, a = tid % kPB, jj = tid / kPB, j = blockIdx.x * kVJ + jj;
+  Ts[tid / kPB][tid % kPB] = Tg[tid];
   double2 s{0.0, 0.0};
-  for (int ch = 0; ch < nch; ++ch) {
-    const double2 v = P[((int64_t)ch * cols + j) * kPB + a];
-    s.x += v.x;
-    s.y += v.y;
+  if (a < pb && j < cols)
+    for (int ch = 0; ch < nch; ++ch) {
+      const double2 v = P[((int64_t)ch * cols + j) * kPB + a];
+      s.x += v.x;
+      s.y += v.y;
+    }
+  Ws[a][jj] = s;
+  __syncthreads();
+  if (a >= pb || j >= cols) return;
+  double2 o{0.0, 0.0};
+  for (int b = 0; b < pb; ++b) {
+    const double2 p = conjT ? cjmul(Ts[b][a], Ws[b][jj]) : cmul(Ts[a][b], Ws[b][jj]);
+    o.x += p.x;
+    o.y += p.y;
   }
-  W[a + (int64_t)pb * j] = s;
+  W[a + (int64_t)pb * j] = o;
 }
 
-// W (pb x cols, ld pb) = V^H C, V rows x pb (ld ldv), C rows x cols (ld ldc); P: partials.
-hipError_t vh_times(const double2* V, int ldv, int rows, int pb, const double2* C, int ldc, int cols, double2* W,
-                    double2* P, hipStream_t s) {
+// W (pb x cols, ld pb) = op(Tn) V^H C, V rows x pb (ld ldv), C rows x cols (ld ldc), Tn the
+// panel's -T (kPB x kPB, row-major); P: partials.
+hipError_t vh_times(const double2* V, int ldv, int rows, int pb, const double2* C, int ldc, int cols,
+                    const double2* Tn, int conjT, double2* W, double2* P, hipStream_t s) {
   const int nch = (rows + kVC - 1) / kVC;
   hipLaunchKernelGGL(vh_partial_kernel, dim3((cols + kVJ - 1) / kVJ, nch), dim3(256), 0, s, V, ldv, rows, pb, C, ldc,
                      cols, P);
-  hipLaunchKernelGGL(vh_reduce_kernel, dim3(grid1((int64_t)pb * cols)), dim3(256), 0, s, P, nch, pb, cols, W);
+  hipLaunchKernelGGL(vh_reduce_kernel, dim3((cols + kVJ - 1) / kVJ), dim3(256), 0, s, P, nch, pb, cols, Tn, conjT, W);
   return hipGetLastError();
 }
 
@@ -680,33 +678,34 @@ int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   if (Q == A) return set_err(ctx, MPSD_ERR_ARG, "house_qr: Q may alias A only for small matrices");
   const int k = std::min(m, n);
   const int64_t mk = (int64_t)m * k;
-  const int64_t wn = std::max(n, k), nch = (m + kVC - 1) / kVC;
-  const size_t need = sizeof(double2) * (size_t)(2 * mk + (int64_t)m * kPB + kPB * wn * (1 + nch));
+  const int64_t wn = std::max(n, k), nch = (m + kVC - 1) / kVC, npan = (k + kPB - 1) / kPB;
+  const size_t need = sizeof(double2) * (size_t)(mk + npan * kPB * kPB + kPB * wn * (1 + nch));
   int rc = ensure_scratch(ctx, 6, need);
   if (rc) return rc;
   double2* V = (double2*)ctx->scratch[6];
-  double2* Z = V + mk;
-  double2* Y = Z + mk;
-  double2* W = Y + (int64_t)m * kPB;
-  double2* P = W + kPB * wn;  // chunk partials of V^H C
+  double2* T = V + mk;                  // -T of panel p at T + p kPB^2
+  double2* W = T + npan * kPB * kPB;
+  double2* P = W + kPB * wn;            // chunk partials of V^H C
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
   for (int j0 = 0; j0 < k && e == hipSuccess; j0 += kPB) {
     const int pb = std::min(kPB, k - j0), rows = m - j0;
     // up to 256 rows: one wave per SIMD; beyond, 512 threads (two waves per SIMD) with one or
     // two rows per thread
+    double2* Tp = T + (int64_t)(j0 / kPB) * kPB * kPB;
+    const double2* Vp = V + j0 + (int64_t)m * j0;
     if (rows <= kQT)
-      hipLaunchKernelGGL((qr_panel_kernel<kQT, 1>), dim3(1), dim3(kQT), 0, s, A, m, j0, pb, V, Y, Z);
+      hipLaunchKernelGGL((qr_panel_kernel<kQT, 1>), dim3(1), dim3(kQT), 0, s, A, m, j0, pb, V, Tp);
     else if (rows <= 512)
-      hipLaunchKernelGGL((qr_panel_kernel<512, 1>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Y, Z);
+      hipLaunchKernelGGL((qr_panel_kernel<512, 1>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Tp);
     else
-      hipLaunchKernelGGL((qr_panel_kernel<512, 2>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Y, Z);
+      hipLaunchKernelGGL((qr_panel_kernel<512, 2>), dim3(1), dim3(512), 0, s, A, m, j0, pb, V, Tp);
     e = hipGetLastError();
     const int cols = n - j0 - pb;
     if (e == hipSuccess && cols > 0) {
       double2* C = A + j0 + (int64_t)m * (j0 + pb);
-      e = vh_times(V + j0 + (int64_t)m * j0, m, rows, pb, C, m, cols, W, P, s);
-      if (e == hipSuccess) e = plus_times(Y + j0, m, rows, pb, W, C, m, cols, s);
+      e = vh_times(Vp, m, rows, pb, C, m, cols, Tp, 1, W, P, s);
+      if (e == hipSuccess) e = plus_times(Vp, m, rows, pb, W, C, m, cols, s);
     }
   }
   if (e == hipSuccess) {
@@ -719,8 +718,10 @@ int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   for (int j0 = last; j0 >= 0 && e == hipSuccess; j0 -= kPB) {
     const int pb = std::min(kPB, k - j0), rows = m - j0, cols = k - j0;
     double2* Qs = Q + j0 + (int64_t)m * j0;
-    e = vh_times(V + j0 + (int64_t)m * j0, m, rows, pb, Qs, m, cols, W, P, s);
-    if (e == hipSuccess) e = plus_times(Z + j0 + (int64_t)m * j0, m, rows, pb, W, Qs, m, cols, s);
+    const double2* Vp = V + j0 + (int64_t)m * j0;
+    const double2* Tp = T + (int64_t)(j0 / kPB) * kPB * kPB;
+    e = vh_times(Vp, m, rows, pb, Qs, m, cols, Tp, 0, W, P, s);
+    if (e == hipSuccess) e = plus_times(Vp, m, rows, pb, W, Qs, m, cols, s);
   }
   return hip_err(ctx, e, "house_qr");
 }
