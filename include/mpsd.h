@@ -293,6 +293,15 @@ int mpsd_site_gauge_l_svd(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, 
 int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* R);
 /* site[s, a, :] := site[s, a, :] . L  -- Site::reduceDimL (Site.cpp:263-283). */
 int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L);
+/* A gauge step and the neighbour update that follows it, as MPS::gaugeCond runs them
+ * (MPS.cpp:306-337, 366-410): gauge_r_absorb = mpsd_site_gauge_r(site, R_out) then
+ * mpsd_site_reduce_dim_r(next, R_out); gauge_l_absorb = mpsd_site_gauge_l(site, L_out) then
+ * mpsd_site_reduce_dim_l(prev, L_out).  Where the single-workgroup QR takes the site (the QR
+ * path, d*Dl <= 64 and Dr <= 64 for gauge_r, d*Dr <= 64 and Dl <= 64 for gauge_l, not a 1x1
+ * edge factor) it also applies the factor from its LDS: one launch instead of two.  The factor
+ * is still written to R_out / L_out.  Same results as the two calls to rounding. */
+int mpsd_site_gauge_r_absorb(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out, mpsd_tensor* next);
+int mpsd_site_gauge_l_absorb(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, mpsd_tensor* prev);
 
 /* ---- device-driven Lanczos steps (replayed hipGraph) ----------------------------------------
  * The Lanczos loop of the eigensolver with every step's kernels driven from a device

@@ -157,6 +157,10 @@ class Site {
   void gaugeL(DevTensor& leftterm, int cutD = 0);
   void reduceDimR(const DevTensor& leftterm);   // Site.cpp:240-261
   void reduceDimL(const DevTensor& rightterm);  // Site.cpp:263-283
+  // gaugeR then next.reduceDimR, gaugeL then prev.reduceDimL, as MPS::gaugeCond runs them
+  // (MPS.cpp:306-337, 366-410): one launch for sites the single-workgroup QR takes
+  void gaugeRInto(DevTensor& rightterm, Site& next);
+  void gaugeLInto(DevTensor& leftterm, Site& prev);
 
  private:
   mpsd_ctx* ctx_;

@@ -337,6 +337,14 @@ void Site::reduceDimL(const DevTensor& rightterm) {
   check(ctx_, mpsd_site_reduce_dim_l(ctx_, dev_.t, rightterm.t), "reduceDimL");
 }
 
+void Site::gaugeRInto(DevTensor& rightterm, Site& next) {
+  check(ctx_, mpsd_site_gauge_r_absorb(ctx_, dev_.t, rightterm.t, next.dev_.t), "gaugeR + reduceDimR");
+}
+
+void Site::gaugeLInto(DevTensor& leftterm, Site& prev) {
+  check(ctx_, mpsd_site_gauge_l_absorb(ctx_, dev_.t, leftterm.t, prev.dev_.t), "gaugeL + reduceDimL");
+}
+
 // ------------------------------------------------------------------------------ MPS
 MPS::MPS(int length, int bond, int d, mpsd_ctx* ctx) : MPS(length, bond, std::vector<int>(length, d), ctx) {}
 
@@ -375,14 +383,22 @@ void MPS::gaugeCond(char dir, bool normal, int cutD) {
   const int L = getLength();
   if (dir == 'L' || dir == 'l') {
     for (int pos = L - 1; pos >= 0; --pos) {
-      A_[pos]->gaugeL(tmp, cutD);
-      if (pos > 0) A_[pos - 1]->reduceDimL(tmp);
+      if (pos > 0 && cutD <= 0) {
+        A_[pos]->gaugeLInto(tmp, *A_[pos - 1]);
+      } else {
+        A_[pos]->gaugeL(tmp, cutD);
+        if (pos > 0) A_[pos - 1]->reduceDimL(tmp);
+      }
     }
     gauge_ = gLeft;
   } else if (dir == 'R' || dir == 'r') {
     for (int pos = 0; pos < L; ++pos) {
-      A_[pos]->gaugeR(tmp, cutD);
-      if (pos < L - 1) A_[pos + 1]->reduceDimR(tmp);
+      if (pos < L - 1 && cutD <= 0) {
+        A_[pos]->gaugeRInto(tmp, *A_[pos + 1]);
+      } else {
+        A_[pos]->gaugeR(tmp, cutD);
+        if (pos < L - 1) A_[pos + 1]->reduceDimR(tmp);
+      }
     }
     gauge_ = gRight;
   } else {
@@ -417,13 +433,21 @@ void MPS::gaugeCond(int pos, char dir, bool next) {
   DevTensor& tmp = factor();
   const int L = getLength();
   if (dir == 'L' || dir == 'l') {
-    const int oldDl = A_[pos]->getDl();
-    A_[pos]->gaugeL(tmp);
-    if ((A_[pos]->getDl() != oldDl || next) && pos > 0) A_[pos - 1]->reduceDimL(tmp);
+    if (next && pos > 0) {
+      A_[pos]->gaugeLInto(tmp, *A_[pos - 1]);
+    } else {
+      const int oldDl = A_[pos]->getDl();
+      A_[pos]->gaugeL(tmp);
+      if (A_[pos]->getDl() != oldDl && pos > 0) A_[pos - 1]->reduceDimL(tmp);
+    }
   } else {
-    const int oldDr = A_[pos]->getDr();
-    A_[pos]->gaugeR(tmp);
-    if (pos < L - 1 && (A_[pos]->getDr() != oldDr || next)) A_[pos + 1]->reduceDimR(tmp);
+    if (next && pos < L - 1) {
+      A_[pos]->gaugeRInto(tmp, *A_[pos + 1]);
+    } else {
+      const int oldDr = A_[pos]->getDr();
+      A_[pos]->gaugeR(tmp);
+      if (pos < L - 1 && A_[pos]->getDr() != oldDr) A_[pos + 1]->reduceDimR(tmp);
+    }
   }
   gauge_ = gNone;
 }

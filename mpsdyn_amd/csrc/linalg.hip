@@ -828,6 +828,76 @@ int mpsd_site_reduce_dim_r(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* 
   return hip_err(ctx, e, "reduce_dim_r");
 }
 
+int mpsd_site_gauge_r_absorb(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* R_out, mpsd_tensor* next) {
+  // gauge_r(site, R) then reduce_dim_r(next, R) (MPS::gaugeCond, MPS.cpp:322-337, 392-401); one
+  // launch when the single-workgroup QR takes the site
+  if (!ctx || !site || !R_out || !next || site == R_out || next == R_out || next == site) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = d * Dl, n = Dr, k = m < n ? m : n;
+  const int nd = next->dims[0], nDl = next->rank > 1 ? next->dims[1] : 1, nDr = next->rank > 2 ? next->dims[2] : 1;
+  if (!(ctx->gauge_qr == 0 && house_qr_small(m, n) && !(k == 1 && n == 1) && nDl == n && next->rank <= 3)) {
+    const int rc = mpsd_site_gauge_r(ctx, site, R_out);
+    return rc ? rc : mpsd_site_reduce_dim_r(ctx, next, R_out);
+  }
+  const int rd[2] = {k, n};
+  int rc = mpsd_tensor_reshape(R_out, 2, rd);
+  if (rc) return rc;
+  double2* old = nullptr;
+  int64_t oldcap = 0;
+  rc = take_new_buffer(ctx, next, nd, k, nDr, &old, &oldcap);
+  if (rc) return rc;
+  Absorb ab;
+  ab.old = old;
+  ab.out = next->ptr;
+  ab.nd = nd;
+  ab.nD = nDr;
+  rc = house_qr_small_absorb(ctx, site->ptr, m, n, site->ptr, R_out->ptr, ab);
+  pool_free(ctx, old, oldcap);
+  if (rc) return rc;
+  const int sd[3] = {d, Dl, k};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  ++site->version;
+  ++R_out->version;
+  ++next->version;
+  return MPSD_OK;
+}
+
+int mpsd_site_gauge_l_absorb(mpsd_ctx* ctx, mpsd_tensor* site, mpsd_tensor* L_out, mpsd_tensor* prev) {
+  // gauge_l(site, L) then reduce_dim_l(prev, L) (MPS::gaugeCond, MPS.cpp:310-320, 370-379); one
+  // launch when the single-workgroup QR takes the site
+  if (!ctx || !site || !L_out || !prev || site == L_out || prev == L_out || prev == site) return MPSD_ERR_ARG;
+  const int d = site->dims[0], Dl = site->rank > 1 ? site->dims[1] : 1, Dr = site->rank > 2 ? site->dims[2] : 1;
+  const int m = Dl, n = d * Dr, k = m < n ? m : n;
+  const int pd = prev->dims[0], pDl = prev->rank > 1 ? prev->dims[1] : 1, pDr = prev->rank > 2 ? prev->dims[2] : 1;
+  if (!(ctx->gauge_qr == 0 && house_qr_small(n, m) && !(k == 1 && m == 1) && pDr == m && prev->rank <= 3)) {
+    const int rc = mpsd_site_gauge_l(ctx, site, L_out);
+    return rc ? rc : mpsd_site_reduce_dim_l(ctx, prev, L_out);
+  }
+  const int ld[2] = {m, k};
+  int rc = mpsd_tensor_reshape(L_out, 2, ld);
+  if (rc) return rc;
+  double2* old = nullptr;
+  int64_t oldcap = 0;
+  rc = take_new_buffer(ctx, prev, pd, pDl, k, &old, &oldcap);
+  if (rc) return rc;
+  Absorb ab;
+  ab.old = old;
+  ab.out = prev->ptr;
+  ab.nd = pd;
+  ab.nD = pDl;
+  rc = house_lq_site_small(ctx, site->ptr, d, Dl, Dr, site->ptr, L_out->ptr, ab);
+  pool_free(ctx, old, oldcap);
+  if (rc) return rc;
+  const int sd[3] = {d, k, Dr};
+  rc = mpsd_tensor_reshape(site, 3, sd);
+  if (rc) return rc;
+  ++site->version;
+  ++L_out->version;
+  ++prev->version;
+  return MPSD_OK;
+}
+
 int mpsd_site_reduce_dim_l(mpsd_ctx* ctx, mpsd_tensor* site, const mpsd_tensor* L) {
   // site[s, a, j] = sum_c site_old[s, a, c] L[c, j]   (Site::reduceDimL, Site.cpp:263-283)
   if (!ctx || !site || !L || site == L) return MPSD_ERR_ARG;

@@ -257,7 +257,19 @@ int house_qr_max_rows();
 bool house_qr_small(int m, int n);
 // LQ of a site tensor [d, Dl, Dr] in one launch (house_qr_small(d Dr, Dl)): L (Dl x k) and the
 // new site [d, k, Dr], k = min(Dl, d Dr), as Site::gaugeL defines them; site_out may be site.
-int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L);
+// The neighbour update that follows a gauge step (Site::reduceDimR / reduceDimL), done by the
+// single-workgroup QR from R in its LDS: old [nd, n, nD] -> out [nd, k, nD] after a QR, old
+// [nd, nD, n] -> out [nd, nD, k] after an LQ (n: the factorised matrix's column count).
+// out == nullptr: no update.
+struct Absorb {
+  const double2* old = nullptr;
+  double2* out = nullptr;
+  int nd = 0, nD = 0;
+};
+int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L,
+                        Absorb ab = Absorb{});
+// house_qr of a small matrix (house_qr_small(m, n)) that also applies R to a neighbour
+int house_qr_small_absorb(mpsd_ctx* ctx, const double2* A, int m, int n, double2* Q, double2* R, Absorb ab);
 // One-workgroup restarted Lanczos for small Heffs (lanczos_fused.hip): whether h fits, and
 // the solve (out6: theta, residual, matvecs, restarts, converged, status 0 / 2 start vector
 // zero or not finite / 3 non-finite coefficients).

@@ -380,7 +380,7 @@ __device__ __forceinline__ void sub_mul(double2& a, double2 x, double2 g) {
 
 template <int RPT, bool Q2>
 __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, int m, int n, double2* Qg,
-                                                             double2* __restrict__ Rg, int lqd) {
+                                                             double2* __restrict__ Rg, int lqd, Absorb ab) {
   __shared__ double2 X[kSQStage];
   __shared__ double2 part[kSQW][64], partB[kSQW][64];
   __shared__ double nrm_s[kSQW][64];
@@ -498,6 +498,37 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     const double2 r = i < j ? X[i + ld * j] : (i == j ? double2{beta_s[i], 0.0} : double2{0.0, 0.0});
     Rg[e] = lqd > 0 ? double2{r.x, -r.y} : r;
   }
+  if (ab.out) {
+    // the factor into the neighbour (Site::reduceDimR / reduceDimL) from R in LDS, with
+    // R[i][j] = X[i + ld j] above the diagonal, beta_i on it and zero below:
+    //   QR: next[s, i, c] = sum_{j >= i} R[i][j] old[s, j, c]        (old [nd, n, nD] -> [nd, k, nD])
+    //   LQ: prev[s, a, j] = sum_{c >= j} old[s, a, c] conj(R[j][c])  (old [nd, nD, n] -> [nd, nD, k])
+    const int tot = ab.nd * k * ab.nD;
+    for (int e = tid; e < tot; e += nt) {
+      const int s = e % ab.nd;
+      double2 acc{0.0, 0.0};
+      if (lqd == 0) {
+        const int i = (e / ab.nd) % k, c = e / (ab.nd * k);
+        const double2* o = ab.old + s + (int64_t)ab.nd * n * c;
+        for (int j = i; j < n; ++j) {
+          const double2 r = j == i ? double2{beta_s[i], 0.0} : X[i + ld * j];
+          const double2 v = o[(int64_t)ab.nd * j];
+          acc.x = fma(r.x, v.x, fma(-r.y, v.y, acc.x));
+          acc.y = fma(r.x, v.y, fma(r.y, v.x, acc.y));
+        }
+      } else {
+        const int a = (e / ab.nd) % ab.nD, j = e / (ab.nd * ab.nD);
+        const double2* o = ab.old + s + (int64_t)ab.nd * a;
+        for (int c = j; c < n; ++c) {
+          const double2 r = c == j ? double2{beta_s[j], 0.0} : X[j + ld * c];
+          const double2 v = o[(int64_t)ab.nd * ab.nD * c];
+          acc.x = fma(v.x, r.x, fma(v.y, r.y, acc.x));
+          acc.y = fma(v.y, r.x, fma(-v.x, r.y, acc.y));
+        }
+      }
+      ab.out[e] = acc;
+    }
+  }
   {
     // Q[q, j] = conj(Q^H[j, q]): row j of lane n + q (fq) or of slot b of lane q, for j < k
     const int q = fq ? u - n : u;
@@ -517,23 +548,24 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
 }
 
 template <int RPT>
-void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd) {
+void launch_qr_small(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd, Absorb ab) {
   const int nw = (m + RPT - 1) / RPT;
   if (n + m <= 64)
-    hipLaunchKernelGGL((qr_small_kernel<RPT, false>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
+    hipLaunchKernelGGL((qr_small_kernel<RPT, false>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd, ab);
   else
-    hipLaunchKernelGGL((qr_small_kernel<RPT, true>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd);
+    hipLaunchKernelGGL((qr_small_kernel<RPT, true>), dim3(1), dim3(64 * nw), 0, s, A, m, n, Q, R, lqd, ab);
 }
 
 // rows per wave: up to four waves of four rows for m <= 16 (fewer waves per SIMD beat shorter
 // row loops there), else up to 16 waves of 2 or 4 rows
-void launch_qr_small_any(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd) {
+void launch_qr_small_any(hipStream_t s, const double2* A, int m, int n, double2* Q, double2* R, int lqd,
+                         Absorb ab = Absorb{}) {
   if (m <= 16)
-    launch_qr_small<4>(s, A, m, n, Q, R, lqd);
+    launch_qr_small<4>(s, A, m, n, Q, R, lqd, ab);
   else if (m <= 32)
-    launch_qr_small<2>(s, A, m, n, Q, R, lqd);
+    launch_qr_small<2>(s, A, m, n, Q, R, lqd, ab);
   else
-    launch_qr_small<4>(s, A, m, n, Q, R, lqd);
+    launch_qr_small<4>(s, A, m, n, Q, R, lqd, ab);
 }
 
 // Q = I (m x k)
@@ -662,11 +694,18 @@ int house_qr_max_rows() { return 1024; }
 // and a zlarfg whatever the size, so it pays where the panels' dozen launches dominate
 bool house_qr_small(int m, int n) { return m <= 4 * kSQW && n <= 64; }
 
-int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L) {
+int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L,
+                        Absorb ab) {
   if (d < 1 || Dl < 1 || Dr < 1 || !house_qr_small(d * Dr, Dl))
     return set_err(ctx, MPSD_ERR_ARG, "house_lq_site_small: unsupported size");
-  launch_qr_small_any(ctx->stream, site, d * Dr, Dl, site_out, L, d);
+  launch_qr_small_any(ctx->stream, site, d * Dr, Dl, site_out, L, d, ab);
   return hip_err(ctx, hipGetLastError(), "house_lq_site_small");
+}
+
+int house_qr_small_absorb(mpsd_ctx* ctx, const double2* A, int m, int n, double2* Q, double2* R, Absorb ab) {
+  if (!house_qr_small(m, n)) return set_err(ctx, MPSD_ERR_ARG, "house_qr_small_absorb: unsupported size");
+  launch_qr_small_any(ctx->stream, A, m, n, Q, R, 0, ab);
+  return hip_err(ctx, hipGetLastError(), "house_qr_small_absorb");
 }
 
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {

@@ -100,6 +100,8 @@ def _optional(lib):
         "mpsd_site_gauge_l": (I, [P, P, P]),
         "mpsd_site_reduce_dim_r": (I, [P, P, P]),
         "mpsd_site_reduce_dim_l": (I, [P, P, P]),
+        "mpsd_site_gauge_r_absorb": (I, [P, P, P, P]),
+        "mpsd_site_gauge_l_absorb": (I, [P, P, P, P]),
         "mpsd_gs_solve": (I, [P, C.POINTER(GsConfig), C.POINTER(GsResult)]),
         "mpsd_gs_solve_export": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_gs_expectation": (I, [P, C.POINTER(GsConfig), C.c_char_p, C.POINTER(D), C.POINTER(D)]),

@@ -135,3 +135,46 @@ def test_ground_state_unchanged_by_qr_path(ctx):
     finally:
         ctx.set_option(dev.OPT_GAUGE_QR, 0)
     assert abs(e0 - e1) <= 1e-10 * abs(e1)
+
+
+# (d, Dl, Dr, Dn): site [d, Dl, Dr] and its neighbour's outer bond Dn.  The single-workgroup
+# form in both Q layouts (m + n <= 64 and above), d = 3, a growing bond (k < n), the edge, and
+# panel sizes where the fused entries fall back to the two calls.
+ABSORB = [(2, 20, 20, 20), (2, 8, 16, 16), (2, 32, 32, 32), (3, 12, 20, 7), (2, 1, 2, 4), (2, 4, 16, 3),
+          (2, 64, 64, 64), (2, 100, 40, 9), (2, 1, 1, 2)]
+
+
+@pytest.mark.parametrize("d,Dl,Dr,Dn", ABSORB)
+def test_gauge_absorb_matches_two_calls(ctx, d, Dl, Dr, Dn):
+    """mpsd_site_gauge_r_absorb / gauge_l_absorb against mpsd_site_gauge_r + reduce_dim_r and
+    mpsd_site_gauge_l + reduce_dim_l: the site and the factor bit for bit (the same QR kernel),
+    the neighbour to 1e-13 relative (a different summation order)."""
+    print(f"gauge+absorb d={d} Dl={Dl} Dr={Dr} Dn={Dn}", flush=True)
+    A = orc.complex_gaussian(300 + Dl + Dr, (d, Dl, Dr))
+    Bn = orc.complex_gaussian(301 + Dn, (d, Dr, Dn))   # right neighbour of a gauge_r step
+    Bp = orc.complex_gaussian(302 + Dn, (d, Dn, Dl))   # left neighbour of a gauge_l step
+    for side, B in (("r", Bn), ("l", Bp)):
+        outs = []
+        for fused in (False, True):
+            tA, tF, tB = ctx.tensor(A), ctx.tensor(dims=(1, 1)), ctx.tensor(B)
+            if fused:
+                f = ctx.lib.mpsd_site_gauge_r_absorb if side == "r" else ctx.lib.mpsd_site_gauge_l_absorb
+                ctx.check(f(ctx.handle, tA.handle, tF.handle, tB.handle))
+            else:
+                g = ctx.lib.mpsd_site_gauge_r if side == "r" else ctx.lib.mpsd_site_gauge_l
+                rdim = ctx.lib.mpsd_site_reduce_dim_r if side == "r" else ctx.lib.mpsd_site_reduce_dim_l
+                ctx.check(g(ctx.handle, tA.handle, tF.handle))
+                ctx.check(rdim(ctx.handle, tB.handle, tF.handle))
+            outs.append([t.download() for t in (tA, tF, tB)])
+        (a0, f0, b0), (a1, f1, b1) = outs
+        assert a0.shape == a1.shape and f0.shape == f1.shape and b0.shape == b1.shape
+        assert np.array_equal(a0, a1) and np.array_equal(f0, f1)
+        assert _rel(b1, b0) <= 1e-13, side
+        # the neighbour is the factor applied to the old neighbour
+        k = f0.shape[0] if side == "r" else f0.shape[1]
+        if side == "r":
+            want = np.einsum("ij,sjc->sic", f0, B)
+        else:
+            want = np.einsum("sac,cj->saj", B, f0)
+        assert b0.shape[1 if side == "r" else 2] == k
+        assert _rel(b1, want) <= 1e-13, side
