@@ -255,8 +255,6 @@ hipError_t heff_full_dev(mpsd_heff* h, double2* out);
 int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R);
 int house_qr_max_rows();
 bool house_qr_small(int m, int n);
-// LQ of a site tensor [d, Dl, Dr] in one launch (house_qr_small(d Dr, Dl)): L (Dl x k) and the
-// new site [d, k, Dr], k = min(Dl, d Dr), as Site::gaugeL defines them; site_out may be site.
 // The neighbour update that follows a gauge step (Site::reduceDimR / reduceDimL), done by the
 // single-workgroup QR from R in its LDS: old [nd, n, nD] -> out [nd, k, nD] after a QR, old
 // [nd, nD, n] -> out [nd, nD, k] after an LQ (n: the factorised matrix's column count).
@@ -266,6 +264,9 @@ struct Absorb {
   double2* out = nullptr;
   int nd = 0, nD = 0;
 };
+// LQ of a site tensor [d, Dl, Dr] in one launch (house_qr_small(d Dr, Dl)): L (Dl x k) and the
+// new site [d, k, Dr], k = min(Dl, d Dr), as Site::gaugeL defines them; site_out may be site;
+// with ab.out set it also updates the left neighbour.
 int house_lq_site_small(mpsd_ctx* ctx, const double2* site, int d, int Dl, int Dr, double2* site_out, double2* L,
                         Absorb ab = Absorb{});
 // house_qr of a small matrix (house_qr_small(m, n)) that also applies R to a neighbour
