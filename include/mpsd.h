@@ -100,6 +100,13 @@ int64_t mpsd_tensor_size(const mpsd_tensor* t); /* number of complex elements */
 /* Change the logical shape; reallocates only if the new size exceeds the capacity.
  * Replaces mwArray::reshape/resize of TmpManager slots (TmpManager.cpp:111-211). */
 int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims);
+/* Change the dimensions keeping the contents by index: the element at each multi-index inside
+ * both the old and the new dimensions is kept, new ones are zero, the rest are cut.  Same rank
+ * (MPSD_ERR_SHAPE otherwise).  Replaces mwArray::resize with zero noise
+ * (src/wrapper/mwArray.cpp:1604-1670), behind Site::changeDimensions (src/core/cpp/Site.cpp:
+ * 525-559) as MPS::increaseBondDimension and MPS::adjustPhysDimensions use it.  The device
+ * address changes (the copy goes to a new buffer). */
+int mpsd_tensor_resize(mpsd_tensor* t, int rank, const int* dims);
 /* Host data of the whole tensor to the device, stream-ordered: the host buffer may be reused
  * as soon as the call returns (the data goes through a pinned staging buffer of the context;
  * tensors larger than it are copied synchronously). */

@@ -33,6 +33,8 @@ typedef struct mpsd_gs_config {
   int precision;      /* MPSD_PREC_* of the matvecs */
   uint64_t seed;      /* random initial MPS */
   int solver;         /* MPSD_SOLVER_*: local eigensolver (Contractor::setEigenSolver) */
+  int want_residual;  /* 1: also compute residual_norm = ||H psi - E psi|| of the result (built-in
+                         models only); 0: not computed (residual_norm = -1) */
 } mpsd_gs_config;
 
 #define MPSD_SOLVER_LANCZOS 0 /* device Lanczos on the matvec (in place of primme / arpack) */
@@ -53,6 +55,9 @@ typedef struct mpsd_gs_result {
   int refine_from;        /* first half-sweep refined in FP64 (-1: FP64 throughout) */
   int n_history;
   double history[MPSD_GS_MAX_HISTORY]; /* energy at the end of each half-sweep */
+  double residual_norm;   /* ||H psi - E psi|| of the final (unit) state when cfg->want_residual, computed
+                             as sqrt(<psi|(H - E)^2|psi>) on the device over the MPO of (H - E)^2, with
+                             -E spread over the sites' local terms; else -1 */
 } mpsd_gs_result;
 
 /* Run one solve on ctx.  Returns MPSD_OK or an MPSD_ERR_* code (message: mpsd_last_error). */
@@ -62,6 +67,17 @@ int mpsd_gs_solve(mpsd_ctx* ctx, const mpsd_gs_config* cfg, mpsd_gs_result* out)
  * (src/core/cpp/MPS.cpp:440-459) to `export_path` (NULL: no export). */
 int mpsd_gs_solve_export(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path,
                          mpsd_gs_result* out);
+
+/* findGroundState from a given start state: the MPS in file mps_in (exportMPS layout), for the
+ * Hamiltonian of cfg's model or of the MPO file at mpo_path (format MPSD_MPO_*) when that is not
+ * NULL.  As the reference's findGroundState does (src/core/cpp/Contractor.cpp:1360-1370), the
+ * state is first fitted to the MPO's physical dimensions (MPS::adjustPhysDimensions,
+ * MPS.cpp:722-756) and then, for cfg->D > 0, grown by zero padding to bond cfg->D
+ * (MPS::increaseBondDimension, MPS.cpp:650-719) or cut to it by the truncating copy
+ * MPS(init, D) (MPS.cpp:761-773); cfg->D = 0 keeps its bond.  cfg->seed is not used.
+ * export_path as in mpsd_gs_solve_export. */
+int mpsd_gs_solve_from(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, const char* mps_in,
+                       const char* export_path, mpsd_gs_result* out);
 
 /* <psi|H|psi> of an MPS file in exportMPS layout for the model of cfg (Contractor::contract,
  * src/core/cpp/Contractor.cpp:528), e.g. to check a checkpoint. */
@@ -121,6 +137,13 @@ int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo
  * bond D by gaugeCond('L', false, D), truncated SVDs from the right with the norm kept in the
  * norm factor; written to mps_out in exportMPS layout. */
 int mpsd_mps_truncate(mpsd_ctx* ctx, const char* mps_in, int D, const char* mps_out);
+
+/* MPS::gaugeCond(pos, dir, next) (src/core/cpp/MPS.cpp:366-419) on an exported state: site pos
+ * (0-based) becomes an isometry by QR (dir 'R') or LQ (dir 'L'); the factor goes into the
+ * neighbour when next is set or the bond changed; with next set at the chain's end (gaugeL at
+ * site 0, gaugeR at the last site) the 1x1 factor's magnitude multiplies the norm factor
+ * (MPS.cpp:380-387, 402-409).  The result is written to mps_out. */
+int mpsd_mps_gauge_site(mpsd_ctx* ctx, const char* mps_in, int pos, char dir, int next, const char* mps_out);
 
 /* Lowest eigenpair of an Heff by the device FP64 Lanczos solver.  Replaces
  * Contractor::solveEigs(multi, 1, "SR", ...) (src/core/cpp/Contractor.cpp:3067-3138) and the

@@ -161,6 +161,16 @@ class Site {
   // (MPS.cpp:306-337, 366-410): one launch for sites the single-workgroup QR takes
   void gaugeRInto(DevTensor& rightterm, Site& next);
   void gaugeLInto(DevTensor& leftterm, Site& prev);
+  // Site::changeDimensions (Site.cpp:525-530): the tensor resized to [d, Dl, Dr] by index,
+  // zero-padded where it grows and cut where it shrinks (mwArray::resize with zero noise)
+  void changeDimensions(int d, int Dl, int Dr);
+  void increaseBondDim(int Dl, int Dr) { changeDimensions(getd(), Dl, Dr); }  // Site.cpp:532-534
+  void increasePhysDim(int d) {                                                // Site.cpp:545-552
+    if (d > getd()) changeDimensions(d, getDl(), getDr());
+  }
+  void decreasePhysDim(int d) {  // Site.cpp:553-559
+    if (d < getd()) changeDimensions(d, getDl(), getDr());
+  }
 
  private:
   mpsd_ctx* ctx_;
@@ -193,6 +203,13 @@ class MPS {
   // most cutD; without normalisation the last factor's magnitude goes into the norm factor
   void gaugeCond(char dir, bool normal = false, int cutD = 0);
   void gaugeCond(int pos, char dir, bool next = false);  // MPS.cpp:366-419
+  // MPS::increaseBondDimension (MPS.cpp:650-719, zero noise): every bond set to
+  // min(D, d-product to its left, d-product to its right); sites zero-padded (or cut where an
+  // old bond exceeds that profile); a no-op when D <= getBond()
+  void increaseBondDimension(int D);
+  // MPS::adjustPhysDimensions (MPS.cpp:722-756): site k resized to physical dimension d[k];
+  // when any site changed, the bond profile is rebuilt by increaseBondDimension(getBond())
+  void adjustPhysDimensions(const std::vector<int>& d);
   Gauge getGauge() const { return gauge_; }
   double getNormFact() const { return normfact_; }
   // Binary format of MPS::exportMPS/importMPS (MPS.cpp:442-509, Site.cpp:561-578).
@@ -413,7 +430,8 @@ class Contractor {
   // Part of a ground-state sweep, sites pos0 .. lim (excluded), moving right when lim >= pos0
   // (Contractor::sweepPart, Contractor.cpp:1260-1350).  Sites before pos0 are right-gauged
   // and sites after it left-gauged first; each site's gauge factor is dropped except at the
-  // last step, as in the reference.  The start state's bond dimension is used as it is.
+  // last step, as in the reference.  The start state is first fitted to ops and D as
+  // findGroundState does.
   void sweepPart(int pos0, int lim, const MPO& ops, int D, MPS& init, double offset = 0., int knr = 1);
 
   // Excited states by projector penalties (Contractor.cpp:2561-2957).  The local problem at
@@ -462,6 +480,11 @@ class Contractor {
   void penaltySweepPart(int pos0, int lim, const MPO& ops, const std::vector<double>& weights,
                         const std::vector<MPS*>& levels, MPS& init, double offset, bool passAlways);
   void checkLevels(const MPO& ops, const MPS& init, const std::vector<MPS*>& levels) const;
+  // The start state fitted to a call (Contractor.cpp:1360-1370, 1263-1279, 2580-2593, 2755-2766,
+  // 3650-3660): physical dimensions from ops (MPS::adjustPhysDimensions), then, for D > 0, the
+  // bond grown by zero padding (MPS::increaseBondDimension) or cut by the truncating copy
+  // MPS(init, D)
+  void adjustStart(const MPO& ops, int D, MPS& init);
   double excorthtol_ = 0.0;
   std::vector<std::pair<int, std::unique_ptr<Operator>>> ident_;
   DevTensor phis_;
@@ -521,5 +544,9 @@ class SchwingerHamiltonian : public Hamiltonian {
 // SplitMix64 + Box-Muller complex Gaussians, column-major over dims (same draws as the
 // Python test generator).
 mwArray complex_gaussian(uint64_t seed, const std::vector<int>& dims);
+
+// The MPO of the operator product A B: site tensors contracted over the inner physical index
+// (A's d_down with B's d_up), bonds merged as (a1, a2) -> a1 + Dl_A a2.
+MPO mpo_product(const MPO& a, const MPO& b);
 
 }  // namespace mpsdyn

@@ -218,6 +218,32 @@ __global__ void __launch_bounds__(256) mix_perm_kernel(const double2* __restrict
   else
     m[(sp + d * al) + (int64_t)d * wl * (be + wr * s)] = W[t];
 }
+
+// Old and new dimensions of a resize (column-major, rank <= 8).
+struct ResizeDims {
+  int rank = 0;
+  int od[8] = {0};
+  int nd[8] = {0};
+};
+
+// mwArray::resize (src/wrapper/mwArray.cpp:1604-1670) with zero noise: element i of the new
+// shape takes the old element at the same multi-index when every index is inside the old
+// dimensions, else zero.  One thread per new element.
+__global__ void __launch_bounds__(256) resize_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                     ResizeDims rd, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rest = i, off = 0, ostride = 1;
+    bool inside = true;
+    for (int r = 0; r < rd.rank; ++r) {
+      const int64_t idx = rest % rd.nd[r];
+      rest /= rd.nd[r];
+      inside = inside && idx < rd.od[r];
+      off += idx * ostride;
+      ostride *= rd.od[r];
+    }
+    out[i] = inside ? in[off] : double2{0.0, 0.0};
+  }
+}
 }  // namespace
 
 }  // namespace mpsd
@@ -473,6 +499,46 @@ int mpsd_tensor_reshape(mpsd_tensor* t, int rank, const int* dims) {
   t->rank = rank;
   for (int i = 0; i < 8; ++i) t->dims[i] = i < rank ? dims[i] : 0;
   t->n = n;
+  return MPSD_OK;
+}
+
+int mpsd_tensor_resize(mpsd_tensor* t, int rank, const int* dims) {
+  if (!t || rank < 1 || rank > 8 || !dims) return MPSD_ERR_ARG;
+  mpsd_ctx* ctx = t->ctx;
+  if (rank != t->rank)
+    return set_err(ctx, MPSD_ERR_SHAPE, "tensor_resize: rank " + std::to_string(rank) + " for a tensor of rank " +
+                                            std::to_string(t->rank));
+  ResizeDims rd;
+  rd.rank = rank;
+  int64_t n = 1;
+  bool same = true;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] <= 0) return set_err(ctx, MPSD_ERR_ARG, "tensor_resize: non-positive dimension");
+    rd.nd[i] = dims[i];
+    rd.od[i] = t->dims[i];
+    same = same && dims[i] == t->dims[i];
+    n *= dims[i];
+  }
+  if (same) return MPSD_OK;  // mwArray::resize: nothing to do
+  double2* p = nullptr;
+  int64_t cap = 0;
+  hipError_t e = pool_alloc(ctx, n, &p, &cap);
+  if (e != hipSuccess) return hip_err(ctx, e, "tensor_resize alloc");
+  const unsigned blocks = (unsigned)std::min<int64_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(resize_kernel, dim3(blocks), dim3(256), 0, ctx->stream, t->ptr, p, rd, n);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    pool_free(ctx, p, cap);
+    return hip_err(ctx, e, "tensor_resize");
+  }
+  // the old buffer goes back to the pool behind the copy on the same stream
+  pool_free(ctx, t->ptr, t->cap);
+  ctx->tensor_bytes += (int64_t)sizeof(double2) * (cap - t->cap);
+  t->ptr = p;
+  t->cap = cap;
+  for (int i = 0; i < rank; ++i) t->dims[i] = dims[i];
+  t->n = n;
+  ++t->version;
   return MPSD_OK;
 }
 

@@ -89,13 +89,40 @@ void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, m
   out->refine_from = st.refine_from;
   out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
   for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+  out->residual_norm = -1.0;
 }
 
 }  // namespace
 
 namespace mpsdyn {
+// The model of cfg with its energy shifted by -E, spread over the sites' local terms (the
+// Ising constant K, the Heisenberg and Schwinger offsets): (H - E) as an MPO whose partial
+// sums stay of the order of the fluctuations.
+std::unique_ptr<Hamiltonian> shifted_model(const mpsd_gs_config* c, double E) {
+  switch (c->model) {
+    case MPSD_MODEL_ISING:
+      return std::unique_ptr<Hamiltonian>(new IsingHamiltonian(c->L, 2, c->p[0], c->p[1], c->p[2], c->p[3] - E));
+    case MPSD_MODEL_HEISENBERG:
+      return std::unique_ptr<Hamiltonian>(new HeisenbergHamiltonian(c->L, c->p[0], c->p[1], c->p[2], c->p[3], 2, -E));
+    case MPSD_MODEL_SCHWINGER:
+      return std::unique_ptr<Hamiltonian>(new SchwingerHamiltonian(c->L, c->p[0], c->p[1], c->p[2], c->p[3], -E));
+    default:
+      throw Error(MPSD_ERR_ARG, "unknown model");
+  }
+}
+
+// ||H psi - E psi|| for a normalised psi: sqrt(<psi|(H - E)^2|psi>) over the MPO of (H - E)^2,
+// the product of the shifted model's MPO with itself (bond w^2), on the device.
+double residual_norm(Contractor& c, const mpsd_gs_config* cfg, MPS& mps, double E) {
+  auto Hs = shifted_model(cfg, E);
+  const MPO sq = mpo_product(Hs->getHMPO(), Hs->getHMPO());
+  const double v = c.contract(mps, sq, mps).real();
+  return std::sqrt(std::max(v, 0.0));
+}
+
 int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo, const char* export_path,
-                      const MultiplierFactory* factory, mpsd_gs_result* out);
+                      const MultiplierFactory* factory, mpsd_gs_result* out, const char* start_path = nullptr,
+                      bool model_residual = false);
 
 // One solve on ctx with an optional multiplier factory (used by the serving loop).
 int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_path, const MultiplierFactory* factory,
@@ -104,24 +131,38 @@ int gs_solve_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* export_p
   if (cfg->L < 2 || cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need L >= 2 and D >= 1");
   try {
     auto H = make_model(cfg);
-    return gs_solve_mpo_with(ctx, cfg, H->getHMPO(), export_path, factory, out);
+    return gs_solve_mpo_with(ctx, cfg, H->getHMPO(), export_path, factory, out, nullptr, true);
   } catch (const Error& e) {
     return fail(ctx, e);
   }
 }
 
 // The sweep itself, on any chain MPO; cfg supplies D, the tolerances, precision, seed and
-// solver (its model, L and p are not read).
+// solver (its model, L and p are not read unless model_residual is set).  start_path: an
+// exported MPS to start from instead of the seeded random state (findGroundState then fits it
+// to the MPO and to D; D = 0 keeps its bond).
 int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo, const char* export_path,
-                      const MultiplierFactory* factory, mpsd_gs_result* out) {
+                      const MultiplierFactory* factory, mpsd_gs_result* out, const char* start_path,
+                      bool model_residual) {
   if (!ctx || !cfg || !out) return MPSD_ERR_ARG;
-  if (cfg->D < 1) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need D >= 1");
+  if (cfg->D < (start_path ? 0 : 1)) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: need D >= 1");
   if (cfg->solver != MPSD_SOLVER_LANCZOS && cfg->solver != MPSD_SOLVER_FULLEIG)
     return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve: unknown solver");
+  if (cfg->want_residual && !model_residual)
+    return mpsd::set_err(ctx, MPSD_ERR_UNSUPPORTED, "gs_solve: the residual norm needs a built-in model");
   memset(out, 0, sizeof(*out));
   try {
-    MPS mps(hmpo.getLength(), cfg->D, hmpo.getDimensions(), ctx);
-    mps.setRandomState(cfg->seed);
+    MPS mps(hmpo.getLength(), std::max(cfg->D, 1), hmpo.getDimensions(), ctx);
+    if (start_path) {
+      mps.importMPS(start_path);
+      if (mps.getLength() != hmpo.getLength())
+        throw Error(MPSD_ERR_SHAPE, "gs_solve: start state and Hamiltonian lengths differ");
+      // physical dimensions first, so that the start energy below can be evaluated (findGroundState
+      // repeats this fit, a no-op by then)
+      mps.adjustPhysDimensions(hmpo.getDimensions());
+    } else {
+      mps.setRandomState(cfg->seed);
+    }
     Contractor c(ctx);
     if (factory) c.setMultiplierFactory(*factory);
     c.setConvTol(cfg->convtol > 0 ? cfg->convtol : 1e-5);
@@ -147,6 +188,8 @@ int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo,
     out->refine_from = st.refine_from;
     out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
     for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+    out->residual_norm = -1.0;
+    if (cfg->want_residual) out->residual_norm = residual_norm(c, cfg, mps, E);
     if (export_path) mps.exportMPS(export_path);
     return MPSD_OK;
   } catch (const Error& e) {
@@ -250,6 +293,23 @@ int mpsd_gs_solve_mpo(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_
   }
 }
 
+int mpsd_gs_solve_from(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo_path, int format, const char* mps_in,
+                       const char* export_path, mpsd_gs_result* out) {
+  if (!ctx || !cfg || !mps_in || !out) return MPSD_ERR_ARG;
+  if (cfg->D < 0) return mpsd::set_err(ctx, MPSD_ERR_ARG, "gs_solve_from: need D >= 0");
+  try {
+    std::unique_ptr<Hamiltonian> model;
+    MPO file;
+    const MPO& H = load_hamiltonian(cfg, mpo_path, format, model, file);
+    return mpsdyn::gs_solve_mpo_with(ctx, cfg, H, export_path, nullptr, out, mps_in, mpo_path == nullptr);
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
 int mpsd_mpo_expectation(mpsd_ctx* ctx, const char* mpo_path, int format, const char* mps_path, double* re,
                          double* im) {
   if (!ctx || !mpo_path || !mps_path || !re || !im) return MPSD_ERR_ARG;
@@ -345,6 +405,25 @@ int mpsd_gs_sweep_part(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const char* mpo
                                       cfg->offset);
     if (energy) *energy = c.contract(init, H, init).real() / c.contract(init, init).real();
     if (mps_out) init.exportMPS(mps_out);
+    return MPSD_OK;
+  } catch (const Error& e) {
+    return fail(ctx, e);
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return MPSD_ERR_ARG;
+  }
+}
+
+int mpsd_mps_gauge_site(mpsd_ctx* ctx, const char* mps_in, int pos, char dir, int next, const char* mps_out) {
+  if (!ctx || !mps_in || !mps_out) return MPSD_ERR_ARG;
+  try {
+    MPS m(1, 1, 2, ctx);
+    m.importMPS(mps_in);
+    if (pos < 0 || pos >= m.getLength()) throw Error(MPSD_ERR_ARG, "mps_gauge_site: position out of range");
+    if (dir != 'L' && dir != 'l' && dir != 'R' && dir != 'r')
+      throw Error(MPSD_ERR_ARG, "mps_gauge_site: direction must be 'L' or 'R'");
+    m.gaugeCond(pos, dir, next != 0);
+    m.exportMPS(mps_out);
     return MPSD_OK;
   } catch (const Error& e) {
     return fail(ctx, e);

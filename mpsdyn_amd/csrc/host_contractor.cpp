@@ -19,6 +19,16 @@ Contractor& Contractor::theContractor() {
   return c;
 }
 
+void Contractor::adjustStart(const MPO& ops, int D, MPS& init) {
+  init.adjustPhysDimensions(ops.getDimensions());
+  if (D > 0) {
+    if (D > init.getBond())
+      init.increaseBondDimension(D);
+    else if (D < init.getBond())
+      init = MPS(init, D);
+  }
+}
+
 void Contractor::calculateL(int pos, TmpManager& mgr, const MPO& ops, const MPS& ket, const MPS& bra) {
   // Contractor::calculateL (Contractor.cpp:723-742), iterative instead of recursive.
   int first = pos;
@@ -49,26 +59,29 @@ void Contractor::calculateR(int pos, TmpManager& mgr, const MPO& ops, const MPS&
 }
 
 complex_t Contractor::contract(const MPS& ket, const MPO& ops, const MPS& bra) {
+  // left environments from the scalar 1 over every site, in two alternating slots
   const int L = ket.getLength();
-  TmpManager mgr(L + 1, ctx_);
-  for (int k = 1; k <= L; ++k)
+  const complex_t one(1.0, 0.0);
+  DevTensor e[2] = {DevTensor(ctx_, mwArray(std::vector<int>{1, 1, 1}, &one)),
+                    DevTensor(ctx_, std::vector<int>{1, 1, 1}, false)};
+  for (int k = 0; k < L; ++k)
     check(ctx_,
-          mpsd_env_left(ctx_, mgr.operL(k - 1).t, ket.getA(k - 1).device(), ops.getOp(k - 1).device(ctx_),
-                        bra.getA(k - 1).device(), mgr.operL(k).t),
+          mpsd_env_left(ctx_, e[k & 1].t, ket.getA(k).device(), ops.getOp(k).device(ctx_), bra.getA(k).device(),
+                        e[(k + 1) & 1].t),
           "contract");
-  mwArray r = mgr.operL(L).download();
+  mwArray r = e[L & 1].download();
   return r.getElement(0);
 }
 
 void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset, int knr,
                                  const std::string& tmpfile, int freqSv, bool useLM) {
   (void)knr;
-  (void)D;
   if (useLM) throw Error(MPSD_ERR_UNSUPPORTED, "findGroundState: useLM is not part of this path");
   const int L = ops.getLength();
   if (init.getLength() != L) throw Error(MPSD_ERR_SHAPE, "findGroundState: MPS and MPO lengths differ");
   if (init.ctx() != ctx_) throw Error(MPSD_ERR_ARG, "findGroundState: MPS lives on another context");
   stats_ = SolveStats();
+  adjustStart(ops, D, init);
   // Contractor.cpp:1371-1373: right then left gauge
   init.gaugeCond('R', true);
   init.gaugeCond('L', true);

@@ -277,6 +277,35 @@ std::vector<int> MPO::getDimensions() const {
   return d;
 }
 
+MPO mpo_product(const MPO& a, const MPO& b) {
+  const int L = a.getLength();
+  if (b.getLength() != L) throw Error(MPSD_ERR_SHAPE, "mpo_product: lengths differ");
+  MPO out(L);
+  for (int k = 0; k < L; ++k) {
+    const mwArray& A = a.getOp(k).getFullData();
+    const mwArray& B = b.getOp(k).getFullData();
+    const int du = A.getDimension(0), al = A.getDimension(1), dm = A.getDimension(2), ar = A.getDimension(3);
+    const int bl = B.getDimension(1), dd = B.getDimension(2), br = B.getDimension(3);
+    if (B.getDimension(0) != dm) throw Error(MPSD_ERR_SHAPE, "mpo_product: inner physical dimensions differ");
+    // P[u, a1 + al a2, v, b1 + ar b2] = sum_s A[u, a1, s, b1] B[s, a2, v, b2]
+    const int pl = al * bl, pr = ar * br;
+    mwArray P(std::vector<int>{du, pl, dd, pr});
+    for (int b2 = 0; b2 < br; ++b2)
+      for (int b1 = 0; b1 < ar; ++b1)
+        for (int v = 0; v < dd; ++v)
+          for (int a2 = 0; a2 < bl; ++a2)
+            for (int a1 = 0; a1 < al; ++a1)
+              for (int u = 0; u < du; ++u) {
+                complex_t acc = 0.0;
+                for (int s = 0; s < dm; ++s)
+                  acc += A.data()[u + du * (a1 + al * (s + dm * b1))] * B.data()[s + dm * (a2 + bl * (v + dd * b2))];
+                P.data()[u + du * ((a1 + al * a2) + pl * (v + dd * (b1 + ar * b2)))] = acc;
+              }
+    out.setOp(k, new Operator(P), true);
+  }
+  return out;
+}
+
 // ------------------------------------------------------------------------------ RNG
 static inline uint64_t splitmix(uint64_t& state) {
   uint64_t z = (state += 0x9E3779B97F4A7C15ull);
@@ -343,6 +372,17 @@ void Site::gaugeRInto(DevTensor& rightterm, Site& next) {
 
 void Site::gaugeLInto(DevTensor& leftterm, Site& prev) {
   check(ctx_, mpsd_site_gauge_l_absorb(ctx_, dev_.t, leftterm.t, prev.dev_.t), "gaugeL + reduceDimL");
+}
+
+void Site::changeDimensions(int d, int Dl, int Dr) {
+  std::vector<int> cur = dev_.dims();
+  if (cur.size() < 3) {
+    // a lower-rank buffer (e.g. a vector set by the eigensolver) is a [d, Dl, Dr] site
+    const std::vector<int> full = dims();
+    check(ctx_, mpsd_tensor_reshape(dev_.t, 3, full.data()), "Site reshape");
+  }
+  const int nd[3] = {d, Dl, Dr};
+  check(ctx_, mpsd_tensor_resize(dev_.t, 3, nd), "Site::changeDimensions");
 }
 
 // ------------------------------------------------------------------------------ MPS
@@ -432,6 +472,7 @@ MPS::MPS(const MPS& other, int D) : ctx_(other.ctx_), bond_(other.bond_), normfa
 void MPS::gaugeCond(int pos, char dir, bool next) {
   DevTensor& tmp = factor();
   const int L = getLength();
+  bool edge = false;  // the factor left over at the chain's end is the state's norm
   if (dir == 'L' || dir == 'l') {
     if (next && pos > 0) {
       A_[pos]->gaugeLInto(tmp, *A_[pos - 1]);
@@ -439,6 +480,7 @@ void MPS::gaugeCond(int pos, char dir, bool next) {
       const int oldDl = A_[pos]->getDl();
       A_[pos]->gaugeL(tmp);
       if (A_[pos]->getDl() != oldDl && pos > 0) A_[pos - 1]->reduceDimL(tmp);
+      edge = next && pos == 0;
     }
   } else {
     if (next && pos < L - 1) {
@@ -447,9 +489,66 @@ void MPS::gaugeCond(int pos, char dir, bool next) {
       const int oldDr = A_[pos]->getDr();
       A_[pos]->gaugeR(tmp);
       if (pos < L - 1 && A_[pos]->getDr() != oldDr) A_[pos + 1]->reduceDimR(tmp);
+      edge = next && pos == L - 1;
     }
   }
+  if (edge) {
+    // MPS.cpp:380-387, 402-409: the 1x1 factor's magnitude goes into the norm factor (its
+    // phase stayed in the edge site)
+    mwArray f = tmp.download();
+    if (f.getNrComponents() != 1) throw Error(MPSD_ERR_SHAPE, "MPS::gaugeCond: edge factor is not a scalar");
+    normfact_ *= std::abs(f.getElement(0));
+  }
   gauge_ = gNone;
+}
+
+void MPS::increaseBondDimension(int D) {
+  // MPS.cpp:650-719 with zero noise.  Bond k (0 < k < L) becomes min(D, prod_{j<k} d_j,
+  // prod_{j>=k} d_j): the reference's running products cumd (left) and cumdr (right, D until
+  // the first site from the right where it drops below D) give the same minimum.  The outer
+  // bonds keep their dimensions.
+  if (D <= bond_) return;
+  const int L = getLength();
+  if (L == 0) return;
+  std::vector<int64_t> left(L + 1, 1), right(L + 1, 1);
+  for (int k = 0; k < L; ++k) left[k + 1] = std::min<int64_t>(left[k] * A_[k]->getd(), D);
+  for (int k = L - 1; k >= 0; --k) right[k] = std::min<int64_t>(right[k + 1] * A_[k]->getd(), D);
+  std::vector<int> bd(L + 1);
+  for (int k = 1; k < L; ++k) bd[k] = (int)std::min<int64_t>(std::min<int64_t>(left[k], right[k]), D);
+  bd[0] = A_[0]->getDl();
+  bd[L] = A_[L - 1]->getDr();
+  int maxbond = 1;
+  for (int k = 0; k < L; ++k) {
+    A_[k]->increaseBondDim(bd[k], bd[k + 1]);
+    maxbond = std::max(maxbond, std::max(bd[k], bd[k + 1]));
+  }
+  bond_ = maxbond;
+  gauge_ = gNone;
+}
+
+void MPS::adjustPhysDimensions(const std::vector<int>& d) {
+  const int L = getLength();
+  if ((int)d.size() != L)
+    throw Error(MPSD_ERR_SHAPE, "MPS::adjustPhysDimensions: " + std::to_string(d.size()) + " dimensions for " +
+                                    std::to_string(L) + " sites");
+  bool changed = false;
+  for (int k = 0; k < L; ++k) {
+    if (d[k] < 1) throw Error(MPSD_ERR_ARG, "MPS::adjustPhysDimensions: physical dimension below 1");
+    if (d[k] > A_[k]->getd()) {
+      A_[k]->increasePhysDim(d[k]);
+      changed = true;
+    } else if (d[k] < A_[k]->getd()) {
+      A_[k]->decreasePhysDim(d[k]);
+      changed = true;
+    }
+  }
+  if (changed) {
+    // MPS.cpp:749-754: the bond profile is rebuilt for the new dimensions
+    const int D = bond_;
+    bond_ = 1;
+    gauge_ = gNone;
+    increaseBondDimension(D);
+  }
 }
 
 void MPS::exportMPS(const char* filename) const {

@@ -196,10 +196,10 @@ static void check_part_range(int pos0, int lim, int L) {
 }
 
 void Contractor::sweepPart(int pos0, int lim, const MPO& ops, int D, MPS& init, double offset, int knr) {
-  (void)D;
   (void)knr;
   const int L = ops.getLength();
   checkLevels(ops, init, {});
+  adjustStart(ops, D, init);
   check_part_range(pos0, lim, L);
   stats_ = SolveStats();
   const bool right = lim >= pos0;
@@ -270,8 +270,9 @@ void Contractor::penaltySweepPart(int pos0, int lim, const MPO& ops, const std::
 void Contractor::sweepPartWithProjectorPenalty(int pos0, int lim, const MPO& ops, int D,
                                                const std::vector<double>& weights, const std::vector<MPS*>& projected,
                                                MPS& init, double offset, int knr) {
-  (void)D;
   (void)knr;
+  checkLevels(ops, init, projected);
+  adjustStart(ops, D, init);
   penaltySweepPart(pos0, lim, ops, weights, projected, init, offset, false);
 }
 
@@ -286,8 +287,9 @@ void Contractor::sweepPartNextExcitedState(int pos0, int lim, const MPO& ops, in
 
 void Contractor::sweepPartWithProjPenalty(int pos0, int lim, const MPO& ops, int D, double mu, MPS& projected,
                                           MPS& init, double offset, int knr) {
-  (void)D;
   (void)knr;
+  checkLevels(ops, init, {&projected});
+  adjustStart(ops, D, init);
   penaltySweepPart(pos0, lim, ops, {-mu}, {&projected}, init, offset, true);
 }
 
@@ -295,12 +297,12 @@ void Contractor::findGroundStateWithProjectorPenalty(const MPO& ops, int D, cons
                                                      const std::vector<MPS*>& projected, double* lambdak, MPS& init,
                                                      double offset, int knr, const std::string& tmpfile, int maxTime,
                                                      bool useLM) {
-  (void)D;
   (void)knr;
   if (useLM) throw Error(MPSD_ERR_UNSUPPORTED, "findGroundStateWithProjectorPenalty: useLM is not part of this path");
   if (penalties.size() != projected.size())
     throw Error(MPSD_ERR_ARG, "findGroundStateWithProjectorPenalty: different number of projectors and penalties");
   checkLevels(ops, init, projected);
+  adjustStart(ops, D, init);
   const int L = ops.getLength();
   stats_ = SolveStats();
   init.gaugeCond('R', true);

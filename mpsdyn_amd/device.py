@@ -121,6 +121,10 @@ def _optional(lib):
                                    C.POINTER(D), I, I, C.c_char_p, C.POINTER(D)]),
         "mpsd_gs_solve_mpo": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, C.POINTER(GsResult)]),
         "mpsd_mpo_expectation": (I, [P, C.c_char_p, I, C.c_char_p, C.POINTER(D), C.POINTER(D)]),
+        "mpsd_gs_solve_from": (I, [P, C.POINTER(GsConfig), C.c_char_p, I, C.c_char_p, C.c_char_p,
+                                   C.POINTER(GsResult)]),
+        "mpsd_tensor_resize": (I, [P, I, C.POINTER(I)]),
+        "mpsd_mps_gauge_site": (I, [P, C.c_char_p, I, C.c_char, I, C.c_char_p]),
         "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
         "mpsd_heff_apply_grouped": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "mpsd_gs_footprint": (I64, [C.POINTER(GsConfig)]),
@@ -167,7 +171,8 @@ class GsConfig(C.Structure):
     """mpsd_gs_config (include/mpsd_gs.h)."""
     _fields_ = [("model", C.c_int), ("L", C.c_int), ("D", C.c_int), ("p", C.c_double * 4),
                 ("offset", C.c_double), ("convtol", C.c_double), ("eigtol", C.c_double),
-                ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64), ("solver", C.c_int)]
+                ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64), ("solver", C.c_int),
+                ("want_residual", C.c_int)]
 
 
 class GsResult(C.Structure):
@@ -176,7 +181,7 @@ class GsResult(C.Structure):
                 ("matvecs", C.c_int64), ("env_updates", C.c_int64), ("seconds", C.c_double),
                 ("residual", C.c_double), ("device_bytes", C.c_int64), ("matvecs_lowp", C.c_int64),
                 ("refine_from", C.c_int), ("n_history", C.c_int),
-                ("history", C.c_double * 256)]
+                ("history", C.c_double * 256), ("residual_norm", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -246,10 +251,10 @@ SOLVER_LANCZOS, SOLVER_FULLEIG = 0, 1
 
 
 def gs_config(model, L, D, p, convtol=1e-10, eigtol=0.0, max_halfsweeps=0, precision=PREC_FP64,
-              seed=1, offset=0.0, solver=SOLVER_LANCZOS):
+              seed=1, offset=0.0, solver=SOLVER_LANCZOS, want_residual=False):
     pp = list(p) + [0.0] * (4 - len(p))
     return GsConfig(model, L, D, (C.c_double * 4)(*pp), offset, convtol, eigtol, max_halfsweeps,
-                    precision, seed, solver)
+                    precision, seed, solver, 1 if want_residual else 0)
 
 
 def gs_solve(ctx, cfg, export_path=None):
@@ -285,6 +290,16 @@ def gs_solve_mpo(ctx, cfg, mpo_path, fmt=MPO_BINARY, export_path=None):
     return res
 
 
+def gs_solve_from(ctx, cfg, mps_in, mpo_path=None, fmt=MPO_BINARY, export_path=None):
+    """findGroundState from the exported MPS mps_in, fitted to the Hamiltonian and to cfg.D
+    (grown by zero padding or cut; D = 0 keeps its bond).  Returns GsResult."""
+    res = GsResult()
+    ctx.check(ctx.lib.mpsd_gs_solve_from(ctx.handle, C.byref(cfg), None if mpo_path is None else str(mpo_path).encode(),
+                                         fmt, str(mps_in).encode(),
+                                         None if export_path is None else str(export_path).encode(), C.byref(res)))
+    return res
+
+
 def mpo_expectation(ctx, mpo_path, fmt, mps_path):
     """<psi|H|psi> of an exported MPS for the MPO in an MPO file."""
     re_, im_ = C.c_double(), C.c_double()
@@ -315,6 +330,12 @@ def gs_sweep_part(ctx, cfg, mps_in, pos0, lim, mps_out=None, proj=(), weights=()
                                          fmt, str(mps_in).encode(), n, paths, w, pos0, lim,
                                          None if mps_out is None else str(mps_out).encode(), C.byref(e)))
     return e.value
+
+
+def mps_gauge_site(ctx, mps_in, pos, direction, nxt, mps_out):
+    """MPS::gaugeCond(pos, dir, next) on an exported state, written to mps_out."""
+    ctx.check(ctx.lib.mpsd_mps_gauge_site(ctx.handle, str(mps_in).encode(), int(pos), direction.encode(),
+                                          1 if nxt else 0, str(mps_out).encode()))
 
 
 def mps_truncate(ctx, mps_in, D, mps_out):
@@ -481,6 +502,11 @@ class Tensor:
 
     def devptr(self):
         return self.ctx.lib.mpsd_tensor_devptr(self._h)
+
+    def resize(self, dims):
+        """mwArray::resize with zero noise: keep elements by multi-index, zero-pad, cut."""
+        d = (C.c_int * len(dims))(*dims)
+        self.ctx.check(self.ctx.lib.mpsd_tensor_resize(self._h, len(dims), d))
 
     def upload(self, arr):
         buf = np.ascontiguousarray(np.reshape(np.asarray(arr, dtype=np.complex128), -1, order="F"))

@@ -64,7 +64,8 @@ __all__ = [
     "chain_mpo", "mps_write", "mps_read", "overlap_left", "overlap_right", "effective_bra",
     "proj_multi_product", "proj_multi_full", "sweep_part", "penalty_sweep_part",
     "find_ground_state_with_projector_penalty", "find_next_excited_state", "overlap",
-    "svd_keep", "mps_truncate",
+    "svd_keep", "mps_truncate", "resize", "increase_bond_dimension", "adjust_phys_dimensions",
+    "fit_start", "gauge_cond_site", "mpo_product", "residual_norm",
 ]
 
 
@@ -571,14 +572,34 @@ def expectation(mps, mpo):
     return complex(e.reshape(-1)[0])
 
 
-def find_ground_state(mpo, mps, convtol=1e-12, offset=0.0, max_halfsweeps=200, dense_below=801):
+def mpo_product(a, b):
+    """The MPO of the operator product A B: P[u, a1 + Dl_A a2, v, b1 + Dr_A b2] =
+    sum_s A[u, a1, s, b1] B[s, a2, v, b2] per site (column-major merged bonds)."""
+    out = []
+    for A, B in zip(a, b):
+        du, al, _dm, ar = A.shape
+        _, bl, dd, br = B.shape
+        P = np.einsum("uasb,scvd->uacvbd", A, B)        # [u, a1, a2, v, b1, b2]
+        out.append(P.reshape((du, al * bl, dd, ar * br), order="F"))
+    return out
+
+
+def residual_norm(mps, shifted_mpo):
+    """||(H - E) psi|| for a unit-norm psi, given the MPO of H - E: sqrt(<psi|(H-E)^2|psi>)."""
+    v = expectation(mps, mpo_product(shifted_mpo, shifted_mpo)).real
+    return math.sqrt(max(v, 0.0))
+
+
+def find_ground_state(mpo, mps, convtol=1e-12, offset=0.0, max_halfsweeps=200, dense_below=801, D=0):
     """Single-site ground-state sweep.  Restates Contractor::findGroundState
-    (src/core/cpp/Contractor.cpp:1352-1533): right-canonicalise, sweep 0..L-1, L-2..0, ...
-    solving the lowest eigenpair of Heff at each site (warm start from the current site),
-    gauge-moving the centre with QR/LQ (MPS::gaugeCond(pos,dir,true), MPS.cpp:366-419) and
-    stopping when |E_prev - E| < convtol |E + offset| at the end of a half-sweep.
+    (src/core/cpp/Contractor.cpp:1352-1533): fit the start state to the MPO and to D
+    (fit_start), right-canonicalise, sweep 0..L-1, L-2..0, ... solving the lowest eigenpair of
+    Heff at each site (warm start from the current site), gauge-moving the centre with QR/LQ
+    (MPS::gaugeCond(pos,dir,true), MPS.cpp:366-419) and stopping when
+    |E_prev - E| < convtol |E + offset| at the end of a half-sweep.
     Returns (energy, mps, n_halfsweeps, energies_per_halfsweep)."""
     Ls = len(mps)
+    mps, _ = fit_start(mpo, mps, D)
     mps = gauge_left_all(gauge_right_all(mps))
     envL, envR = environments(mps, mpo)
     energy = expectation(mps, mpo).real
@@ -704,6 +725,111 @@ def _gauge_site(mps, pos, direction, nxt):
         mps[pos] = q
         if pos > 0 and (q.shape[1] != old or nxt):
             mps[pos - 1] = _absorb_left(mps[pos - 1], lm)
+
+
+def gauge_cond_site(mps, pos, direction, nxt, normfact=1.0):
+    """MPS::gaugeCond(pos, dir, next) (MPS.cpp:366-419) with its edge normalisation: after
+    gaugeL at site 0 or gaugeR at the last site with next set, the 1x1 factor's magnitude
+    multiplies the norm factor (MPS.cpp:380-387, 402-409).  Returns (sites, normfact)."""
+    mps = [a.copy() for a in mps]
+    L = len(mps)
+    A = mps[pos]
+    if direction == "R":
+        q, f = _qr_site(A)
+        edge = nxt and pos == L - 1
+    else:
+        q, f = _lq_site(A)
+        edge = nxt and pos == 0
+    if edge:
+        if f.size != 1:
+            raise ValueError("edge factor is not a scalar")
+        normfact = normfact * abs(f[0, 0])
+    _gauge_site(mps, pos, direction, nxt)
+    return mps, normfact
+
+
+# --------------------------------------------------------------------------- bond and physical dimensions
+
+def resize(a, dims):
+    """mwArray::resize with zero noise (wrapper/mwArray.cpp:1604-1670): the element at each
+    multi-index inside both shapes is kept, new ones are zero, the rest are cut."""
+    a = np.asarray(a, dtype=np.complex128)
+    out = np.zeros(tuple(dims), dtype=np.complex128)
+    box = tuple(slice(0, min(o, n)) for o, n in zip(a.shape, dims))
+    out[box] = a[box]
+    return out
+
+
+def increase_bond_dimension(mps, D, bond):
+    """MPS::increaseBondDimension(D) with zero noise (MPS.cpp:650-719), restated loop for
+    loop: boundR is the first site from the right where the running right product reaches D;
+    cumd / cumdr are the running left / right products of the physical dimensions; every site
+    is resized (Site::increaseBondDimWithNoise -> mwArray::resize).  Returns (sites, bond)."""
+    if D <= bond:
+        return [a.copy() for a in mps], bond
+    L = len(mps)
+    cumd, cumdr_last, bound_r = 1, 1, L - 1
+    while bound_r > 0:
+        dim = mps[bound_r].shape[0]
+        if cumdr_last * dim < D:
+            bound_r -= 1
+            cumdr_last *= dim
+        else:
+            break
+    cumdr = D if bound_r > 0 else cumdr_last * mps[0].shape[0]
+    out, maxbond = [], 1
+    for k in range(L):
+        d, oDl, oDr = mps[k].shape
+        if k == 0:
+            nDl = oDl
+        else:
+            nDl = min(cumd, cumdr) if (cumd < D or cumdr < D) else D
+        if cumd < D:
+            cumd *= d
+        if k == bound_r:
+            cumdr = cumdr_last
+        elif k > bound_r:
+            cumdr //= d
+        if k == L - 1:
+            nDr = oDr
+        else:
+            nDr = min(cumd, cumdr) if (cumd < D or cumdr < D) else D
+        out.append(resize(mps[k], (d, nDl, nDr)))
+        maxbond = max(maxbond, nDl, nDr)
+    return out, maxbond
+
+
+def adjust_phys_dimensions(mps, dvec, bond):
+    """MPS::adjustPhysDimensions (MPS.cpp:722-756): site k resized to physical dimension
+    dvec[k] (Site::increasePhysDim / decreasePhysDim, Site.cpp:545-559); when any site changed,
+    the bond profile is rebuilt by increaseBondDimension(bond) from bond 1.  Returns
+    (sites, bond)."""
+    if len(dvec) != len(mps):
+        raise ValueError("one physical dimension per site")
+    out, changed = [], False
+    for A, dk in zip(mps, dvec):
+        if dk != A.shape[0]:
+            A = resize(A, (dk, A.shape[1], A.shape[2]))
+            changed = True
+        out.append(A.copy())
+    if changed:
+        return increase_bond_dimension(out, bond, 1)
+    return out, bond
+
+
+def fit_start(mpo, mps, D, bond=None):
+    """The start state of findGroundState (Contractor.cpp:1360-1370): physical dimensions from
+    the MPO, then for D > 0 grown by zero padding or cut by mps_truncate (MPS(init, D)).
+    Returns (sites, bond)."""
+    bond = max(max(a.shape[1], a.shape[2]) for a in mps) if bond is None else bond
+    mps, bond = adjust_phys_dimensions(mps, [W.shape[0] for W in mpo], bond)
+    if D > 0:
+        if D > bond:
+            mps, bond = increase_bond_dimension(mps, D, bond)
+        elif D < bond:
+            mps, _ = mps_truncate(mps, D)
+            bond = min(bond, max(max(a.shape[1], a.shape[2]) for a in mps))
+    return mps, bond
 
 
 def _envs_at(mps, mpo, pos):

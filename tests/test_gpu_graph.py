@@ -60,8 +60,9 @@ def test_solve_graph_equals_stream():
     their arithmetic; tolerance 1e-12 relative)."""
     from mpsdyn_amd import device as dev
     cfgs = [dev.gs_config(dev.MODEL_ISING, 20, 20, [-1.0, -1.0, -0.5], convtol=1e-12),
-            dev.gs_config(dev.MODEL_SCHWINGER, 12, 32, [2.5, 100.0, 0.0, 0.0], convtol=1e-12),
+            dev.gs_config(dev.MODEL_SCHWINGER, 12, 64, [2.5, 100.0, 0.0, 0.0], convtol=1e-12),
             dev.gs_config(dev.MODEL_HEISENBERG, 16, 24, [1.0, 1.0, 0.8, 0.0], convtol=1e-12, precision=1)]
+    graph = []
     for cfg in cfgs:
         out = []
         for g in (0, 1):
@@ -72,8 +73,11 @@ def test_solve_graph_equals_stream():
             c.close()
         print(f"stream={out[0]:.16f} graph={out[1]:.16f} bitwise={out[0] == out[1]}", flush=True)
         assert abs(out[0] - out[1]) <= 1e-12 * abs(out[0])
+        graph.append(out[1])
+    # the graph path's Schwinger energy against exact diagonalisation (D = 64 is exact for 12 sites)
     e_ed = ph.ground_energy(ph.schwinger_hamiltonian(12, 2.5, 100.0))
-    assert abs(out[1] - e_ed) > -1  # shape check of the loop; physics pinned elsewhere
+    print(f"schwinger graph {graph[1]:.15f} ED {e_ed:.15f}", flush=True)
+    assert abs(graph[1] - e_ed) <= 1e-9 * abs(e_ed)
 
 
 def test_fused_solver_sweeps_match():

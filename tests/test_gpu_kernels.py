@@ -98,6 +98,57 @@ def test_env_updates_match_oracle(ctx, Dl, Dr, wl, wr):
     assert _rel(out.download(), orc.contract_right(R, ket, W, bra)) <= TOL
 
 
+@pytest.mark.parametrize("D,w", [(256, 3), (512, 3), (256, 5), (512, 5), (400, 5)])
+def test_env_updates_full_size(ctx, D, w):
+    """Environment updates at the sweep shapes the bench times (D=512 w=3, Heisenberg D=512
+    w=5, Schwinger D=400 w=5) against Operator::contractleftop / contractrightop."""
+    from mpsdyn_amd import device as dev
+    print(f"env full-size D={D} w={w}", flush=True)
+    d = 2
+    L = orc.complex_gaussian(21, (D, w, D))
+    R = orc.complex_gaussian(22, (D, w, D))
+    W = orc.complex_gaussian(23, (d, w, d, w))
+    ket = orc.complex_gaussian(24, (d, D, D))
+    bra = orc.complex_gaussian(25, (d, D, D))
+    tk, tb, tW = ctx.tensor(ket), ctx.tensor(bra), ctx.tensor(W)
+    out = ctx.tensor(dims=(1,))
+    dev.env_left(ctx, ctx.tensor(L), tk, tW, tb, out)
+    assert out.dims == (D, w, D)
+    assert _rel(out.download(), orc.contract_left(L, ket, W, bra)) <= TOL
+    dev.env_right(ctx, ctx.tensor(R), tk, tW, tb, out)
+    assert out.dims == (D, w, D)
+    assert _rel(out.download(), orc.contract_right(R, ket, W, bra)) <= TOL
+
+
+@pytest.mark.parametrize("on_device", [0, 1])
+def test_heff_apply_leading_dimensions(ctx, on_device):
+    """A block of 3 vectors with ldx = n + 5, ldy = ny + 7, as PRIMME's matrixMatvec hands them
+    over (eigs.cpp:296-302): each column equals the oracle's product and the padding between
+    columns of y is left untouched."""
+    from mpsdyn_amd import device as dev
+    L, W, R, _ = _operands(24, 30, 3, 5, seed=13, Dk=20, Dkr=26)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R))
+    n, ny = 2 * 20 * 26, 2 * 24 * 30
+    ldx, ldy, nvec = n + 5, ny + 7, 3
+    X = np.full((ldx, nvec), 9.0 - 4.0j, order="F")
+    for j in range(nvec):
+        X[:n, j] = orc.complex_gaussian(60 + j, (n,))
+    sentinel = -123.25 + 7.5j
+    Y = np.full((ldy, nvec), sentinel, dtype=np.complex128, order="F")
+    if on_device:
+        tx = ctx.tensor(np.reshape(X, -1, order="F"))
+        ty = ctx.tensor(np.reshape(Y, -1, order="F"))
+        h.apply_device(tx.devptr(), ty.devptr(), nvec=nvec, ldx=ldx, ldy=ldy)
+        Y = np.reshape(ty.download(), (ldy, nvec), order="F")
+    else:
+        xb = np.asfortranarray(X)
+        ctx.check(ctx.lib.mpsd_heff_apply(h._h, xb.ctypes.data, Y.ctypes.data, nvec, ldx, ldy, 0))
+    for j in range(nvec):
+        assert _rel(Y[:ny, j], orc.heff_product(L, W, R, X[:n, j])) <= TOL
+        assert np.all(Y[ny:, j] == sentinel)
+    h.close()
+
+
 def test_heff_full_matches_oracle(ctx):
     from mpsdyn_amd import device as dev
     L, W, R, _ = _operands(10, 20, 3, 3, seed=9)
@@ -119,9 +170,10 @@ def test_shape_errors_are_reported(ctx):
         dev.env_left(ctx, ctx.tensor(L), ctx.tensor(ket), ctx.tensor(W), ctx.tensor(ket), ctx.tensor(dims=(1,)))
 
 
-@pytest.mark.parametrize("D,w", [(256, 3), (512, 3), (512, 5)])
+@pytest.mark.parametrize("D,w", [(256, 3), (512, 3), (512, 5), (256, 5), (400, 5)])
 def test_heff_full_size(ctx, D, w):
-    """Real DMRG shapes (D=256/512 Ising w=3, Heisenberg w=5) against the oracle,
+    """Real DMRG shapes (D=256/512 Ising w=3, Heisenberg w=5, Schwinger D=256/400 w=5) against
+    the oracle,
     plus Hermiticity <u, H v> = <H u, v> for a Hermitian-structured operator."""
     from mpsdyn_amd import device as dev
     print(f"full-size D={D} w={w}", flush=True)

@@ -26,7 +26,7 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 1e-4)])
 @pytest.mark.parametrize("Db,Dbr,wl,wr", [(20, 20, 3, 3), (37, 50, 5, 5), (1, 4, 1, 3), (128, 128, 5, 5),
-                                          (512, 512, 3, 3)])
+                                          (512, 512, 3, 3), (256, 256, 5, 5), (400, 400, 5, 5)])
 def test_lowp_matvec_accuracy(ctx, prec, tol, Db, Dbr, wl, wr):
     from mpsdyn_amd import device as dev
     print(f"prec={prec} Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
@@ -84,3 +84,29 @@ def test_mixed_precision_ising_closed_form(ctx):
     r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, L, 32, [-1.0, -1.0, 0.0], convtol=1e-12, precision=1))
     e = ph.tfim_free_fermion(L, 1.0, 1.0)
     assert abs(r.energy - e) <= 1e-10 * abs(e)
+
+
+def test_schwinger_L100_precision_forms():
+    """Config 3 at its size: Schwinger L = 100, x = 100, m/g = 0.125 (mu = 2 (m/g) sqrt(x) = 2.5,
+    schwingerPD.cpp:51), zero background field, D = 128.  The complex-FP32 and split-BF16 forms
+    (FP64 orthogonalisation, Rayleigh-Ritz, gauge and refinement) reach the FP64 energy to 1e-10
+    relative, and their states' residuals ||H psi - E psi|| are those of the FP64 state (the D =
+    128 truncation, not the arithmetic, sets them: within 50%)."""
+    from mpsdyn_amd import device as dev
+    p = [2 * 0.125 * np.sqrt(100.0), 100.0, 0.0, 0.0]
+    c = dev.Context(0)
+    out = {}
+    for prec in (0, 1, 2):
+        r = dev.gs_solve(c, dev.gs_config(dev.MODEL_SCHWINGER, 100, 128, p, convtol=1e-10, precision=prec,
+                                          want_residual=True))
+        out[prec] = r
+        print(f"prec={prec} E={r.energy:.12f} residual={r.residual_norm:.3e} halfsweeps={r.halfsweeps} "
+              f"mv={r.matvecs} lowp={r.matvecs_lowp} {r.seconds:.2f}s", flush=True)
+    c.close()
+    e64, res64 = out[0].energy, out[0].residual_norm
+    assert 0.0 < res64 < 1e-2
+    for prec in (1, 2):
+        r = out[prec]
+        assert abs(r.energy - e64) <= 1e-10 * abs(e64)
+        assert abs(r.residual_norm - res64) <= 0.5 * res64
+        assert r.matvecs_lowp > 0

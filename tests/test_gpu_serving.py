@@ -88,6 +88,31 @@ def test_served_results_equal_alone(ctx):
     assert abs(res[2].energy - e_ed) <= 1e-10 * abs(e_ed)
 
 
+@pytest.mark.parametrize("batching", [False, True])
+def test_large_request_served_equals_alone(ctx, batching):
+    """A config-sized request (Ising L = 100, D = 256) served next to a Schwinger L = 100,
+    D = 128 one and two small ones, with and without grouped matvecs: each result equals the
+    same solve alone (1e-12 relative; the paths are built to be bit-identical)."""
+    from mpsdyn_amd import device as dev
+    cfgs = [dev.gs_config(dev.MODEL_ISING, 100, 256, [-1.0, -0.8, -0.2], convtol=1e-10, seed=21),
+            dev.gs_config(dev.MODEL_SCHWINGER, 100, 128, [2.5, 100.0, 0.0, 0.0], convtol=1e-10, seed=22),
+            dev.gs_config(dev.MODEL_HEISENBERG, 24, 32, [1.0, 1.0, 0.8, 0.0], convtol=1e-10, seed=23),
+            dev.gs_config(dev.MODEL_ISING, 40, 64, [-1.0, -1.1, 0.0], convtol=1e-10, seed=24, precision=1)]
+    alone = [dev.gs_solve(ctx, c).energy for c in cfgs]
+    srv = dev.Server(slots=len(cfgs), batching=batching, window_us=300)
+    res = srv.run_trace(cfgs, [0.0, 0.05, 0.1, 0.15])
+    st = srv.stats()
+    srv.close()
+    print(f"batching={batching} batches={st.batches} max_batch={st.max_batch} peak_running={st.peak_running}",
+          flush=True)
+    for a, r in zip(alone, res):
+        assert r.status == 0
+        print(f"alone={a:.16f} served={r.energy:.16f} bitwise={a == r.energy} "
+              f"latency={r.finish_s - r.arrival_s:.2f}s", flush=True)
+        assert abs(a - r.energy) <= 1e-12 * abs(a)
+    assert st.peak_running >= 2  # the solves overlapped on the device
+
+
 def test_admission_respects_budget():
     """With a budget that holds two of the requests, never more than two run at once and
     the admitted footprint stays within the budget; later arrivals wait, then finish."""
