@@ -158,6 +158,19 @@ int64_t mpsd_heff_size(const mpsd_heff* h);
  * and, for host buffers, returns after y has been written. */
 int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx,
                     int64_t ldy, int on_device);
+/* Allocation-free matvec, for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  y = Heff x + offset x in FP64 on device vectors x and y, with
+ * Heff = (envL, W, envR) as in mpsd_heff_create, on ctx's stream; the result is bit-identical to
+ * mpsd_heff_apply of a handle over the same tensors.  The MPO mix matrix and the intermediates
+ * (T1 or the partial products, and T2: about 2 d w D^2 complex numbers) live in one static
+ * 64 MiB device buffer per device, so no call allocates or frees (D = 512 with w = 3 fits;
+ * larger shapes, e.g. D = 512 with w = 5, return MPSD_ERR_UNSUPPORTED).  Every call on a device
+ * uses that device's one buffer: calls that overlap on one device (contexts on different streams)
+ * race and must be ordered by the caller.  Nothing in the buffer needs clearing between calls;
+ * each call overwrites what it reads. */
+int mpsd_heff_apply_noalloc(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* W, const mpsd_tensor* envR,
+                            double offset, const double* x, double* y);
 /* Grouped form: y_i = Heff_i x_i for n effective Hamiltonians of any shapes (independent
  * solves), in one sequence of grouped persistent launches on ctx's stream.  x_i / y_i are
  * device vectors.  Each result is bit-identical to mpsd_heff_apply of the same handle

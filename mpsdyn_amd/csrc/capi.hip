@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mpsd_internal.h"
@@ -121,18 +122,24 @@ static int arith_of(int precision) {
 // one-workgroup eigensolver's bound): one launch instead of three, same bits.
 static bool env_chain_fits(const mpsd_ctx* ctx, double flops) { return ctx->env_chain && flops <= 1.5e6; }
 
+// s0, s1: where T1 and T2 go (the context's grow-only scratch slots 0 and 1 when null).
 static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, const double2* X,
-                       int d, int Dkr, const double2* mix, int wr, int arith = 0, Chain* chain = nullptr) {
-  const size_t t1 = sizeof(double2) * (size_t)Db * wl * d * Dkr;
-  const size_t t2 = sizeof(double2) * (size_t)Db * d * wr * Dkr;
-  int rc = ensure_scratch(ctx, 0, t1);
-  if (rc) return rc;
-  rc = ensure_scratch(ctx, 1, t2);
-  if (rc) return rc;
+                       int d, int Dkr, const double2* mix, int wr, int arith = 0, Chain* chain = nullptr,
+                       double2* s0 = nullptr, double2* s1 = nullptr) {
+  if (!s0 || !s1) {
+    const size_t t1 = sizeof(double2) * (size_t)Db * wl * d * Dkr;
+    const size_t t2 = sizeof(double2) * (size_t)Db * d * wr * Dkr;
+    int rc = ensure_scratch(ctx, 0, t1);
+    if (rc) return rc;
+    rc = ensure_scratch(ctx, 1, t2);
+    if (rc) return rc;
+    s0 = (double2*)ctx->scratch[0];
+    s1 = (double2*)ctx->scratch[1];
+  }
   ZgemmDesc g;
   g.A = E;
   g.B = X;
-  g.C = (double2*)ctx->scratch[0];
+  g.C = s0;
   g.M = Db * wl;
   g.N = Dkr;
   g.K = Dk;
@@ -155,8 +162,8 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
     chain->step[0].g = g;
     ChainStep& mx = chain->step[1];
     mx.kind = 1;
-    mx.in = (const double2*)ctx->scratch[0];
-    mx.out = (double2*)ctx->scratch[1];
+    mx.in = s0;
+    mx.out = s1;
     mx.mix = mix;
     mx.Bx = Db;
     mx.nin = wl * d;
@@ -169,8 +176,7 @@ static int left_stages(mpsd_ctx* ctx, const double2* E, int Db, int wl, int Dk, 
                        [&] { return launch_zbrgemm_f64(g, tile_for(ctx, g), ctx->stream); });
   if (e != hipSuccess) return hip_err(ctx, e, "stage1 launch");
   e = timed(ctx, MPSD_KERNEL_MIX, [&] {
-    return launch_zmix_f64((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], mix, Db,
-                           wl * d, d * wr, Dkr, ctx->stream);
+    return launch_zmix_f64(s0, s1, mix, Db, wl * d, d * wr, Dkr, ctx->stream);
   });
   return hip_err(ctx, e, "mix launch");
 }
@@ -217,6 +223,35 @@ __global__ void __launch_bounds__(256) mix_perm_kernel(const double2* __restrict
     m[(sp + d * be) + (int64_t)d * wr * (al + wl * s)] = W[t];
   else
     m[(sp + d * al) + (int64_t)d * wl * (be + wr * s)] = W[t];
+}
+
+// Scratch of the allocation-free matvec (mpsd_heff_apply_noalloc): the MPO mix matrix, then T1
+// and the partial products, then T2, each starting on a 256-byte boundary.  A file-scope device
+// array, so each device holds its own copy for the life of the process and no call allocates.
+// Host code never uses the name `scratch` as an address: the device address comes from
+// hipGetSymbolAddress, once per device (noalloc_setup, called by mpsd_ctx_create).
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+__device__ __align__(256) double2 scratch[kNoAllocBytes / sizeof(double2)];
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device.
+hipError_t noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  noalloc_ptr[device] = p;
+  return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  return noalloc_ptr[device];
 }
 
 // Old and new dimensions of a resize (column-major, rank <= 8).
@@ -291,6 +326,12 @@ int mpsd_ctx_create(int device, mpsd_ctx** out) {
   mpsd_ctx* c = new mpsd_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MPSD_ERR_HIP;
+  }
+  // the allocation-free matvec's scratch address on this device (once per device)
+  if (noalloc_setup(device) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return MPSD_ERR_HIP;
   }
@@ -873,25 +914,38 @@ int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
   return MPSD_OK;
 }
 
-// One device matvec y = Heff x (+ offset x) on device pointers.
-static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double offset) {
+// One device matvec y = Heff x (+ offset x) on device pointers.  With s0, s1 set, T1 and the
+// partial products go to s0 and T2 to s1 (sized by the caller), and h->mix must already hold
+// the mix matrix: nothing is allocated.  Otherwise the context's scratch slots and W's cached
+// mix matrix are used (and grown or built on first use).
+static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double offset, double2* s0 = nullptr,
+                           double2* s1 = nullptr) {
   mpsd_ctx* ctx = h->ctx;
-  int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
+  const bool fixed = s0 && s1;
+  int rc = MPSD_OK;
+  if (!fixed) rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
   if (rc) return rc;
   const int arith = arith_of(h->precision);
-  rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr, arith);
+  rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr, arith, nullptr, s0, s1);
   if (rc) return rc;
+  if (!fixed) {
+    s0 = (double2*)ctx->scratch[0];
+    s1 = (double2*)ctx->scratch[1];
+  }
   const int64_t ny = (int64_t)h->d * h->Db * h->Dbr;
   // Split the last stage over the MPO bond when its plain grid would leave CUs idle: each
   // beta gives a partial product of the same shape as stage 1, summed in a fixed order.
   const bool split = stage3_split(h, ctx);
   if (split) {
-    rc = ensure_scratch(ctx, 0, sizeof(double2) * (size_t)(ny * h->wr));
-    if (rc) return rc;
-    double2* P = (double2*)ctx->scratch[0];  // T1 is dead once the mix has run
+    if (!fixed) {
+      rc = ensure_scratch(ctx, 0, sizeof(double2) * (size_t)(ny * h->wr));
+      if (rc) return rc;
+      s0 = (double2*)ctx->scratch[0];
+    }
+    double2* P = s0;  // T1 is dead once the mix has run
     // P[be][s' + d*b + d*Db*c] = sum_ar T2[b + Db*s' + Db*d*be + Db*d*wr*ar] R[c + Dbr*be + Dbr*wr*ar]
     ZgemmDesc g;
-    g.A = (const double2*)ctx->scratch[1];
+    g.A = s1;
     g.B = h->R->ptr;
     g.C = P;
     g.M = h->Db;
@@ -924,7 +978,7 @@ static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double of
   }
   // y[s', b, c] = sum_{k = be + wr*ar} T2[b + Db*s' + Db*d*k] R[c + Dbr*k]
   ZgemmDesc g;
-  g.A = (const double2*)ctx->scratch[1];
+  g.A = s1;
   g.B = h->R->ptr;
   g.C = y;
   g.M = h->Db;
@@ -984,6 +1038,58 @@ static int heff_apply_dev(mpsd_heff* h, const double2* x, double2* y) {
   const unsigned blocks = (unsigned)std::min<int64_t>(2048, (n + 255) / 256);
   hipLaunchKernelGGL(herm_combine_kernel, dim3(blocks), dim3(256), 0, ctx->stream, y, h->htmp, x, h->offset, n);
   return hip_err(ctx, hipGetLastError(), "heff hermitian combine");
+}
+
+int mpsd_heff_apply_noalloc(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* W, const mpsd_tensor* envR,
+                            double offset, const double* x, double* y) {
+  if (!ctx || !envL || !W || !envR || !x || !y) return set_err(ctx, MPSD_ERR_ARG, "heff_apply_noalloc: null argument");
+  if (!rank_ok(envL, 3) || !rank_ok(envR, 3) || !rank_ok(W, 4))
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_apply_noalloc: rank too high");
+  // a handle on the stack: the shapes and operands of mpsd_heff_create, no device memory
+  mpsd_heff h;
+  h.ctx = ctx;
+  h.L = envL;
+  h.W = W;
+  h.R = envR;
+  h.offset = offset;
+  h.precision = MPSD_PREC_FP64;
+  h.Db = dim(envL, 0);
+  h.wl = dim(envL, 1);
+  h.Dk = dim(envL, 2);
+  h.d = dim(W, 0);
+  h.Dbr = dim(envR, 0);
+  h.wr = dim(envR, 1);
+  h.Dkr = dim(envR, 2);
+  if (dim(W, 1) != h.wl || dim(W, 3) != h.wr || dim(W, 2) != h.d)
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_apply_noalloc: wrong dimensions: L " + dims_str(envL) + ", W " +
+                                            dims_str(W) + ", R " + dims_str(envR));
+  if (offset != 0.0 && (h.Db != h.Dk || h.Dbr != h.Dkr))
+    return set_err(ctx, MPSD_ERR_SHAPE, "heff_apply_noalloc: offset needs equal bra and ket dimensions");
+  auto up = [](int64_t n) { return (n + 15) & ~int64_t(15); };  // complex elements, 256-byte steps
+  const int64_t nmix = (int64_t)h.d * h.wl * h.d * h.wr;
+  const int64_t ny = (int64_t)h.d * h.Db * h.Dbr;
+  const int64_t t1 = (int64_t)h.Db * h.wl * h.d * h.Dkr;
+  const int64_t t2 = (int64_t)h.Db * h.d * h.wr * h.Dkr;
+  const bool split = stage3_split(&h, ctx);
+  const int64_t s0n = std::max(t1, split ? ny * h.wr : int64_t(0));
+  const int64_t need = (up(nmix) + up(s0n) + up(t2)) * (int64_t)sizeof(double2);
+  if (need > (int64_t)kNoAllocBytes)
+    return set_err(ctx, MPSD_ERR_UNSUPPORTED,
+                   "heff_apply_noalloc: needs " + std::to_string(need) + " bytes of scratch, above the " +
+                       std::to_string(kNoAllocBytes) + "-byte static buffer (e.g. D = 512 with w = 5): use "
+                       "mpsd_heff_apply");
+  double2* base = (double2*)noalloc_scratch(ctx->device);
+  if (!base) return set_err(ctx, MPSD_ERR_HIP, "heff_apply_noalloc: scratch address not set up on this device");
+  double2* mix = base;
+  double2* s0 = base + up(nmix);
+  double2* s1 = s0 + up(s0n);
+  // the mix matrix into the scratch (kind 0, as get_mix builds it for a matvec)
+  hipLaunchKernelGGL(mix_perm_kernel, dim3((unsigned)((nmix + 255) / 256)), dim3(256), 0, ctx->stream, W->ptr, h.d,
+                     h.wl, h.wr, 0, mix);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_err(ctx, e, "heff_apply_noalloc mix");
+  h.mix = mix;
+  return heff_apply_core(&h, (const double2*)x, (double2*)y, offset, s0, s1);
 }
 
 int mpsd_heff_apply(mpsd_heff* h, const double* x, double* y, int nvec, int64_t ldx, int64_t ldy,

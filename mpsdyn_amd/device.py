@@ -125,6 +125,7 @@ def _optional(lib):
                                    C.POINTER(GsResult)]),
         "mpsd_tensor_resize": (I, [P, I, C.POINTER(I)]),
         "mpsd_mps_gauge_site": (I, [P, C.c_char_p, I, C.c_char, I, C.c_char_p]),
+        "mpsd_heff_apply_noalloc": (I, [P, P, P, P, D, P, P]),
         "mpsd_heff_eigs_lowest": (I, [P, P, D, I, C.POINTER(D), C.POINTER(D), C.POINTER(I)]),
         "mpsd_heff_apply_grouped": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "mpsd_gs_footprint": (I64, [C.POINTER(GsConfig)]),
