@@ -20,15 +20,15 @@ Batched form (--batched N, default 64): N independent TFIM solves with seeded un
 L in [16, 64], g in [0.5, 1.5], h in [0, 0.5], D in {16, 32, 64, 128}, all submitted at once to
 the serving loop (16 slots); aggregate solves/s and matvecs/s.
 
-Served form (--served N, default 32): a seeded Poisson stream (--rate per second) of the
+Served form (--served N, default 24): a seeded Poisson stream (--rate per second, default 1) of the
 model mix 50% TFIM / 30% XXZ (Delta in [0.5, 1.5]) / 20% Schwinger (x in [25, 100], m/g in
 [0, 0.5]) with energy tolerance 1e-10, run once in FP64 and once in the mixed-precision form;
-solves/s and p50/p99 latency (arrival to result).  By default L is drawn from [20, 60]
-(Schwinger [40, 60]) and D from {32, 64, 128} so the run stays within minutes;
---full-serving uses the full ranges L in [20, 200] (Schwinger [40, 120]) and D up to 512.
+solves/s and p50/p99 latency (arrival to result).  By default L is drawn from [20, 200]
+(Schwinger [40, 120]) and D from {32, 64, 128, 256, 512} (the full ranges); --reduced-serving draws
+L from [20, 60] and D from {32, 64, 128}.
 
     python bench.py [--steps K] [--warmup W] [--D 512] [--w 3] [--cpu-seconds 15] [--no-cpu]
-                    [--gs-solves N] [--batched N] [--served N] [--rate R] [--full-serving]
+                    [--gs-solves N] [--batched N] [--served N] [--rate R] [--reduced-serving]
 """
 from __future__ import annotations
 
@@ -59,6 +59,16 @@ def load_traffic():
     """HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc and
     recorded under profiles/ (None when no measurement is committed)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def load_mfma():
+    """F64 MFMA MOPs per launch of the matvec BRGEMMs from the rocprofv3 --pmc run recorded under
+    profiles/ (None when absent)."""
+    p = os.path.join(ROOT, "profiles", "mfma.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -132,6 +142,91 @@ def ground_states(ctx, dev, n):
             "max_rel_energy_error_vs_reference": float(max(errs)), "matvecs_per_solve": mv / n}
 
 
+def cpu_ground_states(seconds):
+    """The oracle's sweep (the reference's findGroundState restated in numpy: dense local
+    eigensolves below local size 801, as Contractor::solveEigs switches) on the TFIM L=20,
+    D=20 chain, convtol 1e-12, on the host cores; solves/s over a bounded sample."""
+    from oracle import mpsd_oracle as orc
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("internal_api") == "openblas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    mpo = orc.ising_mpo(20, -1.0, -1.0, -0.5)
+    times, energies = [], []
+    t_end = time.perf_counter() + seconds
+    k = 0
+    while time.perf_counter() < t_end or k < 2:
+        t0 = time.perf_counter()
+        e = orc.find_ground_state(mpo, orc.random_mps(20, 20, seed=k + 1), convtol=1e-12)[0]
+        times.append(time.perf_counter() - t0)
+        energies.append(e)
+        k += 1
+    return {"value": round(len(times) / sum(times), 4), "unit": "solves/s", "cores": int(threads), "kind": "port",
+            "sample": f"{len(times)} TFIM L=20 D=20 solves (convtol 1e-12, dense local eigensolves as the reference "
+                      f"below local size 801), median {float(np.median(times)):.2f} s each, numpy + OpenBLAS "
+                      f"{threads} threads",
+            "p50_latency_s": round(float(np.median(times)), 3), "energy": float(energies[0])}
+
+
+def sweep_leg(dev, cases, halfsweeps):
+    """Ground-state sweeps at the real DMRG shape: L=100 chains from a seeded random state,
+    right-canonicalised, a fixed number of half-sweeps (convtol 1e-16 so none stops early),
+    with the phase breakdown on: environment updates, matvec kernels, the eigensolver's
+    vector work (Gram-Schmidt, coefficients, tridiagonal solve) and gauge steps."""
+    ctx = dev.Context(0)
+    out = []
+    for name, model, D, p in cases:
+        cfg = dev.gs_config(model, 100, D, p, convtol=1e-16, max_halfsweeps=halfsweeps, seed=1, breakdown=True)
+        t0 = time.perf_counter()
+        r = dev.gs_solve(ctx, cfg)
+        wall = time.perf_counter() - t0
+        vec = max(r.t_eig - r.t_matvec, 0.0)
+        parts = {"matvec": r.t_matvec, "eigensolver_vectors": vec, "env": r.t_env, "gauge": r.t_gauge}
+        tot = sum(parts.values())
+        out.append({
+            "workload": name, "D": D, "halfsweeps": r.halfsweeps, "wall_s": round(wall, 3),
+            "s_per_halfsweep": round(wall / max(r.halfsweeps, 1), 4), "energy": r.energy,
+            "energy_change_last_halfsweep": (abs(r.history[r.n_history - 1] - r.history[r.n_history - 2])
+                                             if r.n_history >= 2 else None),
+            "matvecs": r.matvecs, "env_updates": r.env_updates,
+            "breakdown_s": {k: round(v, 4) for k, v in parts.items()},
+            "breakdown_frac": {k: round(v / tot, 4) for k, v in parts.items()} if tot > 0 else None,
+            "matvec_tflops": round(r.matvec_flops / r.t_matvec / 1e12, 3) if r.t_matvec > 0 else None,
+            "env_tflops": round(r.env_flops / r.t_env / 1e12, 3) if r.t_env > 0 else None,
+            "env_ms_per_update": round(1e3 * r.t_env / max(r.env_updates, 1), 4),
+            "gauge_ms_per_site": round(1e3 * r.t_gauge / max(r.halfsweeps * 100, 1), 4),
+        })
+        print(json.dumps({"sweep_leg": out[-1]}), file=sys.stderr, flush=True)
+    ctx.close()
+    return out
+
+
+def precision_leg(dev, Ds):
+    """Precision forms on the Schwinger model, L=100, x=100, m/g=0.125 (mu = 2 (m/g) sqrt(x) =
+    2.5), zero background field, convtol 1e-10: FP64 against complex-FP32 and split-BF16
+    matvecs (FP64 eigensolver arithmetic, gauge and refinement); energy difference to the FP64
+    run, residual ||H psi - E psi|| of each final state, solves/s."""
+    ctx = dev.Context(0)
+    p = [2 * 0.125 * np.sqrt(100.0), 100.0, 0.0, 0.0]
+    out = {"workload": "schwinger_L100_x100_mg0.125", "runs": []}
+    for D in Ds:
+        e64 = None
+        for prec, name in ((0, "fp64"), (1, "mixed_fp32"), (2, "mixed_bf16x3")):
+            r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, 100, D, p, convtol=1e-10, precision=prec,
+                                                want_residual=True, seed=1))
+            if prec == 0:
+                e64 = r.energy
+            out["runs"].append({
+                "D": D, "form": name, "solves_per_s": round(1.0 / r.seconds, 4), "seconds": round(r.seconds, 3),
+                "energy": r.energy, "rel_energy_diff_vs_fp64": abs(r.energy - e64) / abs(e64),
+                "residual_norm": r.residual_norm, "halfsweeps": r.halfsweeps, "matvecs": r.matvecs,
+                "matvecs_lowp": r.matvecs_lowp})
+            print(json.dumps({"precision_leg": out["runs"][-1]}), file=sys.stderr, flush=True)
+    ctx.close()
+    return out
+
+
 def forms_table(dev, gaussian):
     """Matvec time per precision form (FP64, complex FP32, split-BF16) over interior-site
     shapes: one JSON line per (D, w, form), HIP-event times per kernel kind."""
@@ -189,27 +284,42 @@ def traffic_probe(dev, gaussian, D, w):
 
 
 def batched_form(dev, n):
-    """Batched form: n TFIM instances of uneven L, g, h, D submitted at once."""
+    """Batched form: n TFIM instances of uneven L, g, h, D submitted at once, in FP64 (with the
+    local-tolerance schedule) and in the mixed-precision form; aggregate solves/s, matvecs/s and
+    GFLOP/s (sum of the matvecs' algorithmic flops over the wall time)."""
     rng = np.random.default_rng(4)
-    cfgs = []
+    specs = []
     for k in range(n):
         L = int(rng.integers(16, 65))
         g, h = float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.0, 0.5))
         D = int(rng.choice([16, 32, 64, 128]))
-        cfgs.append(dev.gs_config(dev.MODEL_ISING, L, D, [-1.0, -g, -h], convtol=1e-10, seed=k + 1))
-    srv = dev.Server(slots=16, batching=False)
-    t0 = time.perf_counter()
-    res = srv.run_trace(cfgs, [0.0] * n)
-    wall = time.perf_counter() - t0
-    srv.close()
-    mv = sum(r.matvecs for r in res)
-    return {"workload": "tfim_x64_L16-64_D16-128_fp64", "solves": n, "solves_per_s": round(n / wall, 3),
-            "matvecs_per_s": round(mv / wall, 1), "wall_s": round(wall, 3),
-            "failed": sum(1 for r in res if r.status != 0)}
+        specs.append((L, D, [-1.0, -g, -h]))
+    out = {"workload": f"tfim_x{n}_L16-64_D16-128", "solves": n}
+    energies = {}
+    for name, prec in (("fp64", 0), ("mixed_fp32", 1)):
+        cfgs = [dev.gs_config(dev.MODEL_ISING, L, D, p, convtol=1e-10, seed=k + 1, precision=prec,
+                              tol_schedule=(prec == 0)) for k, (L, D, p) in enumerate(specs)]
+        srv = dev.Server(slots=16, batching=False)
+        t0 = time.perf_counter()
+        res = srv.run_trace(cfgs, [0.0] * n)
+        wall = time.perf_counter() - t0
+        srv.close()
+        mv = sum(r.matvecs for r in res)
+        fl = sum(r.matvec_flops for r in res)
+        energies[name] = [r.energy for r in res]
+        out[name] = {"solves_per_s": round(n / wall, 3), "matvecs_per_s": round(mv / wall, 1),
+                     "matvec_gflops_aggregate": round(fl / wall / 1e9, 2), "wall_s": round(wall, 3),
+                     "failed": sum(1 for r in res if r.status != 0)}
+    out["mixed_vs_fp64_max_rel_energy_diff"] = float(max(abs(a - b) / abs(a) for a, b in
+                                                          zip(energies["fp64"], energies["mixed_fp32"])))
+    out["note"] = "fp64 with the local-tolerance schedule; each request runs on its own stream (16 slots)"
+    return out
 
 
 def served_form(dev, n, rate, full):
-    """Served form: Poisson arrivals of a model mix; FP64 and mixed precision."""
+    """Served form: Poisson arrivals of a model mix; FP64 (local-tolerance schedule) and mixed
+    precision.  When the offered rate exceeds what the device sustains, solves/s is the
+    throughput at saturation and the latencies include queueing."""
     rng = np.random.default_rng(5)
     Lmax, Smax, Ds = (200, 120, [32, 64, 128, 256, 512]) if full else (60, 60, [32, 64, 128])
     cfgs, t = [], 0.0
@@ -230,10 +340,11 @@ def served_form(dev, n, rate, full):
             x = float(rng.uniform(25, 100))
             mg = float(rng.uniform(0.0, 0.5))
             cfgs.append((dev.MODEL_SCHWINGER, L, D, [2 * mg * np.sqrt(x), x, 0.0, 0.0]))
-    out = {"workload": ("poisson_mix_full" if full else "poisson_mix_L20-60_D32-128") + f"_rate{rate}",
-           "requests": n}
+    out = {"workload": ("poisson_mix_L20-200_D32-512" if full else "poisson_mix_L20-60_D32-128") + f"_rate{rate}",
+           "requests": n, "offered_rate": rate,
+           "D_counts": {str(D): sum(1 for c in cfgs if c[2] == D) for D in Ds}}
     for name, prec in (("fp64", 0), ("mixed_fp32", 1)):
-        gcfgs = [dev.gs_config(m, L, D, p, convtol=1e-10, seed=i + 1, precision=prec)
+        gcfgs = [dev.gs_config(m, L, D, p, convtol=1e-10, seed=i + 1, precision=prec, tol_schedule=(prec == 0))
                  for i, (m, L, D, p) in enumerate(cfgs)]
         srv = dev.Server(slots=16, batching=False)
         t0 = time.perf_counter()
@@ -241,6 +352,7 @@ def served_form(dev, n, rate, full):
         wall = time.perf_counter() - t0
         srv.close()
         lat = [r.finish_s - r.arrival_s for r in res]
+        print(json.dumps({"served_leg": name, "wall_s": round(wall, 2)}), file=sys.stderr, flush=True)
         out[name] = {"solves_per_s": round(n / wall, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
                      "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
                      "mean_queue_s": round(float(np.mean([r.admit_s - r.arrival_s for r in res])), 4),
@@ -262,9 +374,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gs-solves", type=int, default=8)
     ap.add_argument("--batched", type=int, default=64)
-    ap.add_argument("--served", type=int, default=32)
-    ap.add_argument("--rate", type=float, default=2.0)
-    ap.add_argument("--full-serving", action="store_true")
+    ap.add_argument("--served", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=1.0)
+    ap.add_argument("--reduced-serving", action="store_true",
+                    help="served mix over L in [20, 60], D in {32, 64, 128} instead of the full ranges")
+    ap.add_argument("--sweep-D", type=str, default="128,256",
+                    help="bond dimensions of the L=100 Ising sweep leg (comma list; empty: skip)")
+    ap.add_argument("--sweep-heisenberg-D", type=str, default="",
+                    help="bond dimensions of the L=100 Heisenberg XXZ sweep leg (comma list; empty: skip)")
+    ap.add_argument("--sweep-halfsweeps", type=int, default=20, help="half-sweeps of the sweep leg (10 sweeps)")
+    ap.add_argument("--precision-D", type=str, default="128",
+                    help="bond dimensions of the Schwinger L=100 precision leg (comma list; empty: skip)")
+    ap.add_argument("--cpu-gs-seconds", type=float, default=12.0,
+                    help="CPU ground-state baseline sample (TFIM L=20 D=20 oracle solves); 0: skip")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner (built-in tiles)")
     ap.add_argument("--forms-table", action="store_true",
                     help="only the per-precision-form matvec table (one JSON line per shape and form)")
@@ -350,6 +472,11 @@ def main():
     value = F / (ms_step * 1e-3) / 1e12
     dom_us = 1e3 * ms_dom / max(n_dom, 1)
     achieved = dom_flops / (dom_us * 1e-6) / 1e12
+    mfma = load_mfma()
+    mfma_exec = None
+    if mfma and mfma.get("workload", "").find(f"D={D}, w={w},") >= 0 and dom in mfma["kernels"]:
+        mops = mfma["kernels"][dom]["mfma_mops_f64_per_launch"]
+        mfma_exec = mops * mfma["flops_per_mop"] / (dom_us * 1e-6) / 1e12
     traffic = load_traffic()
     dom_traffic = None
     if traffic and traffic.get("workload", "").find(f"D={D}, w={w},") >= 0:
@@ -380,12 +507,20 @@ def main():
     env_tflops = F / (env["env_left_ms"] * 1e-3) / 1e12
 
     gs = ground_states(ctx, dev, args.gs_solves) if args.gs_solves > 0 else None
+    ints = lambda txt: [int(v) for v in txt.split(",") if v.strip()]
+    cases = [(f"ising_L100_g1_h0_D{D}", dev.MODEL_ISING, D, [-1.0, -1.0, 0.0]) for D in ints(args.sweep_D)]
+    cases += [(f"heisenberg_xxz_L100_delta1_D{D}", dev.MODEL_HEISENBERG, D, [1.0, 1.0, 1.0, 0.0])
+              for D in ints(args.sweep_heisenberg_D)]
+    sweeps = sweep_leg(dev, cases, args.sweep_halfsweeps) if cases else None
+    precision = precision_leg(dev, ints(args.precision_D)) if ints(args.precision_D) else None
     batched = batched_form(dev, args.batched) if args.batched > 0 else None
-    served = served_form(dev, args.served, args.rate, args.full_serving) if args.served > 0 else None
+    served = served_form(dev, args.served, args.rate, not args.reduced_serving) if args.served > 0 else None
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(L, W, R, x, F, args.cpu_seconds)
+        if args.cpu_gs_seconds > 0:
+            cpu["ground_states"] = cpu_ground_states(args.cpu_gs_seconds)
 
     out = {
         "metric": METRIC,
@@ -406,7 +541,12 @@ def main():
         "roofline": {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": dom_traffic},
+                     "traffic": dom_traffic,
+                     "mfma_executed_tflops": round(mfma_exec, 3) if mfma_exec else None,
+                     "mfma_pipe_frac": round(mfma_exec / PEAK_FP64_TFLOPS, 4) if mfma_exec else None,
+                     "mfma_note": "executed F64 MFMA flops = counter MOPs per launch x 512 (3/4 of the "
+                                  "algorithmic flops in the three-product complex form) over the live "
+                                  "launch time; frac counts algorithmic flops"},
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()} | {"pass": "pre-timed, every launch timed"},
         "dominant_us_timed_region": round(dom_us, 2),
         "tuned_tiles": [{"kind": r["kind"], "M": r["M"], "N": r["N"], "K": r["K"], "NB": r["NB"],
@@ -416,6 +556,8 @@ def main():
         "host_buffer_matvec": {"ms": round(host_ms, 4), "tflops": round(F / (host_ms * 1e-3) / 1e12, 3),
                                "note": "x and y in pageable host memory, PCIe both ways per call"},
         "gs": gs,
+        "sweeps_L100": sweeps,
+        "precision_forms": precision,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,

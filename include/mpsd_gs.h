@@ -35,6 +35,11 @@ typedef struct mpsd_gs_config {
   int solver;         /* MPSD_SOLVER_*: local eigensolver (Contractor::setEigenSolver) */
   int want_residual;  /* 1: also compute residual_norm = ||H psi - E psi|| of the result (built-in
                          models only); 0: not computed (residual_norm = -1) */
+  int breakdown;      /* 1: time the sweep's phases (t_env, t_eig, t_gauge, t_matvec of the result);
+                         the stream is drained at each phase boundary */
+  int tol_schedule;   /* 0: every local problem solved to eigtol (the reference's behaviour);
+                         1: FP64 local solves of a half-sweep to max(eigtol, min(1e-3, |dE|/|E| of the
+                         previous half-sweep)), loose while the sweep is far from converged */
 } mpsd_gs_config;
 
 #define MPSD_SOLVER_LANCZOS 0 /* device Lanczos on the matvec (in place of primme / arpack) */
@@ -58,6 +63,11 @@ typedef struct mpsd_gs_result {
   double residual_norm;   /* ||H psi - E psi|| of the final (unit) state when cfg->want_residual, computed
                              as sqrt(<psi|(H - E)^2|psi>) on the device over the MPO of (H - E)^2, with
                              -E spread over the sites' local terms; else -1 */
+  double t_env, t_eig, t_gauge; /* with cfg->breakdown: seconds in environment updates, local eigensolves
+                                   and gauge steps */
+  double t_matvec;        /* with cfg->breakdown: seconds of matvec kernels inside the eigensolves */
+  double matvec_flops;    /* algorithmic flops of all matvecs (8 per complex MAC, per site shape) */
+  double env_flops;       /* algorithmic flops of all environment updates */
 } mpsd_gs_result;
 
 /* Run one solve on ctx.  Returns MPSD_OK or an MPSD_ERR_* code (message: mpsd_last_error). */

@@ -43,6 +43,7 @@ typedef struct mpsd_serve_result {
   int64_t matvecs;
   int64_t matvecs_lowp;
   int64_t footprint_bytes;
+  double matvec_flops;  /* algorithmic flops of the solve's matvecs (mpsd_gs_result.matvec_flops) */
 } mpsd_serve_result;
 
 typedef struct mpsd_serve_stats {

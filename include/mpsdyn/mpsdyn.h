@@ -396,7 +396,20 @@ struct SolveStats {
   int64_t env_updates = 0;
   double final_residual = 0.0;
   std::vector<double> energies;  // energy at the end of each half-sweep
+  // Wall-clock split of findGroundState's sweep in seconds, filled when the breakdown is on
+  // (Contractor::setBreakdown): environment updates, local eigensolves (matvecs plus the
+  // eigensolver's vector work), gauge steps; t_matvec is the matvec kernels' time inside the
+  // eigensolves, from stream events.
+  double t_env = 0.0, t_eig = 0.0, t_gauge = 0.0, t_matvec = 0.0;
+  // algorithmic flops (complex MAC = 8) of the matvecs and of the environment updates done
+  double matvec_flops = 0.0, env_flops = 0.0;
 };
+
+// Algorithmic real flops of one matvec or environment update over a site [d, Dl, Dr] with MPO
+// bonds wl, wr (bra and ket bonds equal): 8 (d wl Dl^2 Dr + d^2 wl wr Dl Dr + d wr Dl Dr^2).
+inline double contraction_flops(int d, int Dl, int Dr, int wl, int wr) {
+  return 8.0 * ((double)d * wl * Dl * Dl * Dr + (double)d * d * wl * wr * Dl * Dr + (double)d * wr * Dl * Dr * Dr);
+}
 
 // Builds the Multiplier a sweep applies at one site (default: TensorMultiplierOffset on the
 // Contractor's context).  A serving runtime substitutes one that batches matvecs of
@@ -418,6 +431,14 @@ class Contractor {
   void setMaxHalfSweeps(int n) { max_halfsweeps_ = n; }
   void setPrecision(int p) { precision_ = p; }
   void setMultiplierFactory(MultiplierFactory f) { factory_ = std::move(f); }
+  // Time the phases of findGroundState (SolveStats::t_*): the stream is drained at each phase
+  // boundary, which costs a little time, so it is off by default.
+  void setBreakdown(bool on) { breakdown_ = on; }
+  // Local-tolerance schedule of findGroundState: off (default) solves every local problem to
+  // the eigensolver tolerance, as the reference does; on, the FP64 local solves of a half-sweep
+  // stop at max(eigtol, min(1e-3, |dE| / |E| of the previous half-sweep)), 1e-3 at the start
+  // (the FP64 form only: the mixed forms' precision ladder schedules its own tolerances).
+  void setEigTolSchedule(bool on) { schedule_ = on; }
   // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).
   void findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset = 0.,
                        int knr = 1, const std::string& tmpfile = "", int freqSv = 200, bool useLM = false);
@@ -496,6 +517,8 @@ class Contractor {
   SolveStats stats_;
   std::unique_ptr<LanczosSolver> eig_;
   MultiplierFactory factory_;
+  bool breakdown_ = false;
+  bool schedule_ = false;
 };
 
 // ------------------------------------------------------------------------------ models

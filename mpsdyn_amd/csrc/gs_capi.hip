@@ -74,6 +74,7 @@ void configure(Contractor& c, const mpsd_gs_config* cfg) {
   c.setMaxHalfSweeps(cfg->max_halfsweeps);
   c.setPrecision(cfg->precision);
   if (cfg->solver == MPSD_SOLVER_FULLEIG) c.setEigenSolver(fulleig);
+  c.setEigTolSchedule(cfg->tol_schedule != 0);
 }
 
 void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, mpsd_gs_result* out) {
@@ -90,6 +91,8 @@ void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, m
   out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
   for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
   out->residual_norm = -1.0;
+  out->matvec_flops = st.matvec_flops;
+  out->env_flops = st.env_flops;
 }
 
 }  // namespace
@@ -170,6 +173,8 @@ int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo,
     c.setMaxHalfSweeps(cfg->max_halfsweeps);
     c.setPrecision(cfg->precision);
     if (cfg->solver == MPSD_SOLVER_FULLEIG) c.setEigenSolver(fulleig);
+    c.setBreakdown(cfg->breakdown != 0);
+    c.setEigTolSchedule(cfg->tol_schedule != 0);
     out->initial_energy = c.contract(mps, hmpo, mps).real();
     double E = 0.0;
     auto t0 = std::chrono::steady_clock::now();
@@ -188,6 +193,12 @@ int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo,
     out->refine_from = st.refine_from;
     out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
     for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+    out->t_env = st.t_env;
+    out->t_eig = st.t_eig;
+    out->t_gauge = st.t_gauge;
+    out->t_matvec = st.t_matvec;
+    out->matvec_flops = st.matvec_flops;
+    out->env_flops = st.env_flops;
     out->residual_norm = -1.0;
     if (cfg->want_residual) out->residual_norm = residual_norm(c, cfg, mps, E);
     if (export_path) mps.exportMPS(export_path);

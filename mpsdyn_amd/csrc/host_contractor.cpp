@@ -39,6 +39,9 @@ void Contractor::calculateL(int pos, TmpManager& mgr, const MPO& ops, const MPS&
                         bra.getA(k - 1).device(), mgr.operL(k).t),
           "env_left");
     ++stats_.env_updates;
+    const Site& s = ket.getA(k - 1);
+    stats_.env_flops += contraction_flops(s.getd(), s.getDl(), s.getDr(), ops.getOp(k - 1).getDimension(1),
+                                          ops.getOp(k - 1).getDimension(3));
     mgr.storedL(k);
   }
 }
@@ -54,6 +57,9 @@ void Contractor::calculateR(int pos, TmpManager& mgr, const MPO& ops, const MPS&
                          bra.getA(k + 1).device(), mgr.operR(k).t),
           "env_right");
     ++stats_.env_updates;
+    const Site& s = ket.getA(k + 1);
+    stats_.env_flops += contraction_flops(s.getd(), s.getDl(), s.getDr(), ops.getOp(k + 1).getDimension(1),
+                                          ops.getOp(k + 1).getDimension(3));
     mgr.storedR(k);
   }
 }
@@ -108,11 +114,39 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   bool refine = !lowp;
   bool sweep_all_fp64 = refine;
   bool have_fp64 = false;
+  // Tolerance schedule (setEigTolSchedule): the FP64 local solves of a half-sweep stop at
+  // max(eigtol, min(1e-3, relative energy change of the half-sweep before)), 1e-3 for the first
+  // ones.  While the environments are far from converged a loose local solve loses nothing; the
+  // tolerance reaches eigtol before the stopping rule |dE| < convtol |E| can hold.  The mixed
+  // forms keep their own ladder (their FP64 refinement runs at eigtol).
+  const bool sched = schedule_ && !lowp;
+  double fp64_tol = sched ? std::max(tol, 1e-3) : tol;
+  int tight = 0;  // consecutive half-sweeps solved at eigtol: the stopping rule needs two
+  // phase breakdown: drain the stream at each phase boundary and charge the wall time since the
+  // last boundary to the phase that ended
+  auto last = std::chrono::steady_clock::now();
+  auto mark = [&](double& acc) {
+    if (!breakdown_) return;
+    check(ctx_, mpsd_ctx_sync(ctx_), "breakdown sync");
+    const auto now = std::chrono::steady_clock::now();
+    acc += std::chrono::duration<double>(now - last).count();
+    last = now;
+  };
+  const unsigned mv_kinds = (1u << MPSD_KERNEL_STAGE1) | (1u << MPSD_KERNEL_MIX) | (1u << MPSD_KERNEL_HEFF_STAGE3);
   while (!done) {
+    mark(stats_.t_gauge);
     calculateL(pos, mgr, ops, init, init);
     calculateR(pos, mgr, ops, init, init);
+    mark(stats_.t_env);
+    if (breakdown_) {
+      check(ctx_, mpsd_ctx_set_timing_kinds(ctx_, mv_kinds), "timing kinds");
+      check(ctx_, mpsd_ctx_set_timing(ctx_, 1), "timing on");
+    }
     Site& site = init.site(pos);
     double* xv = (double*)mpsd_tensor_devptr(site.device());
+    const double site_flops = contraction_flops(site.getd(), site.getDl(), site.getDr(),
+                                                ops.getOp(pos).getDimension(1), ops.getOp(pos).getDimension(3));
+    const int64_t mv_before = stats_.matvecs;
     {
       const int first = lowp ? (refine ? MPSD_PREC_FP32 : phase) : MPSD_PREC_FP64;
       std::unique_ptr<Multiplier> mp;
@@ -134,12 +168,24 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
       }
       if (refine) {
         if (lowp) multi.setPrecision(MPSD_PREC_FP64);
-        EigStats es = dense ? eig_->dense(multi, xv) : eig_->lowest(multi, xv, tol);
+        EigStats es = dense ? eig_->dense(multi, xv) : eig_->lowest(multi, xv, fp64_tol);
         stats_.matvecs += es.matvecs;
         stats_.final_residual = es.residual;
         val = es.value - offset;
       }
     }
+    stats_.matvec_flops += site_flops * (double)(stats_.matvecs - mv_before);
+    if (breakdown_) {
+      for (int k = 0; k < MPSD_KERNEL_KINDS; ++k) {
+        double ms = 0.0;
+        int64_t n = 0;
+        if ((mv_kinds >> k) & 1u) check(ctx_, mpsd_ctx_timing(ctx_, k, &ms, &n), "timing");
+        stats_.t_matvec += 1e-3 * ms;
+      }
+      check(ctx_, mpsd_ctx_set_timing(ctx_, 0), "timing off");
+      check(ctx_, mpsd_ctx_set_timing_kinds(ctx_, ~0u), "timing kinds");
+    }
+    mark(stats_.t_eig);
     // energy-increase guard (Contractor.cpp:1417-1434), between FP64 energies only
     if (refine && have_fp64 && (val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
         (val - energy > std::fabs(energy + offset) * convtol_)) {
@@ -159,10 +205,20 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     const bool endLeft = !right && pos == 0;
     if (endRight || endLeft) {
       const double dE = std::fabs(energy - val);
+      tight = fp64_tol <= tol ? tight + 1 : 0;
       if (sweep_all_fp64) {
-        done = dE < convtol_ * std::fabs(energy + offset);
+        // with the schedule, the reference's test between two half-sweeps solved at eigtol: loose
+        // local solves can stall a sweep below convtol before the state has converged
+        done = dE < convtol_ * std::fabs(energy + offset) && (!sched || tight >= 2);
         if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
-      } else if (!refine && (dE < switch_tol(phase) * std::fabs(val + offset) || (round > 0 && val > energy))) {
+      }
+      if (sched && have_fp64 && !done) {
+        const double rel = dE / std::max(std::fabs(val + offset), 1e-300);
+        fp64_tol = std::max(tol, std::min(1e-3, 1e-2 * rel));
+        if (dE < convtol_ * std::fabs(energy + offset)) fp64_tol = tol;  // confirm at eigtol
+      }
+      if (!sweep_all_fp64 && !refine &&
+          (dE < switch_tol(phase) * std::fabs(val + offset) || (round > 0 && val > energy))) {
         if (phase == MPSD_PREC_BF16X3) {
           phase = MPSD_PREC_FP32;  // next half-sweep in FP32
         } else {
@@ -190,6 +246,7 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
       pos += right ? 1 : -1;
     }
   }
+  mark(stats_.t_gauge);
   stats_.halfsweeps = round;
   *lambda = energy;
 }

@@ -408,7 +408,15 @@ void MPS::setRandomState(uint64_t seed) {
   for (int k = 0; k <= L; ++k) bd[k] = (int)std::min<int64_t>(std::min<int64_t>(left[k], right[k]), bond_);
   for (int k = 0; k < L; ++k) {
     const int d = A_[k]->getd();
-    A_[k]->setA(complex_gaussian(seed * 1000003ull + (uint64_t)k, {d, bd[k], bd[k + 1]}));
+    // Each site is scaled by the power of two nearest 1 / sqrt(2 d Dr), so the unnormalised
+    // state's norm stays near 1 along the chain: with unit-variance entries it would grow like
+    // prod_k sqrt(2 d_k Dr_k), past the double range for long chains (e.g. 1e160 at L = 109,
+    // D = 256), and the gauge factors absorbed below carry it.  A power of two scales exactly,
+    // and the normalisation removes it, so the gauged state does not depend on it.
+    mwArray a = complex_gaussian(seed * 1000003ull + (uint64_t)k, {d, bd[k], bd[k + 1]});
+    const double sc = std::ldexp(1.0, -(int)std::lround(0.5 * std::log2(2.0 * d * bd[k + 1])));
+    for (int64_t i = 0; i < a.getNrComponents(); ++i) a.data()[i] *= sc;
+    A_[k]->setA(a);
   }
   gaugeCond('R', true);
 }

@@ -141,10 +141,11 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
       for (int i = 0; i < RPT; ++i)
         if (row[i] > c) s1[0].x += a[i][t].x * a[i][t].x + a[i][t].y * a[i][t].y;
       block_sum<1, NW>(s1, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
-      // A column whose squares underflow is rescaled by its largest entry first (the role of
-      // zlarfg's safmin loop); the test reads shared memory, so every thread takes the branch.
+      // A column whose squares underflow or overflow is rescaled by its largest entry first (the
+      // role of zlarfg's safmin loop and of dznrm2's scaling); the test reads shared memory, so
+      // every thread takes the branch.
       double xs2 = sums[0].x, S = 1.0;
-      if (xs2 < 1e-280) {
+      if (xs2 < 1e-280 || !(xs2 < 1e280)) {
         double mx = 0.0;
 #pragma unroll
         for (int i = 0; i < RPT; ++i)
@@ -336,8 +337,9 @@ __device__ __forceinline__ double2 tree_sum<double2>(double2 (*v)[64], int u) {
 }
 
 // zlarfg on alpha and the squared norm xs2 of the rest of the column.  Unless the column's squares
-// underflow (scaled: xs2 is then the norm of the column over S, its largest entry), |alpha - beta|^2
-// >= xs2 >= 1e-280 and two plain reciprocals suffice; otherwise every step is scaled.
+// underflow or overflow (scaled: xs2 is then the norm of the column over S, its largest entry),
+// 1e-280 <= xs2 <= |alpha - beta|^2 < 1e281 and two plain reciprocals suffice; otherwise every step
+// is scaled.
 __device__ __forceinline__ Reflector zlarfg_of(double2 alpha, double xs2, double S, bool scaled) {
   Reflector f{{0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}, alpha.x, false};
   if (xs2 == 0.0 && alpha.y == 0.0) return f;
@@ -431,9 +433,10 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
     const double2* xc = X + ld * c;
     const double2 alpha = xc[c];
     double S = 1.0;
-    const bool scaled = xs2 < 1e-280;
+    const bool scaled = xs2 < 1e-280 || !(xs2 < 1e280);
     if (scaled) {
-      // squares that underflow: rescale by the column's largest entry (zlarfg's safmin loop)
+      // squares that underflow or overflow: rescale by the column's largest entry (zlarfg's
+      // safmin loop, dznrm2's scaling)
       double mx = 0.0;
       for (int r = c; r < m; ++r) mx = fmax(mx, fmax(fabs(xc[r].x), fabs(xc[r].y)));
       double s2 = 0.0;
@@ -450,11 +453,14 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       beta_s[c] = f.beta;
     }
     // (H_c = I when !reflect: tau = scal = 0 and the step below only refreshes the partial norms)
+    // v = scal x: for a rescaled column the step works on x / S and S scal, so that x^H a neither
+    // overflows nor underflows (the same numbers as unscaled columns, where S = 1).
+    const double2 sv = scaled ? double2{f.scal.x * S, f.scal.y * S} : f.scal;
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = r0 + i;
-      const double2 x = xc[r];
-      xr[i] = r > c && r < m ? x : (r == c ? f.dd : double2{0.0, 0.0});
+      const double2 x = r > c && r < m ? xc[r] : (r == c ? f.dd : double2{0.0, 0.0});
+      xr[i] = scaled ? double2{x.x / S, x.y / S} : x;
     }
     // this wave's partial of v^H a_u
     const bool mine = (u > c && u < n) || (fq && u >= n && u < n + m);
@@ -463,23 +469,23 @@ __global__ void __launch_bounds__(64 * kSQW) qr_small_kernel(const double2* Ag, 
       double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], a[i]);
-      if (mine) part[w][u] = cjmul(f.scal, double2{acc0.x + acc1.x, acc0.y + acc1.y});
+      if (mine) part[w][u] = cjmul(sv, double2{acc0.x + acc1.x, acc0.y + acc1.y});
     }
     if constexpr (q2) {
       double2 acc0{0.0, 0.0}, acc1{0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < RPT; ++i) cj_fma(i & 1 ? acc1 : acc0, xr[i], b[i]);
-      if (mineB) partB[w][u] = cjmul(f.scal, double2{acc0.x + acc1.x, acc0.y + acc1.y});
+      if (mineB) partB[w][u] = cjmul(sv, double2{acc0.x + acc1.x, acc0.y + acc1.y});
     }
     __syncthreads();
     if (mineB) {
-      const double2 g2 = cmul(f.scal, cjmul(f.tau, tree_sum<double2>(partB, u)));
+      const double2 g2 = cmul(sv, cjmul(f.tau, tree_sum<double2>(partB, u)));
 #pragma unroll
       for (int i = 0; i < RPT; ++i) sub_mul(b[i], xr[i], g2);
     }
     // H_c^H on the trailing columns: a_u -= conj(tau) v (v^H a_u), then back to LDS
     if (mine) {
-      const double2 g2 = cmul(f.scal, cjmul(f.tau, tree_sum<double2>(part, u)));
+      const double2 g2 = cmul(sv, cjmul(f.tau, tree_sum<double2>(part, u)));
       double s2 = 0.0;
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {

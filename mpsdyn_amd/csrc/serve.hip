@@ -285,6 +285,7 @@ struct mpsd_server {
         r->res.halfsweeps = g.halfsweeps;
         r->res.matvecs = g.matvecs;
         r->res.matvecs_lowp = g.matvecs_lowp;
+        r->res.matvec_flops = g.matvec_flops;
         r->res.finish_s = now();
         r->done = true;
         admitted -= r->res.footprint_bytes;

@@ -173,7 +173,7 @@ class GsConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("L", C.c_int), ("D", C.c_int), ("p", C.c_double * 4),
                 ("offset", C.c_double), ("convtol", C.c_double), ("eigtol", C.c_double),
                 ("max_halfsweeps", C.c_int), ("precision", C.c_int), ("seed", C.c_uint64), ("solver", C.c_int),
-                ("want_residual", C.c_int)]
+                ("want_residual", C.c_int), ("breakdown", C.c_int), ("tol_schedule", C.c_int)]
 
 
 class GsResult(C.Structure):
@@ -182,7 +182,9 @@ class GsResult(C.Structure):
                 ("matvecs", C.c_int64), ("env_updates", C.c_int64), ("seconds", C.c_double),
                 ("residual", C.c_double), ("device_bytes", C.c_int64), ("matvecs_lowp", C.c_int64),
                 ("refine_from", C.c_int), ("n_history", C.c_int),
-                ("history", C.c_double * 256), ("residual_norm", C.c_double)]
+                ("history", C.c_double * 256), ("residual_norm", C.c_double), ("t_env", C.c_double),
+                ("t_eig", C.c_double), ("t_gauge", C.c_double), ("t_matvec", C.c_double),
+                ("matvec_flops", C.c_double), ("env_flops", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -195,7 +197,8 @@ class ServeResult(C.Structure):
     """mpsd_serve_result (include/mpsd_serve.h)."""
     _fields_ = [("id", C.c_int), ("status", C.c_int), ("energy", C.c_double), ("arrival_s", C.c_double),
                 ("admit_s", C.c_double), ("finish_s", C.c_double), ("halfsweeps", C.c_int),
-                ("matvecs", C.c_int64), ("matvecs_lowp", C.c_int64), ("footprint_bytes", C.c_int64)]
+                ("matvecs", C.c_int64), ("matvecs_lowp", C.c_int64), ("footprint_bytes", C.c_int64),
+                ("matvec_flops", C.c_double)]
 
 
 class ServeStats(C.Structure):
@@ -252,10 +255,12 @@ SOLVER_LANCZOS, SOLVER_FULLEIG = 0, 1
 
 
 def gs_config(model, L, D, p, convtol=1e-10, eigtol=0.0, max_halfsweeps=0, precision=PREC_FP64,
-              seed=1, offset=0.0, solver=SOLVER_LANCZOS, want_residual=False):
+              seed=1, offset=0.0, solver=SOLVER_LANCZOS, want_residual=False, breakdown=False,
+              tol_schedule=False):
     pp = list(p) + [0.0] * (4 - len(p))
     return GsConfig(model, L, D, (C.c_double * 4)(*pp), offset, convtol, eigtol, max_halfsweeps,
-                    precision, seed, solver, 1 if want_residual else 0)
+                    precision, seed, solver, 1 if want_residual else 0, 1 if breakdown else 0,
+                    1 if tol_schedule else 0)
 
 
 def gs_solve(ctx, cfg, export_path=None):

@@ -95,3 +95,37 @@ def test_export_import_checkpoint(ctx, tmp_path):
     # header of MPS::exportMPS: int L, int bond, double normfact, int gauge
     raw = open(path, "rb").read()
     assert np.frombuffer(raw[:8], dtype=np.int32).tolist() == [12, 16]
+
+
+@pytest.mark.parametrize("model,L,D,p,convtol", [
+    (0, 100, 128, [-1.0, -1.0, 0.0], 1e-10),
+    (0, 100, 256, [-1.0, -1.0, 0.0], 1e-10),
+    (2, 40, 48, [2.5, 100.0, 0.0, 0.0], 1e-12),
+    (1, 32, 64, [1.0, 1.0, 1.0, 0.0], 1e-12),
+    (0, 20, 20, [-1.0, -1.0, -0.5], 1e-12),
+])
+def test_tolerance_schedule_reaches_the_fixed_tolerance_energy(ctx, model, L, D, p, convtol):
+    """The local-tolerance schedule (loose FP64 local solves while the sweep is far from
+    converged, eigtol for the last two half-sweeps) ends at the energy of the sweep that solves
+    every local problem to eigtol, to 1e-10 relative, with fewer matvecs on the large sites."""
+    from mpsdyn_amd import device as dev
+    fixed = dev.gs_solve(ctx, dev.gs_config(model, L, D, p, convtol=convtol))
+    sched = dev.gs_solve(ctx, dev.gs_config(model, L, D, p, convtol=convtol, tol_schedule=True))
+    print(f"model {model} L={L} D={D}: fixed E={fixed.energy:.15f} mv={fixed.matvecs} {fixed.seconds:.2f}s | "
+          f"schedule E={sched.energy:.15f} mv={sched.matvecs} {sched.seconds:.2f}s", flush=True)
+    assert abs(sched.energy - fixed.energy) <= 1e-10 * abs(fixed.energy)
+    if L == 100:
+        assert sched.matvecs < fixed.matvecs
+
+
+@pytest.mark.parametrize("L,D,g", [(180, 64, 1.5), (120, 256, 1.5)])
+def test_long_chain_reaches_the_closed_form(ctx, L, D, g):
+    """Long chains (the served mix draws L up to 200): the seeded start state stays inside the
+    double range through its right gauge (its norm would otherwise grow like prod sqrt(2 d D) to
+    about 1e160 here), and the open TFIM at g = 1.5 reaches its free-fermion energy."""
+    from mpsdyn_amd import device as dev
+    r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_ISING, L, D, [-1.0, -g, 0.0], convtol=1e-10, tol_schedule=True))
+    e = ph.tfim_free_fermion(L, 1.0, g)
+    print(f"L={L} D={D} g={g}: E={r.energy:.12f} closed form {e:.12f} ({r.seconds:.2f}s, {r.matvecs} matvecs)",
+          flush=True)
+    assert abs(r.energy - e) <= 1e-10 * abs(e)

@@ -178,3 +178,18 @@ def test_gauge_absorb_matches_two_calls(ctx, d, Dl, Dr, Dn):
             want = np.einsum("sac,cj->saj", B, f0)
         assert b0.shape[1 if side == "r" else 2] == k
         assert _rel(b1, want) <= 1e-13, side
+
+
+@pytest.mark.parametrize("d,Dl,Dr", [(2, 16, 12), (2, 20, 20), (2, 128, 96), (2, 256, 200)])
+@pytest.mark.parametrize("scale", [1e200, 1e-200])
+def test_huge_and_tiny_columns(ctx, d, Dl, Dr, scale):
+    """Columns whose squared norms overflow (entries 1e200) or underflow (1e-200) are rescaled
+    before the reflector (as zlarfg / dznrm2 do): the factorisation of s A is the one of A, with R
+    scaled by s, in the single-workgroup and the panel form."""
+    print(f"scaled gauge_r d={d} Dl={Dl} Dr={Dr} scale={scale}", flush=True)
+    A = orc.complex_gaussian(11 + Dl, (d, Dl, Dr))
+    Q1, R1 = _gauge_r(ctx, A, 0)
+    Qs, Rs = _gauge_r(ctx, scale * A, 0)
+    assert np.all(np.isfinite(Qs)) and np.all(np.isfinite(Rs))
+    assert _rel(Qs, Q1) <= 1e-13 * np.sqrt(d * Dl)
+    assert _rel(Rs / scale, R1) <= 1e-13 * np.sqrt(d * Dl)
