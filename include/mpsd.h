@@ -85,6 +85,8 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
                                    or 0 (three launches); the results are bit-identical */
 #define MPSD_OPT_BATCH_GROUP 4  /* batches per BRGEMM workgroup: 0 built-in (default), 1, or 2 when an
                                    operand is shared by the batches; results are the same for all */
+#define MPSD_OPT_KRYLOV 8       /* Krylov basis size of the ground-state sweep's local Lanczos solves:
+                                   0 (default, 30) or 4..62 (more restarts, less Gram-Schmidt work) */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
 int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option);
 

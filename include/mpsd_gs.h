@@ -68,6 +68,8 @@ typedef struct mpsd_gs_result {
   double t_matvec;        /* with cfg->breakdown: seconds of matvec kernels inside the eigensolves */
   double matvec_flops;    /* algorithmic flops of all matvecs (8 per complex MAC, per site shape) */
   double env_flops;       /* algorithmic flops of all environment updates */
+  double residual_floor;  /* with cfg->want_residual: the rounding level of residual_norm (the square root
+                             of the rounding of <(H - E)^2>); residuals below it are not resolved */
 } mpsd_gs_result;
 
 /* Run one solve on ctx.  Returns MPSD_OK or an MPSD_ERR_* code (message: mpsd_last_error). */

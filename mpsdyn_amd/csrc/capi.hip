@@ -422,6 +422,7 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_GAUGE_QR: return ctx->gauge_qr;
     case MPSD_OPT_FUSED_LANCZOS: return ctx->fused_lanczos;
     case MPSD_OPT_ENV_CHAIN: return ctx->env_chain;
+    case MPSD_OPT_KRYLOV: return ctx->krylov;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -450,6 +451,11 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
       return MPSD_OK;
     case MPSD_OPT_ENV_CHAIN:
       ctx->env_chain = value != 0;
+      return MPSD_OK;
+    case MPSD_OPT_KRYLOV:
+      if (value != 0 && (value < 4 || value > 62))
+        return set_err(ctx, MPSD_ERR_ARG, "set_option: Krylov basis size must be 0 (default) or 4..62");
+      ctx->krylov = value;
       return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");

@@ -91,6 +91,7 @@ void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, m
   out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
   for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
   out->residual_norm = -1.0;
+  out->residual_floor = -1.0;
   out->matvec_flops = st.matvec_flops;
   out->env_flops = st.env_flops;
 }
@@ -115,11 +116,19 @@ std::unique_ptr<Hamiltonian> shifted_model(const mpsd_gs_config* c, double E) {
 }
 
 // ||H psi - E psi|| for a normalised psi: sqrt(<psi|(H - E)^2|psi>) over the MPO of (H - E)^2,
-// the product of the shifted model's MPO with itself (bond w^2), on the device.
-double residual_norm(Contractor& c, const mpsd_gs_config* cfg, MPS& mps, double E) {
+// the product of the shifted model's MPO with itself (bond w^2), on the device.  *floor gets the
+// rounding level of that variance: <(H - E - delta)^2> - delta^2 - <(H - E)^2> is zero in exact
+// arithmetic (up to 2 delta (<H> - E), which is far smaller), so the difference of the two
+// evaluations measures the rounding of either; a residual below sqrt of it is not resolved.
+double residual_norm(Contractor& c, const mpsd_gs_config* cfg, MPS& mps, double E, double* floor) {
   auto Hs = shifted_model(cfg, E);
   const MPO sq = mpo_product(Hs->getHMPO(), Hs->getHMPO());
   const double v = c.contract(mps, sq, mps).real();
+  const double delta = std::ldexp(1.0, -10);  // delta^2 exact
+  auto Hd = shifted_model(cfg, E + delta);
+  const MPO sqd = mpo_product(Hd->getHMPO(), Hd->getHMPO());
+  const double vd = c.contract(mps, sqd, mps).real();
+  *floor = std::sqrt(std::fabs(vd - delta * delta - v));
   return std::sqrt(std::max(v, 0.0));
 }
 
@@ -200,7 +209,8 @@ int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo,
     out->matvec_flops = st.matvec_flops;
     out->env_flops = st.env_flops;
     out->residual_norm = -1.0;
-    if (cfg->want_residual) out->residual_norm = residual_norm(c, cfg, mps, E);
+    out->residual_floor = -1.0;
+    if (cfg->want_residual) out->residual_norm = residual_norm(c, cfg, mps, E, &out->residual_floor);
     if (export_path) mps.exportMPS(export_path);
     return MPSD_OK;
   } catch (const Error& e) {

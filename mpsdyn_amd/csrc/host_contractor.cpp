@@ -97,31 +97,30 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   int pos = 0, round = 0;
   bool right = true, done = false;
   const double tol = eigtol_ > 0 ? eigtol_ : 1e-10;
-  // Precision form, a ladder of phases: split-BF16 -> complex FP32 -> FP64.  A phase ends
-  // when a half-sweep changes the energy by less than that arithmetic's floor; in the last
-  // phase each site is first solved with the FP32 matvec and then refined with the FP64
-  // one from that vector, and only FP64 energies enter the stopping rule and the
-  // energy-increase guard.
   // fulleig: the Hermitian multiplier and the dense eigensolver, in FP64
   // (Contractor.cpp:1403-1404, 3087-3100)
   const bool dense = solver_ == fulleig;
   const bool lowp = precision_ != MPSD_PREC_FP64 && !dense;
-  int phase = precision_;  // MPSD_PREC_BF16X3 (2) -> MPSD_PREC_FP32 (1) -> FP64 refinement (0)
-  auto low_tol = [](int p) { return p == MPSD_PREC_FP32 ? 1e-6 : 1e-4; };      // local residual
-  // relative dE per half-sweep below which a phase ends (or when its energy stops decreasing:
-  // the arithmetic's noise floor is reached)
-  auto switch_tol = [](int p) { return p == MPSD_PREC_FP32 ? 1e-9 : 3e-6; };
-  bool refine = !lowp;
-  bool sweep_all_fp64 = refine;
-  bool have_fp64 = false;
-  // Tolerance schedule (setEigTolSchedule): the FP64 local solves of a half-sweep stop at
-  // max(eigtol, min(1e-3, relative energy change of the half-sweep before)), 1e-3 for the first
-  // ones.  While the environments are far from converged a loose local solve loses nothing; the
-  // tolerance reaches eigtol before the stopping rule |dE| < convtol |E| can hold.  The mixed
-  // forms keep their own ladder (their FP64 refinement runs at eigtol).
-  const bool sched = schedule_ && !lowp;
-  double fp64_tol = sched ? std::max(tol, 1e-3) : tol;
-  int tight = 0;  // consecutive half-sweeps solved at eigtol: the stopping rule needs two
+  // Tolerance schedule (setEigTolSchedule, and always for the precision forms): the local solves
+  // of a half-sweep stop at max(eigtol, min(1e-3, 1e-2 |dE| / |E| of the half-sweep before)),
+  // 1e-3 for the first ones.  While the environments are far from converged a loose local solve
+  // loses nothing; the stopping rule |dE| < convtol |E| is only tested between two half-sweeps
+  // solved at eigtol, since loose solves can stall a sweep below convtol before the state has
+  // converged.
+  const bool sched = (schedule_ || lowp) && !dense;
+  double cur_tol = sched ? std::max(tol, 1e-3) : tol;
+  int tight = 0;  // consecutive half-sweeps solved at eigtol
+  // Precision form: a local solve at tolerance t uses the lowest arithmetic the form allows whose
+  // matvec error (about 1e-5 relative for split-BF16, 1e-7 for complex FP32) stays two decades
+  // below t; the eigensolver's vectors, the gauge steps and the stopping test stay FP64, and the
+  // half-sweeps at eigtol run the FP64 matvec, so the final energy is an FP64 one.
+  auto arith_for = [&](double t) {
+    if (!lowp) return MPSD_PREC_FP64;
+    if (precision_ == MPSD_PREC_BF16X3 && t >= 1e-3) return MPSD_PREC_BF16X3;
+    if (t >= 1e-5) return MPSD_PREC_FP32;
+    return MPSD_PREC_FP64;
+  };
+  bool base_fp64 = false;  // the energy-increase guard's baseline comes from an FP64 solve
   // phase breakdown: drain the stream at each phase boundary and charge the wall time since the
   // last boundary to the phase that ended
   auto last = std::chrono::steady_clock::now();
@@ -133,6 +132,8 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     last = now;
   };
   const unsigned mv_kinds = (1u << MPSD_KERNEL_STAGE1) | (1u << MPSD_KERNEL_MIX) | (1u << MPSD_KERNEL_HEFF_STAGE3);
+  const int krylov_opt = mpsd_ctx_get_option(ctx_, MPSD_OPT_KRYLOV);
+  const int krylov = krylov_opt > 0 ? krylov_opt : 30;
   while (!done) {
     mark(stats_.t_gauge);
     calculateL(pos, mgr, ops, init, init);
@@ -146,35 +147,24 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     double* xv = (double*)mpsd_tensor_devptr(site.device());
     const double site_flops = contraction_flops(site.getd(), site.getDl(), site.getDr(),
                                                 ops.getOp(pos).getDimension(1), ops.getOp(pos).getDimension(3));
-    const int64_t mv_before = stats_.matvecs;
+    const int arith = arith_for(cur_tol);
     {
-      const int first = lowp ? (refine ? MPSD_PREC_FP32 : phase) : MPSD_PREC_FP64;
       std::unique_ptr<Multiplier> mp;
       if (factory_)
-        mp = factory_(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, first);
+        mp = factory_(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t, offset, arith);
       else if (dense)
         mp.reset(new TensorMultiplierHermitianOffset(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_),
-                                                     mgr.operR(pos).t, offset, first));
+                                                     mgr.operR(pos).t, offset, arith));
       else
         mp.reset(new TensorMultiplierOffset(ctx_, mgr.operL(pos).t, ops.getOp(pos).device(ctx_), mgr.operR(pos).t,
-                                            offset, first));
-      Multiplier& multi = *mp;
-      if (lowp) {
-        EigStats es = eig_->lowest(multi, xv, std::max(tol, low_tol(first)));
-        stats_.matvecs += es.matvecs;
-        stats_.matvecs_lowp += es.matvecs;
-        stats_.final_residual = es.residual;
-        val = es.value - offset;
-      }
-      if (refine) {
-        if (lowp) multi.setPrecision(MPSD_PREC_FP64);
-        EigStats es = dense ? eig_->dense(multi, xv) : eig_->lowest(multi, xv, fp64_tol);
-        stats_.matvecs += es.matvecs;
-        stats_.final_residual = es.residual;
-        val = es.value - offset;
-      }
+                                            offset, arith));
+      EigStats es = dense ? eig_->dense(*mp, xv) : eig_->lowest(*mp, xv, cur_tol, krylov);
+      stats_.matvecs += es.matvecs;
+      if (arith != MPSD_PREC_FP64) stats_.matvecs_lowp += es.matvecs;
+      stats_.final_residual = es.residual;
+      stats_.matvec_flops += site_flops * (double)es.matvecs;
+      val = es.value - offset;
     }
-    stats_.matvec_flops += site_flops * (double)(stats_.matvecs - mv_before);
     if (breakdown_) {
       for (int k = 0; k < MPSD_KERNEL_KINDS; ++k) {
         double ms = 0.0;
@@ -187,16 +177,17 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     }
     mark(stats_.t_eig);
     // energy-increase guard (Contractor.cpp:1417-1434), between FP64 energies only
-    if (refine && have_fp64 && (val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
+    const bool fp64 = arith == MPSD_PREC_FP64;
+    if (fp64 && base_fp64 && (val - energy > 1e-8) && (val - energy > std::fabs(energy + offset) * 1e-10) &&
         (val - energy > std::fabs(energy + offset) * convtol_)) {
       char buf[256];
       snprintf(buf, sizeof buf, "energy increasing in this step: old %.15g new %.15g (round %d, pos %d)", energy,
                val, round, pos);
       throw Error(MPSD_ERR_NUMERIC, buf);
     }
-    if (refine && !have_fp64) {
+    if (fp64 && !base_fp64) {
       energy = val;  // first FP64 value: the guard's baseline from here on
-      have_fp64 = true;
+      base_fp64 = true;
     }
     init.gaugeCond(pos, right ? 'R' : 'L', true);
     for (int l = pos + 1; l < L; ++l) mgr.outofdateL(l);
@@ -205,31 +196,19 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     const bool endLeft = !right && pos == 0;
     if (endRight || endLeft) {
       const double dE = std::fabs(energy - val);
-      tight = fp64_tol <= tol ? tight + 1 : 0;
-      if (sweep_all_fp64) {
-        // with the schedule, the reference's test between two half-sweeps solved at eigtol: loose
-        // local solves can stall a sweep below convtol before the state has converged
-        done = dE < convtol_ * std::fabs(energy + offset) && (!sched || tight >= 2);
-        if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
-      }
-      if (sched && have_fp64 && !done) {
+      tight = cur_tol <= tol ? tight + 1 : 0;
+      done = dE < convtol_ * std::fabs(energy + offset) && (!sched || tight >= 2);
+      if (!done && std::fabs(val) < 1e-13) done = true;  // Contractor.cpp:1473-1478
+      if (lowp && arith == MPSD_PREC_FP64 && stats_.refine_from < 0) stats_.refine_from = round;
+      if (sched && !done) {
         const double rel = dE / std::max(std::fabs(val + offset), 1e-300);
-        fp64_tol = std::max(tol, std::min(1e-3, 1e-2 * rel));
-        if (dE < convtol_ * std::fabs(energy + offset)) fp64_tol = tol;  // confirm at eigtol
-      }
-      if (!sweep_all_fp64 && !refine &&
-          (dE < switch_tol(phase) * std::fabs(val + offset) || (round > 0 && val > energy))) {
-        if (phase == MPSD_PREC_BF16X3) {
-          phase = MPSD_PREC_FP32;  // next half-sweep in FP32
-        } else {
-          refine = true;  // next half-sweep refines every site in FP64
-          stats_.refine_from = round + 1;
-        }
+        cur_tol = std::max(tol, std::min(1e-3, 1e-2 * rel));
+        if (dE < convtol_ * std::fabs(energy + offset)) cur_tol = tol;  // confirm at eigtol
       }
       energy = val;
+      base_fp64 = fp64;
       stats_.energies.push_back(val);
       ++round;
-      sweep_all_fp64 = refine;
       if (max_halfsweeps_ > 0 && round >= max_halfsweeps_) done = true;
       if (!done) {
         if (endRight) {

