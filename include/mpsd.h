@@ -277,6 +277,17 @@ int mpsd_lanczos_coef_update(mpsd_ctx* ctx, const double* V, int64_t ldv, const 
 int mpsd_lanczos_gs1_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int m, double* w, int64_t n, double* h_dev);
 int mpsd_lanczos_gs2_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, const double* h1_dev, double* h2_dev,
                          double* ab_dev, double* w, int64_t n);
+/* Both Gram-Schmidt passes and the coefficients of Lanczos step j in one call, reading the
+ * basis V_0..V_j three times instead of four: h1 = V^H w, w1 = w - V h1 (kept in registers,
+ * never stored), h2[0..j] = V^H w1 and h2[j+1] = |w1|^2, then w = (w1 - V h2) / beta_j with
+ * alpha_j, beta_j at ab_dev[2j], ab_dev[2j+1] (mpsd_lanczos_coef's formulas).  h1_dev holds
+ * j + 1 and h2_dev j + 2 complex values on return.  The sums run in a fixed order of their own
+ * (it depends on n and j only), so the results agree with gs1 + gs2 to rounding, not bit for
+ * bit.  j + 1 <= 32 (MPSD_ERR_UNSUPPORTED above); w need not be a column of V.
+ * Serves the eigensolver loop of Contractor::findGroundState (the PRIMME/ARPACK iterations of
+ * src/wrapper/eigs.cpp run the same orthogonalisation inside the library). */
+int mpsd_lanczos_gs_step_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, double* w, int64_t n,
+                             double* h1_dev, double* h2_dev, double* ab_dev);
 /* Projector terms of TensorMultiplierProjMulti::product (src/wrapper/multipliers/
  * TensorMultiplierProjMulti.cpp:87-96): y -= sum_j weights[j] <phi_j|x> phi_j, phi_j at
  * Phi + 2*j*ldphi, j < m <= 32; weights is a host array of m reals; x and y distinct device

@@ -350,6 +350,7 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   for (int i = 0; i < 2; ++i)
     if (ctx->stage[i]) (void)hipFree(ctx->stage[i]);
   if (ctx->dots) (void)hipFree(ctx->dots);
+  if (ctx->gs_partial) (void)hipFree(ctx->gs_partial);
   if (ctx->gtab) (void)hipFree(ctx->gtab);
   if (ctx->gtab_host) (void)hipHostFree(ctx->gtab_host);
   if (ctx->gtab_ev) (void)hipEventDestroy(ctx->gtab_ev);
@@ -474,7 +475,7 @@ void* mpsd_ctx_stream(mpsd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr
 int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx) {
   if (!ctx) return 0;
   int64_t b = ctx->tensor_bytes + ctx->pool_bytes + (int64_t)ctx->stage_bytes[0] + (int64_t)ctx->stage_bytes[1] +
-              (int64_t)ctx->dots_bytes;
+              (int64_t)ctx->dots_bytes + (int64_t)ctx->gs_partial_bytes;
   for (int i = 0; i < mpsd_ctx::kScratch; ++i) b += (int64_t)ctx->scratch_bytes[i];
   return b;
 }

@@ -145,6 +145,9 @@ double tridiag_lowest(const std::vector<double>& a, const std::vector<double>& b
 // few steps), then 4.  Shared by both launch paths so they build the same basis.
 static int lanczos_chunk(int j) { return j < 6 ? 3 : 4; }
 
+// Largest basis (j + 1 vectors) whose Gram-Schmidt step runs fused.
+static constexpr int kFusedGsCols = 32;
+
 LanczosSolver::~LanczosSolver() {
   if (lz_) mpsd_lanczos_destroy(lz_);
 }
@@ -234,9 +237,11 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
   // Restarted Lanczos with two-pass full reorthogonalisation.  The iterations run on the
   // device in chunks (lanczos_chunk): Gram-Schmidt coefficients, alpha and beta stay in device
   // memory, and the host reads the chunk's alphas/betas once to solve the tridiagonal
-  // problem.  The stopping step is the first converged one inside the chunk.  This
-  // stream-launched form and the graph-replayed form (lowest_graph) run the same operations
-  // in the same order, so they return the same bits.
+  // problem.  The stopping step is the first converged one inside the chunk.  Up to 32 basis
+  // vectors a step's two passes run fused (mpsd_lanczos_gs_step_dev: three reads of the basis,
+  // three launches below n = 32768 and four above); larger bases take the two separate passes.
+  // The graph-replayed form (lowest_graph) keeps the separate passes, so the two forms agree to
+  // rounding.
   EigStats st;
   const int64_t n = H.getSize();
   const int m = (int)std::max<int64_t>(1, std::min<int64_t>(max_krylov, n));
@@ -297,8 +302,12 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
         double* w = vec(jj + 1);
         H.product_device(vec(jj), w);
         ++st.matvecs;
-        check(ctx_, mpsd_lanczos_gs1_dev(ctx_, V, ldv, jj + 1, w, n, h1), "Gram-Schmidt pass 1");
-        check(ctx_, mpsd_lanczos_gs2_dev(ctx_, V, ldv, jj, h1, h2, ab + 2, w, n), "Gram-Schmidt pass 2 + coef");
+        if (jj + 1 <= kFusedGsCols) {
+          check(ctx_, mpsd_lanczos_gs_step_dev(ctx_, V, ldv, jj, w, n, h1, h2, ab + 2), "Gram-Schmidt step");
+        } else {
+          check(ctx_, mpsd_lanczos_gs1_dev(ctx_, V, ldv, jj + 1, w, n, h1), "Gram-Schmidt pass 1");
+          check(ctx_, mpsd_lanczos_gs2_dev(ctx_, V, ldv, jj, h1, h2, ab + 2, w, n), "Gram-Schmidt pass 2 + coef");
+        }
       }
       check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef download");
       if (j == 0) {

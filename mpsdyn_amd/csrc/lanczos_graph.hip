@@ -7,8 +7,9 @@
 // step moves j forward and patches the tables' x/y pointers.  A graph of k consecutive steps
 // therefore has fixed kernel arguments: it is captured once per solver and replayed for
 // every chunk of every site, whatever the site's shapes; binding a site only rewrites the
-// tables (one small host-to-device copy).  Arithmetic is the same as the stream-launched
-// path (same tile function, same reduction order).
+// tables (one small host-to-device copy).  The matvec is the stream-launched path's (same
+// tile function, same reduction order); the Gram-Schmidt passes run unfused here (dots, update,
+// dots, update), where the stream path fuses them, so the two paths agree to rounding.
 
 #include <algorithm>
 #include <cstring>

@@ -18,6 +18,7 @@
 
 #include "mpsd_internal.h"
 #include "vec_kernels.h"
+#include "wave_xor.h"
 
 namespace mpsd {
 namespace {
@@ -123,6 +124,236 @@ __global__ void __launch_bounds__(kT) coef_fold_kernel(const double2* __restrict
     w[t] = double2{v.x * inv, v.y * inv};
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ab[2 * j] = alpha;
+    ab[2 * j + 1] = beta;
+  }
+}
+
+// ---- Lanczos step with the Gram-Schmidt passes fused ---------------------------------------
+// Classical Gram-Schmidt applied twice to w against V_0..V_j (m1 = j + 1 vectors):
+//   h1 = V^H w,  w1 = w - V h1,  h2 = V^H w1 and |w1|^2,  w = (w1 - V h2) / beta_j.
+// The unfused form reads V four times and writes w twice.  Here w1 is never stored:
+//   dots_partial_kernel : block partials of h1 (dot_block's blocks);
+//   gs_mid_kernel       : h1 folded from those partials by one wave each; per element w1 in
+//                         registers and its terms of h2 and |w1|^2, reduced per block (nb2
+//                         blocks);
+//   gs_fold_kernel      : h2 from the block partials (one launch, or folded by a wave inside
+//                         gs_update_kernel when nb2 <= kGsFoldMax);
+//   gs_update_kernel    : w1 again, by the same operations (same bits), then w1 - V h2, scaled
+//                         (one element per thread).
+// Three passes over V, one store of w.  Every sum has an order fixed by n and j alone, so a
+// step's bits do not depend on what else runs on the device.
+// The basis columns of one element stay in registers, so the kernels are instantiated for
+// M >= m1 columns (8, 16, 24, 32).
+constexpr int kGsCols = 32;       // most basis vectors of a fused step
+constexpr int kGsMaxBlocks = 512; // partial-sum blocks of the second pass, and their stride
+constexpr int kGsFoldMax = 64;
+
+inline int gs_blocks(int64_t n) {
+  const int64_t b = (n + 511) / 512;
+  return b < 1 ? 1 : (b > kGsMaxBlocks ? kGsMaxBlocks : (int)b);
+}
+
+// w1[t] = w[t] - sum_{i<m1} V_i[t] c[i], gemv_elem's operations; the columns are left in v.
+template <int M>
+__device__ __forceinline__ double2 gs_w1(const double2* __restrict__ V, int64_t ldv, int m1,
+                                         const double2* c, const double2* __restrict__ w, int64_t t,
+                                         double2 (&v)[M]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    if (i < m1) v[i] = V[(int64_t)i * ldv + t];
+  const double2 wi = w[t];
+  double2 acc{0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    if (i < m1) {
+      const double2 ci = c[i];
+      acc.x = fma(v[i].x, ci.x, fma(-v[i].y, ci.y, acc.x));
+      acc.y = fma(v[i].x, ci.y, fma(v[i].y, ci.x, acc.y));
+    }
+  return double2{fma(-1.0, acc.x, wi.x), fma(-1.0, acc.y, wi.y)};
+}
+
+// Sum of nb <= 128 partials p[0..nb) by one wave: lane b adds p[b] and p[b + 64], then the
+// wave's butterfly.  Every lane returns the same value.
+__device__ __forceinline__ double2 gs_fold_wave(const double2* __restrict__ p, int nb) {
+  const int b = (int)__lane_id();
+  double2 s = b < nb ? p[b] : double2{0.0, 0.0};
+  if (b + 64 < nb) {
+    s.x += p[b + 64].x;
+    s.y += p[b + 64].y;
+  }
+  return double2{wave_sum_x(s.x), wave_sum_x(s.y)};
+}
+
+// Sum of acc[0..M) over the wave, scattered: returns the total of coefficient gs_lane_coef<M>()
+// (lanes l and l ^ 1 hold the same one, for M = 32; for smaller M, more lanes share one).
+// Halving steps over the lane bits from 32 down: at offset O a lane keeps the half of its
+// values selected by its bit O and adds its partner's copy of that half; the remaining lane
+// bits are summed as a butterfly.  A fixed order: the same inputs give the same bits.
+template <int M>
+__device__ __forceinline__ int gs_lane_coef() {
+  const int l = (int)__lane_id();
+  constexpr int steps = M == 32 ? 5 : (M == 16 ? 4 : 3);
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k < steps; ++k) idx = 2 * idx + ((l >> (5 - k)) & 1);
+  return idx;
+}
+
+template <int K, int O>
+__device__ __forceinline__ void gs_halve(double2* a) {
+  const bool up = (__lane_id() & O) != 0;
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) {
+    const double2 send = up ? a[k] : a[k + K / 2];
+    const double2 keep = up ? a[k + K / 2] : a[k];
+    const double2 recv = xor_lane<O>(send);
+    a[k] = double2{keep.x + recv.x, keep.y + recv.y};
+  }
+}
+
+template <int M>
+__device__ __forceinline__ double2 gs_wave_scatter(double2 (&acc)[M]) {
+  gs_halve<M, 32>(acc);
+  gs_halve<M / 2, 16>(acc);
+  gs_halve<M / 4, 8>(acc);
+  if constexpr (M >= 16) gs_halve<M / 8, 4>(acc);
+  if constexpr (M >= 32) gs_halve<M / 16, 2>(acc);
+  double2 r = acc[0];
+  if constexpr (M <= 8) r = double2{r.x + xor_lane<4>(r).x, r.y + xor_lane<4>(r).y};
+  if constexpr (M <= 16) r = double2{r.x + xor_lane<2>(r).x, r.y + xor_lane<2>(r).y};
+  r = double2{r.x + xor_lane<1>(r).x, r.y + xor_lane<1>(r).y};
+  return r;
+}
+
+template <int M>
+__global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ V, int64_t ldv, int m1,
+                                                    const double2* __restrict__ p1, int nb1,
+                                                    const double2* __restrict__ w, int64_t n, int nb2,
+                                                    double2* __restrict__ h1, double2* __restrict__ p2) {
+  __shared__ double2 c[M];
+  __shared__ double2 red[kT / 64][M + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m1; i += kT / 64) {
+    const double2 s = gs_fold_wave(p1 + (int64_t)i * kDotBlocks, nb1);
+    if (lane == 0) {
+      c[i] = s;
+      if (blockIdx.x == 0) h1[i] = s;
+    }
+  }
+  __syncthreads();
+  constexpr int S = M <= 8 ? 8 : (M <= 16 ? 16 : 32);  // accumulators: a power of two for the scatter
+  double2 acc[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) acc[i] = double2{0.0, 0.0};
+  double nrm = 0.0;
+  const int64_t chunk = (n + nb2 - 1) / nb2;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = min(n, lo + chunk);
+  for (int64_t t = lo + threadIdx.x; t < hi; t += kT) {
+    // h1 is read from LDS at its use: kept in registers across the loop it would cost 4 M
+    // VGPRs on top of the columns and the accumulators
+    __asm__ volatile("" ::: "memory");
+    double2 v[M];
+    const double2 w1 = gs_w1<M>(V, ldv, m1, c, w, t, v);
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+      if (i < m1) {  // dot_block's operations
+        acc[i].x = fma(v[i].x, w1.x, fma(v[i].y, w1.y, acc[i].x));
+        acc[i].y = fma(v[i].x, w1.y, fma(-v[i].y, w1.x, acc[i].y));
+      }
+    nrm = fma(w1.x, w1.x, fma(w1.y, w1.y, nrm));
+  }
+  const double2 r = gs_wave_scatter<S>(acc);
+  constexpr int share = 64 / S;  // lanes holding one coefficient
+  const int coef = gs_lane_coef<S>();
+  if (lane % share == 0 && coef < M) red[wave][coef] = r;
+  nrm = wave_sum_x(nrm);
+  __syncthreads();  // red[.][m1] is written after every coefficient slot has been
+  if (lane == 0) red[wave][m1] = double2{nrm, 0.0};
+  __syncthreads();
+  if ((int)threadIdx.x <= m1) {
+    double2 s = red[0][threadIdx.x];
+    for (int k = 1; k < kT / 64; ++k) {
+      s.x += red[k][threadIdx.x].x;
+      s.y += red[k][threadIdx.x].y;
+    }
+    p2[(int64_t)threadIdx.x * kGsMaxBlocks + blockIdx.x] = s;
+  }
+}
+
+// h2 from the nb2 block partials of gs_mid_kernel (slot i at p2 + i * kGsMaxBlocks), i <= m1
+// (slot m1 is |w1|^2): p[t] + p[t + 256] per thread, each wave's butterfly, then the four
+// waves in order.  One block per slot; used when nb2 > kGsFoldMax.
+__global__ void __launch_bounds__(kT) gs_fold_kernel(const double2* __restrict__ p2, int nb2,
+                                                     double2* __restrict__ h2) {
+  __shared__ double2 red[kT / 64];
+  const double2* p = p2 + (int64_t)blockIdx.x * kGsMaxBlocks;
+  const int t = threadIdx.x;
+  double2 s = t < nb2 ? p[t] : double2{0.0, 0.0};
+  if (t + kT < nb2) {
+    s.x += p[t + kT].x;
+    s.y += p[t + kT].y;
+  }
+  s.x = wave_sum_x(s.x);
+  s.y = wave_sum_x(s.y);
+  if ((t & 63) == 0) red[t >> 6] = s;
+  __syncthreads();
+  if (t == 0) {
+    double2 r = red[0];
+    for (int k = 1; k < kT / 64; ++k) {
+      r.x += red[k].x;
+      r.y += red[k].y;
+    }
+    h2[blockIdx.x] = r;
+  }
+}
+
+// One element per thread (grid ceil(n / kT)), so the coefficients are read from LDS once.
+template <int M>
+__global__ void __launch_bounds__(kT) gs_update_kernel(const double2* __restrict__ V, int64_t ldv, int m1,
+                                                       const double2* __restrict__ h1,
+                                                       const double2* __restrict__ p2, int nb2, int fold,
+                                                       double2* __restrict__ h2, double* __restrict__ ab,
+                                                       double2* __restrict__ w, int64_t n) {
+  __shared__ double2 c1[M];
+  __shared__ double2 c2[M + 1];
+  if ((int)threadIdx.x < m1) c1[threadIdx.x] = h1[threadIdx.x];
+  if (fold) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = wave; i <= m1; i += kT / 64) {
+      const double2 s = gs_fold_wave(p2 + (int64_t)i * kGsMaxBlocks, nb2);
+      if (lane == 0) {
+        c2[i] = s;
+        if (blockIdx.x == 0) h2[i] = s;
+      }
+    }
+  } else if ((int)threadIdx.x <= m1) {
+    c2[threadIdx.x] = h2[threadIdx.x];
+  }
+  __syncthreads();
+  const int j = m1 - 1;
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t < n) {
+    double2 v[M];
+    const double2 w1 = gs_w1<M>(V, ldv, m1, c1, w, t, v);
+    double2 acc{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+      if (i < m1) {
+        const double2 ci = c2[i];
+        acc.x = fma(v[i].x, ci.x, fma(-v[i].y, ci.y, acc.x));
+        acc.y = fma(v[i].x, ci.y, fma(v[i].y, ci.x, acc.y));
+      }
+    const double2 w2{fma(-1.0, acc.x, w1.x), fma(-1.0, acc.y, w1.y)};
+    double alpha, beta, inv;
+    lanczos_scalars(c1, c2, j, alpha, beta, inv);
+    w[t] = double2{w2.x * inv, w2.y * inv};
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double alpha, beta, inv;
+    lanczos_scalars(c1, c2, j, alpha, beta, inv);
     ab[2 * j] = alpha;
     ab[2 * j + 1] = beta;
   }
@@ -494,6 +725,50 @@ int mpsd_lanczos_gs2_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, con
                      (const double2*)h1_dev, (const double2*)partial, nb, j, ab_dev, (double2*)w, n,
                      (double2*)h2_dev);
   return hip_err(ctx, hipGetLastError(), "lanczos_gs2");
+}
+
+int mpsd_lanczos_gs_step_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j, double* w, int64_t n,
+                             double* h1_dev, double* h2_dev, double* ab_dev) {
+  if (!ctx || !V || !w || !h1_dev || !h2_dev || !ab_dev || j < 0 || n <= 0 || (j > 0 && ldv < n))
+    return MPSD_ERR_ARG;
+  const int m1 = j + 1;
+  if (m1 > kGsCols) return set_err(ctx, MPSD_ERR_UNSUPPORTED, "lanczos_gs_step: at most 32 basis vectors");
+  int rc = ensure_dots(ctx, m1);
+  if (rc) return rc;
+  if (!ctx->gs_partial) {
+    const size_t bytes = sizeof(double2) * (size_t)(kGsCols + 1) * kGsMaxBlocks;
+    hipError_t e = hipMalloc(&ctx->gs_partial, bytes);
+    if (e != hipSuccess) {
+      ctx->gs_partial = nullptr;
+      return hip_err(ctx, e, "gs partial alloc");
+    }
+    ctx->gs_partial_bytes = bytes;
+  }
+  hipStream_t s = ctx->stream;
+  const double2* Vc = (const double2*)V;
+  double2* p1 = (double2*)ctx->dots;
+  double2* p2 = (double2*)ctx->gs_partial;
+  double2* h1 = (double2*)h1_dev;
+  double2* h2 = (double2*)h2_dev;
+  const int nb1 = dot_blocks(n), nb2 = gs_blocks(n);
+  const int fold = nb2 <= kGsFoldMax ? 1 : 0;
+  hipLaunchKernelGGL(dots_partial_kernel, dim3(nb1, m1), dim3(kT), 0, s, Vc, ldv, (const double2*)w, n, nb1, p1);
+  auto steps = [&](auto mid, auto upd) {
+    hipLaunchKernelGGL(mid, dim3(nb2), dim3(kT), 0, s, Vc, ldv, m1, (const double2*)p1, nb1, (const double2*)w, n,
+                       nb2, h1, p2);
+    if (!fold) hipLaunchKernelGGL(gs_fold_kernel, dim3(m1 + 1), dim3(kT), 0, s, (const double2*)p2, nb2, h2);
+    hipLaunchKernelGGL(upd, dim3(nblk(n)), dim3(kT), 0, s, Vc, ldv, m1, (const double2*)h1, (const double2*)p2,
+                       nb2, fold, h2, ab_dev, (double2*)w, n);
+  };
+  if (m1 <= 8)
+    steps(gs_mid_kernel<8>, gs_update_kernel<8>);
+  else if (m1 <= 16)
+    steps(gs_mid_kernel<16>, gs_update_kernel<16>);
+  else if (m1 <= 24)
+    steps(gs_mid_kernel<24>, gs_update_kernel<24>);
+  else
+    steps(gs_mid_kernel<32>, gs_update_kernel<32>);
+  return hip_err(ctx, hipGetLastError(), "lanczos_gs_step");
 }
 
 int mpsd_vec_normalize_dev(mpsd_ctx* ctx, const double* x, double* y, int64_t n, double* nrm2_dev) {

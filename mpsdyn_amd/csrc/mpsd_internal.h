@@ -153,6 +153,8 @@ struct mpsd_ctx {
   void* blas = nullptr;
   void* dots = nullptr;
   size_t dots_bytes = 0;
+  void* gs_partial = nullptr;  // second-pass block partials of the fused Gram-Schmidt step
+  size_t gs_partial_bytes = 0;
   void* coef = nullptr;       // device ring, 8 slots x 1024 complex
   void* coef_host = nullptr;  // pinned host ring, same layout
   hipEvent_t coef_ev[8] = {nullptr};

@@ -113,6 +113,7 @@ def _optional(lib):
         "mpsd_lanczos_coef_update": (I, [P, P, I64, P, P, I, P, P, I64]),
         "mpsd_lanczos_gs1_dev": (I, [P, P, I64, I, P, I64, P]),
         "mpsd_lanczos_gs2_dev": (I, [P, P, I64, I, P, P, P, P, I64]),
+        "mpsd_lanczos_gs_step_dev": (I, [P, P, I64, I, P, I64, P, P, P]),
         "mpsd_site_gauge_r_svd": (I, [P, P, P, I]),
         "mpsd_site_gauge_l_svd": (I, [P, P, P, I]),
         "mpsd_mps_truncate": (I, [P, C.c_char_p, I, C.c_char_p]),

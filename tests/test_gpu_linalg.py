@@ -86,6 +86,93 @@ def test_gram_schmidt_passes_bit_identical(ctx, n, j):
     assert np.allclose(h1d, V[:, : j + 1].conj().T @ V[:, j + 1], rtol=1e-12, atol=1e-12 * np.sqrt(n))
 
 
+def _cgs2_numpy(V, w):
+    """Classical Gram-Schmidt applied twice, the Lanczos step's coefficients (numpy)."""
+    h1 = V.conj().T @ w
+    w1 = w - V @ h1
+    h2 = V.conj().T @ w1
+    nw1 = np.vdot(w1, w1).real
+    beta = np.sqrt(nw1 - np.vdot(h2, h2).real)
+    return h1, h2, nw1, (w1 - V @ h2) / beta, (h1[-1] + h2[-1]).real, beta
+
+
+# n chosen for both ways of summing the second pass (n <= 32768: folded into the update
+# kernel; above: a separate launch), j for each register width (8, 16, 24, 32 columns)
+@pytest.mark.parametrize("n,j,pad", [(1, 0, 0), (300, 2, 0), (2049, 7, 0), (32768, 15, 0), (32769, 16, 3),
+                                     (131072, 23, 0), (70000, 24, 0), (300001, 29, 5), (5000, 31, 0)])
+def test_gram_schmidt_step_fused(ctx, n, j, pad):
+    """mpsd_lanczos_gs_step_dev against numpy's CGS2 and against the two unfused passes: a
+    random w (well conditioned), and a w that is the basis plus 1e-7 noise (as H v_j is late in
+    a Lanczos run: the second pass carries the orthogonality)."""
+    print(f"fused gram-schmidt n={n} j={j} ldv={n + pad}", flush=True)
+    m1 = j + 1
+    ldv = n + pad
+    Q, _ = np.linalg.qr(orc.complex_gaussian(11 + j, (n, m1)))
+    exhausted = n <= m1  # w lies in the span of V: beta is rounding noise (the solver stops there)
+    for case in ("random",) if exhausted else ("random", "near_span"):
+        w = orc.complex_gaussian(12 + j, (n,))
+        if case == "near_span":
+            w = Q @ orc.complex_gaussian(13 + j, (m1,)) + 1e-7 * w
+        tight = case == "random"
+        if exhausted:
+            h1r = Q.conj().T @ w
+        else:
+            h1r, h2r, nw1, w2r, alpha, beta = _cgs2_numpy(Q, w)
+        Vp = np.zeros((ldv, m1), dtype=np.complex128)
+        Vp[:n] = Q
+        tV, tw = ctx.tensor(Vp), ctx.tensor(w)
+        h1, h2, ab = (ctx.tensor(np.zeros(k, dtype=np.complex128)) for k in (m1, m1 + 1, m1 + 1))
+        ctx.check(ctx.lib.mpsd_lanczos_gs_step_dev(ctx.handle, tV.devptr(), ldv, j, tw.devptr(), n, h1.devptr(),
+                                                   h2.devptr(), ab.devptr()))
+        wo, g1, g2 = tw.download(), h1.download(), h2.download()
+        abv = ab.download().view(np.float64)
+        scale = np.linalg.norm(w)
+        assert np.allclose(g1, h1r, rtol=0, atol=1e-12 * scale)
+        if exhausted:
+            assert abv[2 * j + 1] <= 1e-14 * scale
+            continue
+        assert np.allclose(g2[:m1], h2r, rtol=0, atol=1e-12 * scale)
+        assert abs(g2[m1].real - nw1) <= (1e-12 if tight else 1e-8) * nw1
+        assert abs(abv[2 * j] - alpha) <= 1e-12 * scale
+        assert abs(abv[2 * j + 1] - beta) <= (1e-12 if tight else 1e-7) * beta
+        assert np.abs(Q.conj().T @ wo).max() <= 1e-12
+        assert abs(np.linalg.norm(wo) - 1.0) <= 1e-12
+        assert np.allclose(wo, w2r, rtol=0, atol=1e-13 if tight else 1e-8)
+        # the unfused passes (j + 1 basis vectors, w in column j + 1)
+        Vu = np.zeros((ldv, m1 + 1), dtype=np.complex128)
+        Vu[:n, :m1], Vu[:n, m1] = Q, w
+        tU = ctx.tensor(Vu)
+        u1, u2, uab = (ctx.tensor(np.zeros(k, dtype=np.complex128)) for k in (m1, m1 + 1, m1 + 1))
+        base = tU.devptr()
+        ctx.check(ctx.lib.mpsd_lanczos_gs1_dev(ctx.handle, base, ldv, m1, base + m1 * ldv * 16, n, u1.devptr()))
+        ctx.check(ctx.lib.mpsd_lanczos_gs2_dev(ctx.handle, base, ldv, j, u1.devptr(), u2.devptr(), uab.devptr(),
+                                               base + m1 * ldv * 16, n))
+        wu = np.asfortranarray(tU.download())[:n, m1]
+        assert np.allclose(wo, wu, rtol=0, atol=1e-13 if tight else 1e-8)
+        ua = uab.download().view(np.float64)
+        assert abs(abv[2 * j] - ua[2 * j]) <= 1e-12 * scale
+        assert abs(abv[2 * j + 1] - ua[2 * j + 1]) <= (1e-12 if tight else 1e-7) * beta
+    # the same call twice gives the same bits
+    outs = []
+    for _ in range(2):
+        tV, tw = ctx.tensor(Vp), ctx.tensor(w)
+        h1, h2, ab = (ctx.tensor(np.zeros(k, dtype=np.complex128)) for k in (m1, m1 + 1, m1 + 1))
+        ctx.check(ctx.lib.mpsd_lanczos_gs_step_dev(ctx.handle, tV.devptr(), ldv, j, tw.devptr(), n, h1.devptr(),
+                                                   h2.devptr(), ab.devptr()))
+        outs.append(np.concatenate([tw.download(), h1.download(), h2.download(), ab.download()]))
+    assert np.array_equal(outs[0].view(np.float64), outs[1].view(np.float64))
+
+
+def test_gram_schmidt_step_fused_limit(ctx):
+    """Above 32 basis vectors the fused step refuses (the solver then takes the two passes)."""
+    n, j = 64, 32
+    tV = ctx.tensor(np.zeros((n, j + 2), dtype=np.complex128))
+    h = ctx.tensor(np.zeros(j + 2, dtype=np.complex128))
+    rc = ctx.lib.mpsd_lanczos_gs_step_dev(ctx.handle, tV.devptr(), n, j, tV.devptr() + (j + 1) * n * 16, n,
+                                          h.devptr(), h.devptr(), h.devptr())
+    assert rc == -6 and "32" in ctx.lib.mpsd_last_error(ctx.handle).decode()
+
+
 @pytest.mark.parametrize("d,Dl,Dr", [(2, 8, 16), (2, 16, 8), (2, 1, 2), (2, 64, 64), (2, 4, 1), (3, 5, 7)])
 def test_gauge_r_and_reduce(ctx, d, Dl, Dr):
     A = orc.complex_gaussian(7 + Dl, (d, Dl, Dr))
