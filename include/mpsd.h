@@ -72,7 +72,8 @@ int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches
  * Each timed launch adds two event records to the stream, so timing only the kind being
  * measured keeps the others' schedule as it is untimed. */
 int mpsd_ctx_set_timing_kinds(mpsd_ctx* ctx, unsigned mask);
-/* Force the BRGEMM tile (tm, tn in {32, 64}); 0, 0 restores the built-in choice. */
+/* Force the BRGEMM tile (tm, tn in {32, 64}, or 16, 16: one wave per workgroup); 0, 0 restores
+ * the built-in choice (32 x 32 for FP64).  Every tile gives the same bits. */
 int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
 /* Other launch options of the context (used by the tile tuner). */
 #define MPSD_OPT_TILE_K 1       /* K depth of a BRGEMM LDS stage: 16 (default) or 32 */

@@ -405,8 +405,8 @@ static void update_tile_user(mpsd_ctx* ctx) {
 
 int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn) {
   if (!ctx) return MPSD_ERR_ARG;
-  const bool ok = (tm == 0 && tn == 0) || ((tm == 32 || tm == 64) && (tn == 32 || tn == 64));
-  if (!ok) return set_err(ctx, MPSD_ERR_ARG, "set_tile: tm, tn must be 32 or 64 (or both 0)");
+  const bool ok = (tm == 0 && tn == 0) || (tm == 16 && tn == 16) || ((tm == 32 || tm == 64) && (tn == 32 || tn == 64));
+  if (!ok) return set_err(ctx, MPSD_ERR_ARG, "set_tile: tm, tn must be 32 or 64, both 16 or both 0");
   ctx->tile_override.tm = tm;
   ctx->tile_override.tn = tn;
   update_tile_user(ctx);

@@ -46,7 +46,7 @@ namespace {
 // Candidates: the built-in choice first (kept unless another is clearly faster).
 const TileCfg kCandidates[] = {
     {0, 0, 16, 0},  {32, 32, 16, 1}, {32, 32, 32, 0}, {32, 32, 32, 1}, {64, 32, 16, 0},
-    {64, 32, 16, 1}, {32, 64, 16, 0}, {32, 64, 16, 1}, {64, 64, 16, 0},
+    {64, 32, 16, 1}, {32, 64, 16, 0}, {32, 64, 16, 1}, {64, 64, 16, 0}, {16, 16, 16, 0},
 };
 constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
 constexpr double kKeepBuiltin = 0.97;  // a candidate must beat the built-in by 3%

@@ -25,6 +25,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int kThreads = 256;
+// Threads of a BRGEMM workgroup: 2 x 2 waves, each with a (TM/2) x (TN/2) block of the tile;
+// a 16 x 16 tile is one wave.
+constexpr int tile_threads(int tm, int tn) { return tm * tn <= 256 ? 64 : kThreads; }
 
 // Arithmetic of the BRGEMM's MFMAs.  Operands always arrive as FP64 from HBM and leave as
 // FP64; the FP32 form converts when staging into LDS and accumulates in FP32.
@@ -132,16 +135,18 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
   constexpr int NB_ = BSH ? 1 : NBW;
   constexpr int NBUF = Smem<TM, TN, TK, NBW, ASH, BSH, ARITH>::NBUF;
 
-  constexpr int WTM = TM / 2, WTN = TN / 2;  // 2x2 waves
+  constexpr int NT = tile_threads(TM, TN);
+  constexpr int WG = NT / 64 == 4 ? 2 : 1;   // waves along m and along n
+  constexpr int WTM = TM / WG, WTN = TN / WG;
   constexpr int BM = WTM / 16, BN = WTN / 16;
-  constexpr int AE = TM * TK / kThreads;
-  constexpr int BE = TK * TN / kThreads;
-  static_assert(AE >= 1 && BE >= 1, "tile too small for 256 threads");
+  constexpr int AE = TM * TK / NT;
+  constexpr int BE = TK * TN / NT;
+  static_assert(AE >= 1 && BE >= 1 && BM >= 1 && BN >= 1, "tile too small for its threads");
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = WG == 2 ? (wave & 1) : 0, wn = WG == 2 ? (wave >> 1) : 0;
   const int nbw = min(NBW, p.NB - z * NBW);
 
   // FP64 takes the three-product form of the complex product (P1 = ar*br, P2 = ai*bi,
@@ -166,7 +171,7 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
   const int ntiles = nk * p.R;
 
   auto a_coord = [&](int e, int& m, int& k) {
-    const int idx = tid + kThreads * e;
+    const int idx = tid + NT * e;
     if (p.a_mfast) {
       m = idx % TM;
       k = idx / TM;
@@ -176,7 +181,7 @@ __device__ __forceinline__ void brgemm_tile(const ZgemmDesc& p, int m0, int n0, 
     }
   };
   auto b_coord = [&](int e, int& k, int& n) {
-    const int idx = tid + kThreads * e;
+    const int idx = tid + NT * e;
     if (p.b_kfast) {
       k = idx % TK;
       n = idx / TK;
@@ -363,7 +368,7 @@ constexpr int min_waves(int tm, int tn, int tk, int nbw) {
 }
 
 template <int TM, int TN, int TK, int NBW, bool ASH, bool BSH, int ARITH>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(min_waves(TM, TN, TK, NBW)))) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
+__global__ void __launch_bounds__(tile_threads(TM, TN)) __attribute__((amdgpu_waves_per_eu(min_waves(TM, TN, TK, NBW)))) zbrgemm_kernel(const ZgemmDesc p, int gm, int gn) {
   __shared__ Smem<TM, TN, TK, NBW, ASH, BSH, ARITH> sm;
   int tm_i, tn_i, z;
   tile_of(gm, gn, tm_i, tn_i, z);
@@ -396,7 +401,8 @@ hipError_t launch_cfg_a(const ZgemmDesc& p, hipStream_t s) {
   const int gm = (p.M + TM - 1) / TM, gn = (p.N + TN - 1) / TN, gz = (p.NB + NBW - 1) / NBW;
   const int64_t total = (int64_t)gm * gn * gz;
   if (total > 0x7fffffff) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((zbrgemm_kernel<TM, TN, TK, NBW, ASH, BSH, ARITH>), dim3((unsigned)total), dim3(kThreads), 0,
+  hipLaunchKernelGGL((zbrgemm_kernel<TM, TN, TK, NBW, ASH, BSH, ARITH>), dim3((unsigned)total),
+                     dim3(tile_threads(TM, TN)), 0,
                      s, p, gm, gn);
   return hipGetLastError();
 }
@@ -417,6 +423,7 @@ hipError_t launch_tile(const ZgemmDesc& p, int nbw, hipStream_t s) {
 
 template <int TK>
 hipError_t launch_tk(const ZgemmDesc& p, TileCfg cfg, int nbw, hipStream_t s) {
+  if (cfg.tm == 16 && cfg.tn == 16) return launch_tile<16, 16, TK>(p, nbw, s);
   if (cfg.tm == 64 && cfg.tn == 64) return launch_tile<64, 64, TK>(p, nbw, s);
   if (cfg.tm == 64 && cfg.tn == 32) return launch_tile<64, 32, TK>(p, nbw, s);
   if (cfg.tm == 32 && cfg.tn == 64) return launch_tile<32, 64, TK>(p, nbw, s);
@@ -484,12 +491,21 @@ hipError_t launch_zbrgemm_grouped(const GroupedProblem* d, int n, int grid, int 
 hipError_t launch_zbrgemm_f64(const ZgemmDesc& pin, TileCfg cfg, hipStream_t s) {
   if (pin.M <= 0 || pin.N <= 0 || pin.NB <= 0) return hipSuccess;
   ZgemmDesc p = pin;
+  if (cfg.nbw == 0 && p.group == 0 && p.NB >= 2 && p.arith != 2 && (p.sAb == 0 || p.sBb == 0) &&
+      (int64_t)((p.M + 31) / 32) * ((p.N + 31) / 32) * ((p.NB + 1) / 2) < 512) {
+    // Two batches per workgroup share the common operand's LDS tile.  Below 512 such 32x32
+    // workgroups (D <= 256 at w = 3), one batch each: twice the workgroups, half the MFMA
+    // chains per wave; 8-12% faster matvecs at D = 64..256.  The sums are the same either way.
+    cfg.nbw = 1;
+  }
   const int nbw = choose_group(p, cfg.nbw);
   if (nbw < 0) return hipErrorInvalidValue;
   if (p.arith == 2) return launch_zbrgemm_bf16x3(p, nbw, s);  // fixed 32x32 tiles
   if ((cfg.tm == 0 || cfg.tn == 0) && p.arith == 0) {
-    // FP64: 32x32 on every shape measured (D = 128..1024, w = 3, 5); larger FP64 tiles hold
-    // one or two waves per SIMD, too few to keep the FP64 MFMA issuing.
+    // FP64: 32x32 on every shape measured (D = 64..1024, w = 3, 5).  Larger FP64 tiles hold
+    // one or two waves per SIMD, too few to keep the FP64 MFMA issuing.  One-wave 16x16 tiles
+    // (four times the workgroups, same bits) measured 5-14% slower at D = 64..256: a wave's
+    // chain of K stages is as long as in the 32x32 tile, and it loads four times the operands.
     cfg.tm = 32, cfg.tn = 32;
   } else if (cfg.tm == 0 || cfg.tn == 0) {
     // Largest tile that still gives >= 512 workgroups (two per CU); 32x32 otherwise.

@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 CONFIGS = [(0, 0, 16, 0), (32, 32, 16, 1), (32, 32, 32, 0), (32, 32, 32, 1), (64, 32, 16, 0),
-           (64, 32, 16, 1), (32, 64, 16, 0), (32, 64, 16, 1), (64, 64, 16, 0), (64, 64, 32, 2)]
+           (64, 32, 16, 1), (32, 64, 16, 0), (32, 64, 16, 1), (64, 64, 16, 0), (64, 64, 32, 2),
+           (16, 16, 16, 0), (16, 16, 32, 0), (16, 16, 16, 1), (16, 16, 32, 2)]
 
 
 @pytest.fixture(scope="module")
