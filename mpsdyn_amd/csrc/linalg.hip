@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kT) coef_fold_kernel(const double2* __restrict
 //   h1 = V^H w,  w1 = w - V h1,  h2 = V^H w1 and |w1|^2,  w = (w1 - V h2) / beta_j.
 // The unfused form reads V four times and writes w twice.  Here w1 is never stored:
 //   dots_partial_kernel : block partials of h1 (dot_block's blocks);
-//   gs_mid_kernel       : h1 folded from those partials by one wave each; per element w1 in
+//   gs_mid_kernel       : h1 folded from those partials, one wave per coefficient; per element w1 in
 //                         registers and its terms of h2 and |w1|^2, reduced per block (nb2
 //                         blocks);
 //   gs_fold_kernel      : h2 from the block partials (one launch, or folded by a wave inside
@@ -147,10 +147,11 @@ __global__ void __launch_bounds__(kT) coef_fold_kernel(const double2* __restrict
 // M >= m1 columns (8, 16, 24, 32).
 constexpr int kGsCols = 32;       // most basis vectors of a fused step
 constexpr int kGsMaxBlocks = 512; // partial-sum blocks of the second pass, and their stride
-constexpr int kGsFoldMax = 64;
+constexpr int kGsFoldMax = 128;  // gs_fold_waves' limit
 
+// one block per 256 elements (one per thread) up to kGsMaxBlocks
 inline int gs_blocks(int64_t n) {
-  const int64_t b = (n + 511) / 512;
+  const int64_t b = (n + 255) / 256;
   return b < 1 ? 1 : (b > kGsMaxBlocks ? kGsMaxBlocks : (int)b);
 }
 
@@ -174,16 +175,37 @@ __device__ __forceinline__ double2 gs_w1(const double2* __restrict__ V, int64_t 
   return double2{fma(-1.0, acc.x, wi.x), fma(-1.0, acc.y, wi.y)};
 }
 
-// Sum of nb <= 128 partials p[0..nb) by one wave: lane b adds p[b] and p[b + 64], then the
-// wave's butterfly.  Every lane returns the same value.
-__device__ __forceinline__ double2 gs_fold_wave(const double2* __restrict__ p, int nb) {
-  const int b = (int)__lane_id();
-  double2 s = b < nb ? p[b] : double2{0.0, 0.0};
-  if (b + 64 < nb) {
-    s.x += p[b + 64].x;
-    s.y += p[b + 64].y;
+// Sums of nb <= 128 partials, coefficient i at p + i * ld for i = wave, wave + 4, ... < nc
+// (nc <= C): lane b adds p[b] and p[b + 64], then the wave's butterfly; lane 0 of the wave
+// passes (i, sum) to put.  All of a wave's loads are issued before the first butterfly.
+template <int C, class Put>
+__device__ __forceinline__ void gs_fold_waves(const double2* __restrict__ p, int64_t ld, int nb, int nc,
+                                              Put put) {
+  constexpr int kIt = (C + 3) / 4;
+  const int wave = threadIdx.x >> 6, b = (int)__lane_id();
+  double2 s[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = wave + 4 * it;
+    s[it] = double2{0.0, 0.0};
+    if (i < nc) {
+      const double2* q = p + (int64_t)i * ld;
+      if (b < nb) s[it] = q[b];
+      if (b + 64 < nb) {
+        const double2 u = q[b + 64];
+        s[it].x += u.x;
+        s[it].y += u.y;
+      }
+    }
   }
-  return double2{wave_sum_x(s.x), wave_sum_x(s.y)};
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int i = wave + 4 * it;
+    if (i < nc) {
+      const double2 r{wave_sum_x(s[it].x), wave_sum_x(s[it].y)};
+      if (b == 0) put(i, r);
+    }
+  }
 }
 
 // Sum of acc[0..M) over the wave, scattered: returns the total of coefficient gs_lane_coef<M>()
@@ -235,13 +257,10 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
   __shared__ double2 c[M];
   __shared__ double2 red[kT / 64][M + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = wave; i < m1; i += kT / 64) {
-    const double2 s = gs_fold_wave(p1 + (int64_t)i * kDotBlocks, nb1);
-    if (lane == 0) {
-      c[i] = s;
-      if (blockIdx.x == 0) h1[i] = s;
-    }
-  }
+  gs_fold_waves<M>(p1, kDotBlocks, nb1, m1, [&](int i, double2 v) {
+    c[i] = v;
+    if (blockIdx.x == 0) h1[i] = v;
+  });
   __syncthreads();
   constexpr int S = M <= 8 ? 8 : (M <= 16 ? 16 : 32);  // accumulators: a power of two for the scatter
   double2 acc[S];
@@ -321,14 +340,10 @@ __global__ void __launch_bounds__(kT) gs_update_kernel(const double2* __restrict
   __shared__ double2 c2[M + 1];
   if ((int)threadIdx.x < m1) c1[threadIdx.x] = h1[threadIdx.x];
   if (fold) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = wave; i <= m1; i += kT / 64) {
-      const double2 s = gs_fold_wave(p2 + (int64_t)i * kGsMaxBlocks, nb2);
-      if (lane == 0) {
-        c2[i] = s;
-        if (blockIdx.x == 0) h2[i] = s;
-      }
-    }
+    gs_fold_waves<M + 1>(p2, kGsMaxBlocks, nb2, m1 + 1, [&](int i, double2 v) {
+      c2[i] = v;
+      if (blockIdx.x == 0) h2[i] = v;
+    });
   } else if ((int)threadIdx.x <= m1) {
     c2[threadIdx.x] = h2[threadIdx.x];
   }
