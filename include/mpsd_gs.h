@@ -38,8 +38,9 @@ typedef struct mpsd_gs_config {
   int breakdown;      /* 1: time the sweep's phases (t_env, t_eig, t_gauge, t_matvec of the result);
                          the stream is drained at each phase boundary */
   int tol_schedule;   /* 0: every local problem solved to eigtol (the reference's behaviour);
-                         1: FP64 local solves of a half-sweep to max(eigtol, min(1e-3, |dE|/|E| of the
-                         previous half-sweep)), loose while the sweep is far from converged */
+                         1: FP64 local solves of a half-sweep to max(eigtol, min(1e-3, 1e-2 |dE|/|E| of
+                         the previous half-sweep, 1e-2 times the previous half-sweep's tolerance)), loose
+                         while the sweep is far from converged (the precision forms always schedule) */
 } mpsd_gs_config;
 
 #define MPSD_SOLVER_LANCZOS 0 /* device Lanczos on the matvec (in place of primme / arpack) */
@@ -70,6 +71,10 @@ typedef struct mpsd_gs_result {
   double env_flops;       /* algorithmic flops of all environment updates */
   double residual_floor;  /* with cfg->want_residual: the rounding level of residual_norm (the square root
                              of the rounding of <(H - E)^2>); residuals below it are not resolved */
+  /* per half-sweep (the first n_history): wall seconds, matvecs, local eigensolver tolerance */
+  double hs_seconds[MPSD_GS_MAX_HISTORY];
+  double hs_tol[MPSD_GS_MAX_HISTORY];
+  int64_t hs_matvecs[MPSD_GS_MAX_HISTORY];
 } mpsd_gs_result;
 
 /* Run one solve on ctx.  Returns MPSD_OK or an MPSD_ERR_* code (message: mpsd_last_error). */

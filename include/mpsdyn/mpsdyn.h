@@ -396,6 +396,9 @@ struct SolveStats {
   int64_t env_updates = 0;
   double final_residual = 0.0;
   std::vector<double> energies;  // energy at the end of each half-sweep
+  // per half-sweep: wall seconds, matvecs, and the local eigensolver tolerance it ran at
+  std::vector<double> hs_seconds, hs_tol;
+  std::vector<int64_t> hs_matvecs;
   // Wall-clock split of findGroundState's sweep in seconds, filled when the breakdown is on
   // (Contractor::setBreakdown): environment updates, local eigensolves (matvecs plus the
   // eigensolver's vector work), gauge steps; t_matvec is the matvec kernels' time inside the
@@ -435,9 +438,9 @@ class Contractor {
   // boundary, which costs a little time, so it is off by default.
   void setBreakdown(bool on) { breakdown_ = on; }
   // Local-tolerance schedule of findGroundState: off (default) solves every local problem to
-  // the eigensolver tolerance, as the reference does; on, the FP64 local solves of a half-sweep
-  // stop at max(eigtol, min(1e-3, |dE| / |E| of the previous half-sweep)), 1e-3 at the start
-  // (the FP64 form only: the mixed forms' precision ladder schedules its own tolerances).
+  // the eigensolver tolerance, as the reference does; on, the local solves of a half-sweep stop
+  // at max(eigtol, min(1e-3, 1e-2 |dE| / |E| of the previous half-sweep, 1e-2 times the previous
+  // half-sweep's tolerance)), 1e-3 at the start.  The precision forms always run the schedule.
   void setEigTolSchedule(bool on) { schedule_ = on; }
   // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).
   void findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset = 0.,

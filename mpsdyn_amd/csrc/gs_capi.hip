@@ -89,7 +89,12 @@ void fill_result(mpsd_ctx* ctx, const Contractor& c, double E, double seconds, m
   out->matvecs_lowp = st.matvecs_lowp;
   out->refine_from = st.refine_from;
   out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
-  for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+  for (int i = 0; i < out->n_history; ++i) {
+    out->history[i] = st.energies[i];
+    out->hs_seconds[i] = i < (int)st.hs_seconds.size() ? st.hs_seconds[i] : 0.0;
+    out->hs_tol[i] = i < (int)st.hs_tol.size() ? st.hs_tol[i] : 0.0;
+    out->hs_matvecs[i] = i < (int)st.hs_matvecs.size() ? st.hs_matvecs[i] : 0;
+  }
   out->residual_norm = -1.0;
   out->residual_floor = -1.0;
   out->matvec_flops = st.matvec_flops;
@@ -201,7 +206,12 @@ int gs_solve_mpo_with(mpsd_ctx* ctx, const mpsd_gs_config* cfg, const MPO& hmpo,
     out->matvecs_lowp = st.matvecs_lowp;
     out->refine_from = st.refine_from;
     out->n_history = (int)std::min<size_t>(st.energies.size(), MPSD_GS_MAX_HISTORY);
-    for (int i = 0; i < out->n_history; ++i) out->history[i] = st.energies[i];
+    for (int i = 0; i < out->n_history; ++i) {
+      out->history[i] = st.energies[i];
+      out->hs_seconds[i] = i < (int)st.hs_seconds.size() ? st.hs_seconds[i] : 0.0;
+      out->hs_tol[i] = i < (int)st.hs_tol.size() ? st.hs_tol[i] : 0.0;
+      out->hs_matvecs[i] = i < (int)st.hs_matvecs.size() ? st.hs_matvecs[i] : 0;
+    }
     out->t_env = st.t_env;
     out->t_eig = st.t_eig;
     out->t_gauge = st.t_gauge;

@@ -102,11 +102,13 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   const bool dense = solver_ == fulleig;
   const bool lowp = precision_ != MPSD_PREC_FP64 && !dense;
   // Tolerance schedule (setEigTolSchedule, and always for the precision forms): the local solves
-  // of a half-sweep stop at max(eigtol, min(1e-3, 1e-2 |dE| / |E| of the half-sweep before)),
-  // 1e-3 for the first ones.  While the environments are far from converged a loose local solve
-  // loses nothing; the stopping rule |dE| < convtol |E| is only tested between two half-sweeps
-  // solved at eigtol, since loose solves can stall a sweep below convtol before the state has
-  // converged.
+  // of a half-sweep stop at max(eigtol, min(1e-3, 1e-2 |dE| / |E| of the half-sweep before,
+  // 1e-2 times that half-sweep's tolerance)), 1e-3 for the first ones.  While the environments
+  // are far from converged a loose local solve loses nothing.  Each half-sweep tightens at least
+  // 100-fold: one solved at the tolerance its sites already meet would change nothing, since
+  // every local problem stops within its first chunk of steps.  The stopping rule
+  // |dE| < convtol |E| is only tested between two half-sweeps solved at eigtol, since loose
+  // solves can stall a sweep below convtol before the state has converged.
   const bool sched = (schedule_ || lowp) && !dense;
   double cur_tol = sched ? std::max(tol, 1e-3) : tol;
   int tight = 0;  // consecutive half-sweeps solved at eigtol
@@ -131,6 +133,8 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     acc += std::chrono::duration<double>(now - last).count();
     last = now;
   };
+  auto hs_start = std::chrono::steady_clock::now();
+  int64_t hs_mv0 = 0;
   const unsigned mv_kinds = (1u << MPSD_KERNEL_STAGE1) | (1u << MPSD_KERNEL_MIX) | (1u << MPSD_KERNEL_HEFF_STAGE3);
   const int krylov_opt = mpsd_ctx_get_option(ctx_, MPSD_OPT_KRYLOV);
   const int krylov = krylov_opt > 0 ? krylov_opt : 30;
@@ -195,6 +199,12 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
     const bool endRight = right && pos == L - 1;
     const bool endLeft = !right && pos == 0;
     if (endRight || endLeft) {
+      const auto now = std::chrono::steady_clock::now();
+      stats_.hs_seconds.push_back(std::chrono::duration<double>(now - hs_start).count());
+      stats_.hs_matvecs.push_back(stats_.matvecs - hs_mv0);
+      stats_.hs_tol.push_back(cur_tol);
+      hs_start = now;
+      hs_mv0 = stats_.matvecs;
       const double dE = std::fabs(energy - val);
       tight = cur_tol <= tol ? tight + 1 : 0;
       done = dE < convtol_ * std::fabs(energy + offset) && (!sched || tight >= 2);
@@ -202,7 +212,7 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
       if (lowp && arith == MPSD_PREC_FP64 && stats_.refine_from < 0) stats_.refine_from = round;
       if (sched && !done) {
         const double rel = dE / std::max(std::fabs(val + offset), 1e-300);
-        cur_tol = std::max(tol, std::min(1e-3, 1e-2 * rel));
+        cur_tol = std::max(tol, std::min(std::min(1e-3, 1e-2 * rel), 1e-2 * cur_tol));
         if (dE < convtol_ * std::fabs(energy + offset)) cur_tol = tol;  // confirm at eigtol
       }
       energy = val;

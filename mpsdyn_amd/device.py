@@ -186,7 +186,8 @@ class GsResult(C.Structure):
                 ("refine_from", C.c_int), ("n_history", C.c_int),
                 ("history", C.c_double * 256), ("residual_norm", C.c_double), ("t_env", C.c_double),
                 ("t_eig", C.c_double), ("t_gauge", C.c_double), ("t_matvec", C.c_double),
-                ("matvec_flops", C.c_double), ("env_flops", C.c_double), ("residual_floor", C.c_double)]
+                ("matvec_flops", C.c_double), ("env_flops", C.c_double), ("residual_floor", C.c_double),
+                ("hs_seconds", C.c_double * 256), ("hs_tol", C.c_double * 256), ("hs_matvecs", C.c_int64 * 256)]
 
 
 class ServeConfig(C.Structure):

@@ -118,6 +118,29 @@ def test_tolerance_schedule_reaches_the_fixed_tolerance_energy(ctx, model, L, D,
         assert sched.matvecs < fixed.matvecs
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_schedule_tightens_every_half_sweep(ctx, prec):
+    """Per-half-sweep record of a scheduled solve (Schwinger L=40, D=48): each half-sweep's
+    local tolerance is at least 100 times tighter than the one before until it reaches eigtol
+    (1e-10), the last two run at eigtol, the matvec counts add up to the total and the seconds fit
+    in the wall time."""
+    from mpsdyn_amd import device as dev
+    r = dev.gs_solve(ctx, dev.gs_config(dev.MODEL_SCHWINGER, 40, 48, [2.5, 100.0, 0.0, 0.0], convtol=1e-12,
+                                        tol_schedule=True, precision=prec))
+    n = r.n_history
+    tols = [r.hs_tol[i] for i in range(n)]
+    mvs = [r.hs_matvecs[i] for i in range(n)]
+    print(f"prec={prec}: tols {tols} matvecs {mvs} seconds {[round(r.hs_seconds[i], 3) for i in range(n)]}",
+          flush=True)
+    assert n == r.halfsweeps and n >= 3
+    assert tols[0] == 1e-3
+    for a, b in zip(tols, tols[1:]):
+        assert b == 1e-10 or b <= 1e-2 * a * (1 + 1e-12)
+    assert tols[-1] == tols[-2] == 1e-10
+    assert sum(mvs) == r.matvecs and all(m > 0 for m in mvs)
+    assert 0 < sum(r.hs_seconds[i] for i in range(n)) <= r.seconds
+
+
 @pytest.mark.parametrize("L,D,g", [(180, 64, 1.5), (120, 256, 1.5)])
 def test_long_chain_reaches_the_closed_form(ctx, L, D, g):
     """Long chains (the served mix draws L up to 200): the seeded start state stays inside the
