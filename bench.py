@@ -238,7 +238,14 @@ def forms_table(dev, gaussian):
         tL, tW, tR = ctx.tensor(L), ctx.tensor(W), ctx.tensor(R)
         tx, ty = ctx.tensor(gaussian(4, (d, D, D))), ctx.tensor(dims=(d, D, D))
         F = sum(matvec_flops(D, D, D, D, w, w, d))
-        for prec, name in ((dev.PREC_FP64, "fp64"), (dev.PREC_FP32, "complex_fp32"), (dev.PREC_BF16X3, "split_bf16x3")):
+        # low-precision forms: packed operands with each tile (0 = automatic), and FP64 operands
+        # converted per tile (packed = 0)
+        variants = [(dev.PREC_FP64, "fp64", 1, 0)]
+        for prec, name in ((dev.PREC_FP32, "complex_fp32"), (dev.PREC_BF16X3, "split_bf16x3")):
+            variants += [(prec, name, 1, t) for t in range(5)] + [(prec, name, 0, 0)]
+        for prec, name, packed, tile in variants:
+            ctx.set_option(dev.OPT_LP_PACKED, packed)
+            ctx.set_option(dev.OPT_LP_TILE, tile)
             h = dev.Heff(ctx, tL, tW, tR, precision=prec)
             for _ in range(3):
                 h.apply_device(tx.devptr(), ty.devptr())
@@ -255,9 +262,11 @@ def forms_table(dev, gaussian):
                 ms, c = ctx.timing(k)
                 us[nm] = round(1e3 * ms / max(c, 1), 1)
             ctx.set_timing(False)
-            print(json.dumps({"D": D, "w": w, "form": name, "ms": round(t * 1e3, 4),
-                              "tflops": round(F / t / 1e12, 2)} | us), flush=True)
+            print(json.dumps({"D": D, "w": w, "form": name, "packed": packed, "tile": tile,
+                              "ms": round(t * 1e3, 4), "tflops": round(F / t / 1e12, 2)} | us), flush=True)
             h.close()
+        ctx.set_option(dev.OPT_LP_PACKED, 1)
+        ctx.set_option(dev.OPT_LP_TILE, 0)
     ctx.close()
 
 

@@ -88,6 +88,11 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
                                    operand is shared by the batches; results are the same for all */
 #define MPSD_OPT_KRYLOV 8       /* Krylov basis size of the ground-state sweep's local Lanczos solves:
                                    0 (default, 30) or 4..62 (more restarts, less Gram-Schmidt work) */
+#define MPSD_OPT_LP_PACKED 9    /* precision forms: 1 (default) L and R packed once per handle into the
+                                   form's arithmetic, x and the middle stage packed per matvec; 0 FP64
+                                   operands converted while each tile is staged */
+#define MPSD_OPT_LP_TILE 10     /* packed-operand BRGEMM tile: 0 automatic (default), 1 32x32, 2 64x32,
+                                   3 32x64, 4 64x64 */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
 int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option);
 

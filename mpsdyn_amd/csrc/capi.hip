@@ -3,6 +3,7 @@
 // any launch; every kernel below only ever sees operands whose sizes match its grid.
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -424,6 +425,8 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_FUSED_LANCZOS: return ctx->fused_lanczos;
     case MPSD_OPT_ENV_CHAIN: return ctx->env_chain;
     case MPSD_OPT_KRYLOV: return ctx->krylov;
+    case MPSD_OPT_LP_PACKED: return ctx->lp_packed;
+    case MPSD_OPT_LP_TILE: return ctx->lp_tile;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -457,6 +460,13 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
       if (value != 0 && (value < 4 || value > 62))
         return set_err(ctx, MPSD_ERR_ARG, "set_option: Krylov basis size must be 0 (default) or 4..62");
       ctx->krylov = value;
+      return MPSD_OK;
+    case MPSD_OPT_LP_PACKED:
+      ctx->lp_packed = value != 0;
+      return MPSD_OK;
+    case MPSD_OPT_LP_TILE:
+      if (value < 0 || value > 4) return set_err(ctx, MPSD_ERR_ARG, "set_option: packed tile must be 0..4");
+      ctx->lp_tile = value;
       return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");
@@ -861,6 +871,9 @@ int mpsd_env_right(mpsd_ctx* ctx, const mpsd_tensor* envR, const mpsd_tensor* ke
                  "env_right stage3");
 }
 
+// Stamps of Heff handles (mpsd_heff::lp_stamp): unique over the process.
+static std::atomic<uint64_t> g_lp_stamp{0};
+
 int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* W,
                      const mpsd_tensor* envR, double offset, int precision, mpsd_heff** out) {
   if (!ctx || !envL || !W || !envR || !out) return set_err(ctx, MPSD_ERR_ARG, "heff_create: null argument");
@@ -876,6 +889,7 @@ int mpsd_heff_create(mpsd_ctx* ctx, const mpsd_tensor* envL, const mpsd_tensor* 
   h->R = envR;
   h->offset = offset;
   h->precision = precision;
+  h->lp_stamp = ++g_lp_stamp;
   h->Db = dim(envL, 0);
   h->wl = dim(envL, 1);
   h->Dk = dim(envL, 2);
@@ -918,7 +932,123 @@ int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
   if (precision < MPSD_PREC_FP64 || precision > MPSD_PREC_BF16X3)
     return set_err(h->ctx, MPSD_ERR_ARG, "heff_set_precision: unknown precision");
   h->precision = precision;
+  h->lp_stamp = ++g_lp_stamp;  // the packed operands are taken afresh
   return MPSD_OK;
+}
+
+// Precision form on packed operands: L and R packed once per handle stamp (scratch slots 9,
+// 10), x packed per call (slot 11), T1 in FP32 (slot 0), T2 packed by the mix (slot 1), the
+// per-MPO-bond partial products in FP32 (slot 0 again, T1 being dead by then), summed into y
+// in FP64 with the offset term.
+static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offset) {
+  mpsd_ctx* ctx = h->ctx;
+  const int prec = h->precision;
+  const int d = h->d, Db = h->Db, wl = h->wl, Dk = h->Dk, Dbr = h->Dbr, wr = h->wr, Dkr = h->Dkr;
+  const int M = Db * wl;
+  const int Kp1 = lp_pad(Dk), Kp3 = lp_pad(Dkr);
+  const int64_t ny = (int64_t)d * Db * Dbr;
+  const size_t bL = 8 * (size_t)M * Kp1, bR = 8 * (size_t)wr * Dbr * Kp3, bX = 8 * (size_t)d * Dkr * Kp1;
+  const size_t bT1 = sizeof(float2) * (size_t)M * d * Dkr, bP = sizeof(float2) * (size_t)wr * ny;
+  const size_t bT2 = 8 * (size_t)d * wr * Db * Kp3;
+  if (ctx->scratch_bytes[9] < bL || ctx->scratch_bytes[10] < bR) ctx->lp_stamp = 0;  // slots reallocated
+  int rc = ensure_scratch(ctx, 0, std::max(bT1, bP));
+  if (!rc) rc = ensure_scratch(ctx, 1, bT2);
+  if (!rc) rc = ensure_scratch(ctx, 9, bL);
+  if (!rc) rc = ensure_scratch(ctx, 10, bR);
+  if (!rc) rc = ensure_scratch(ctx, 11, bX);
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  if (ctx->lp_stamp != h->lp_stamp || ctx->lp_prec != prec) {
+    // L as the first stage's A (rows b + Db al, k = a); R as the last stage's B per MPO bond
+    // (rows c, k = a_r)
+    LpPackDesc pl;
+    pl.src = h->L->ptr;
+    pl.dst = ctx->scratch[9];
+    pl.rows = M;
+    pl.K = Dk;
+    pl.Kp = Kp1;
+    pl.srow = 1;
+    pl.sk = M;
+    LpPackDesc pr;
+    pr.src = h->R->ptr;
+    pr.dst = ctx->scratch[10];
+    pr.nb = wr;
+    pr.rows = Dbr;
+    pr.K = Dkr;
+    pr.Kp = Kp3;
+    pr.sb = Dbr;
+    pr.srow = 1;
+    pr.sk = (int64_t)Dbr * wr;
+    e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {
+      hipError_t r = launch_lp_pack(pl, prec, ctx->stream);
+      return r == hipSuccess ? launch_lp_pack(pr, prec, ctx->stream) : r;
+    });
+    if (e != hipSuccess) return hip_err(ctx, e, "heff pack environments");
+    ctx->lp_stamp = h->lp_stamp;
+    ctx->lp_prec = prec;
+  }
+  // x[s + d a + d Dk a_r] as the first stage's B per s (rows a_r, k = a)
+  LpPackDesc px;
+  px.src = x;
+  px.dst = ctx->scratch[11];
+  px.nb = d;
+  px.rows = Dkr;
+  px.K = Dk;
+  px.Kp = Kp1;
+  px.sb = 1;
+  px.srow = (int64_t)d * Dk;
+  px.sk = d;
+  // T1[b + Db al + M s + M d a_r] = sum_a L[b, al, a] x[s, a, a_r]
+  LpDesc g1;
+  g1.A = ctx->scratch[9];
+  g1.B = ctx->scratch[11];
+  g1.C = (float2*)ctx->scratch[0];
+  g1.M = M;
+  g1.N = Dkr;
+  g1.Kp = Kp1;
+  g1.NB = d;
+  g1.zdiv = d;
+  g1.prec = prec;
+  g1.sB0 = (int64_t)Dkr * Kp1;
+  g1.sCm = 1;
+  g1.sCn = (int64_t)M * d;
+  g1.sC0 = M;
+  e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {
+    hipError_t r = launch_lp_pack(px, prec, ctx->stream);
+    return r == hipSuccess ? launch_lp_brgemm(g1, ctx->lp_tile, ctx->stream) : r;
+  });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff lp stage1");
+  // T2[o = s' + d be][b][a_r] = sum_{al, s} W[s', al, s, be] T1[b, al, s, a_r], packed
+  e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+    return launch_lp_mix((const float2*)ctx->scratch[0], ctx->scratch[1], h->mix, Db, wl * d, d * wr, Dkr, Kp3, prec,
+                         ctx->stream);
+  });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff lp mix");
+  // P[be][s' + d b + d Db c] = sum_{a_r} T2[s' + d be][b][a_r] R[c, be, a_r]
+  LpDesc g3;
+  g3.A = ctx->scratch[1];
+  g3.B = ctx->scratch[10];
+  g3.C = (float2*)ctx->scratch[0];
+  g3.M = Db;
+  g3.N = Dbr;
+  g3.Kp = Kp3;
+  g3.NB = d * wr;
+  g3.zdiv = d;
+  g3.prec = prec;
+  g3.sA0 = (int64_t)Db * Kp3;
+  g3.sA1 = (int64_t)d * Db * Kp3;
+  g3.sB1 = (int64_t)Dbr * Kp3;
+  g3.sCm = d;
+  g3.sCn = (int64_t)d * Db;
+  g3.sC0 = 1;
+  g3.sC1 = ny;
+  e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3, [&] { return launch_lp_brgemm(g3, ctx->lp_tile, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff lp stage3");
+  e = timed(ctx, MPSD_KERNEL_MIX, [&] {
+    return launch_lp_sum((const float2*)ctx->scratch[0], wr, ny, offset != 0.0 ? x : nullptr, offset, y, ny,
+                         ctx->stream);
+  });
+  return hip_err(ctx, e, "heff lp sum");
 }
 
 // One device matvec y = Heff x (+ offset x) on device pointers.  With s0, s1 set, T1 and the
@@ -932,6 +1062,7 @@ static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double of
   int rc = MPSD_OK;
   if (!fixed) rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
   if (rc) return rc;
+  if (!fixed && h->precision != MPSD_PREC_FP64 && ctx->lp_packed) return heff_apply_lp(h, x, y, offset);
   const int arith = arith_of(h->precision);
   rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr, arith, nullptr, s0, s1);
   if (rc) return rc;
@@ -1311,8 +1442,10 @@ int mpsd_heff_apply_grouped(mpsd_ctx* ctx, int n, mpsd_heff* const* hs, const do
     if (!h || !x[i] || !y[i]) return set_err(ctx, MPSD_ERR_ARG, "apply_grouped: null handle or vector");
     int rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);
     if (rc) return rc;
-    if (h->precision == MPSD_PREC_BF16X3 || h->d != 2 || h->herm) {
-      // split-BF16, d != 2 and the Hermitian form take the single-problem path on this stream
+    if (h->precision == MPSD_PREC_BF16X3 || (h->precision != MPSD_PREC_FP64 && ctx->lp_packed) || h->d != 2 ||
+        h->herm) {
+      // the precision forms, d != 2 and the Hermitian form take the single-problem path on this
+      // stream (grouped FP32 launches only on FP64 operands converted per tile)
       mpsd_ctx* own = h->ctx;
       h->ctx = ctx;
       rc = heff_apply_dev(h, (const double2*)x[i], (double2*)y[i]);

@@ -125,6 +125,40 @@ hipError_t launch_zchain_f64(const Chain& c, hipStream_t s);
 hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, int Bx, int nin,
                            int nout, int64_t Ax, hipStream_t s);
 
+// Precision forms with packed low-precision operands (zgemm_lp.hip).  A packed operand holds,
+// per batch, `rows` rows of Kp = K rounded up to 32 complex values (zeros past K), 8 bytes
+// each (FP32 re/im or split-BF16 hi/lo planes, grouped 8 k at a time); offsets and strides
+// below are in those 8-byte elements.
+//   C[z](m, n) = sum_k A[z](m, k) B[z](n, k)  (FP32 out), with batch z split as
+//   (z % zdiv, z / zdiv) for every operand's two batch strides.
+struct LpDesc {
+  const void* A = nullptr;
+  const void* B = nullptr;
+  float2* C = nullptr;
+  int M = 0, N = 0, Kp = 0, NB = 1, zdiv = 1;
+  int prec = MPSD_PREC_FP32;
+  int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0;  // batch strides (packed elements)
+  int64_t sCm = 0, sCn = 0, sC0 = 0, sC1 = 0;  // output strides (float2 elements)
+};
+// Pack src(b, row, k) = src[b*sb + row*srow + k*sk] (k < K, FP64) into dst (nb x rows x Kp).
+struct LpPackDesc {
+  const double2* src = nullptr;
+  void* dst = nullptr;
+  int nb = 1, rows = 0, K = 0, Kp = 0;
+  int64_t sb = 0, srow = 0, sk = 0;
+};
+inline int lp_pad(int k) { return (k + 31) / 32 * 32; }
+// tile: 1 = 32 x 32, 2 = 64 x 32, 3 = 32 x 64, 4 = 64 x 64, 0 = lp_auto_tile
+int lp_auto_tile(const LpDesc& p);
+hipError_t launch_lp_brgemm(const LpDesc& p, int tile, hipStream_t s);
+hipError_t launch_lp_pack(const LpPackDesc& p, int prec, hipStream_t s);
+// T2p[o][b][a_r packed] = sum_i mix[o + nout i] T1[b + Db i + Db nin a_r] (T1 in FP32)
+hipError_t launch_lp_mix(const float2* T1, void* T2p, const double2* mix, int Db, int nin, int nout, int Dkr,
+                         int Kp, int prec, hipStream_t s);
+// y[i] = sum_{q < np} P[q stride + i] (FP32 partials, FP64 sum) + beta x[i] (x may be null)
+hipError_t launch_lp_sum(const float2* P, int np, int64_t stride, const double2* x, double beta, double2* y,
+                         int64_t n, hipStream_t s);
+
 // Dense Heff (rows s',b_l,b_r ; cols s,a_l,a_r), column-major, plus offset on the diagonal.
 hipError_t launch_heff_full_f64(const double2* L, const double2* W, const double2* R, double2* H,
                                 int d, int Db, int wl, int Dk, int Dbr, int wr, int Dkr,
@@ -139,10 +173,17 @@ struct mpsd_ctx {
   // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
   // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step;
   // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec; 6 = Householder QR workspace
-  // (V, Z, Y, W); 7 = second gauge buffer; 8 = Krylov basis of the one-workgroup eigensolver.
-  static constexpr int kScratch = 9;
-  void* scratch[kScratch] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t scratch_bytes[kScratch] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // (V, Z, Y, W); 7 = second gauge buffer; 8 = Krylov basis of the one-workgroup eigensolver;
+  // 9, 10, 11 = packed low-precision L, R and x of a precision-form matvec.
+  static constexpr int kScratch = 12;
+  void* scratch[kScratch] = {};
+  size_t scratch_bytes[kScratch] = {};
+  // Packed low-precision environments in slots 9 and 10: the stamp of the Heff handle (and
+  // its precision) they were packed for, 0 when none.
+  uint64_t lp_stamp = 0;
+  int lp_prec = 0;
+  int lp_packed = 1;  // precision forms on packed operands (1) or on FP64 operands converted per tile (0)
+  int lp_tile = 0;    // packed BRGEMM tile: 0 automatic, 1 32x32, 2 64x32, 3 32x64, 4 64x64
   // Problem tables of grouped launches: device buffer + pinned host staging, reused after
   // the event of the previous upload has completed.
   void* gtab = nullptr;
@@ -233,6 +274,9 @@ struct mpsd_heff {
   const mpsd_tensor* R = nullptr;
   double offset = 0.0;
   int precision = MPSD_PREC_FP64;
+  // Identifies the operands as this handle saw them when it was created or its precision was
+  // last set: the packed low-precision copies of L and R are taken once per stamp.
+  uint64_t lp_stamp = 0;
   int d = 0, Db = 0, wl = 0, Dk = 0, Dbr = 0, wr = 0, Dkr = 0;
   double2* mix = nullptr;  // device (d*wr) x (wl*d) mix matrix from W (owned by W's cache)
   // Hermitian form (mpsd_heff_set_hermitian): y = (H x + H' x) / 2 + offset x, where H' is the

@@ -31,6 +31,8 @@ OPT_GAUGE_QR = 5
 OPT_FUSED_LANCZOS = 6
 OPT_ENV_CHAIN = 7
 OPT_KRYLOV = 8
+OPT_LP_PACKED = 9
+OPT_LP_TILE = 10
 
 _lib = None
 

@@ -110,3 +110,66 @@ def test_schwinger_L100_precision_forms():
         assert abs(r.energy - e64) <= 1e-10 * abs(e64)
         assert abs(r.residual_norm - res64) <= 0.5 * res64
         assert r.matvecs_lowp > 0
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-6), (2, 1e-4)])
+@pytest.mark.parametrize("Db,Dbr,wl,wr", [(20, 20, 3, 3), (37, 50, 5, 5), (1, 4, 1, 3), (3, 1, 3, 1),
+                                          (100, 70, 3, 5), (256, 256, 3, 3)])
+def test_packed_tiles_bit_identical(ctx, prec, tol, Db, Dbr, wl, wr):
+    """Packed-operand precision forms: every BRGEMM tile (32x32, 64x32, 32x64, 64x64) sums each
+    output in the same k order, so all give the same bits; they match the oracle to the form's
+    tolerance, as does the form on FP64 operands converted per tile."""
+    from mpsdyn_amd import device as dev
+    print(f"prec={prec} Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
+    L = orc.complex_gaussian(5, (Db, wl, Db))
+    W = orc.complex_gaussian(6, (2, wl, 2, wr))
+    R = orc.complex_gaussian(7, (Dbr, wr, Dbr))
+    x = orc.complex_gaussian(8, (2 * Db * Dbr,))
+    ref = orc.heff_product(L, W, R, x, offset=-0.7)
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R), offset=-0.7, precision=prec)
+    outs = []
+    try:
+        for tile in (0, 1, 2, 3, 4):
+            ctx.set_option(dev.OPT_LP_TILE, tile)
+            outs.append(h.apply(x))
+        ctx.set_option(dev.OPT_LP_TILE, 0)
+        ctx.set_option(dev.OPT_LP_PACKED, 0)
+        staged = h.apply(x)
+    finally:
+        ctx.set_option(dev.OPT_LP_TILE, 0)
+        ctx.set_option(dev.OPT_LP_PACKED, 1)
+    for k, y in enumerate(outs[1:]):
+        assert np.array_equal(y, outs[0]), f"tile option {k + 1} differs"
+    assert _rel(outs[0], ref) <= tol
+    assert _rel(staged, ref) <= tol
+    h.close()
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_packed_operands_per_handle(ctx, prec):
+    """Two handles on one context, applied alternately, each use their own packed environments;
+    set_precision takes a fresh copy after the environment tensor was rewritten."""
+    from mpsdyn_amd import device as dev
+    tol = 1e-6 if prec == 1 else 1e-4
+    hs, data = [], []
+    for k, (D, w) in enumerate([(48, 3), (33, 5)]):
+        L = orc.complex_gaussian(50 + k, (D, w, D))
+        W = orc.complex_gaussian(60 + k, (2, w, 2, w))
+        R = orc.complex_gaussian(70 + k, (D, w, D))
+        tL = ctx.tensor(L)
+        keep = (tL, ctx.tensor(W), ctx.tensor(R))
+        hs.append(dev.Heff(ctx, *keep, offset=0.1, precision=prec))
+        data.append([L, W, R, keep])
+    for it in range(3):
+        for k, h in enumerate(hs):
+            L, W, R, _ = data[k]
+            x = orc.complex_gaussian(80 + 10 * it + k, (h.size,))
+            assert _rel(h.apply(x), orc.heff_product(L, W, R, x, offset=0.1)) <= tol
+    L, W, R, keep = data[0]
+    L2 = orc.complex_gaussian(99, L.shape)
+    keep[0].upload(L2)
+    hs[0].set_precision(prec)
+    x = orc.complex_gaussian(98, (hs[0].size,))
+    assert _rel(hs[0].apply(x), orc.heff_product(L2, W, R, x, offset=0.1)) <= tol
+    for h in hs:
+        h.close()
