@@ -240,12 +240,14 @@ def forms_table(dev, gaussian):
         F = sum(matvec_flops(D, D, D, D, w, w, d))
         # low-precision forms: packed operands with each tile (0 = automatic), and FP64 operands
         # converted per tile (packed = 0)
-        variants = [(dev.PREC_FP64, "fp64", 1, 0)]
-        for prec, name in ((dev.PREC_FP32, "complex_fp32"), (dev.PREC_BF16X3, "split_bf16x3")):
+        variants = []
+        for prec, name in ((dev.PREC_FP64, "fp64"), (dev.PREC_FP32, "complex_fp32"), (dev.PREC_BF16X3, "split_bf16x3")):
             variants += [(prec, name, 1, t) for t in range(5)] + [(prec, name, 0, 0)]
         for prec, name, packed, tile in variants:
             ctx.set_option(dev.OPT_LP_PACKED, packed)
             ctx.set_option(dev.OPT_LP_TILE, tile)
+            ctx.set_option(dev.OPT_PACKED_FP64, packed)
+            ctx.set_option(dev.OPT_PACKED_FP64_TILE, tile)
             h = dev.Heff(ctx, tL, tW, tR, precision=prec)
             for _ in range(3):
                 h.apply_device(tx.devptr(), ty.devptr())
@@ -258,7 +260,7 @@ def forms_table(dev, gaussian):
             ctx.sync()
             t = (time.perf_counter() - t0) / n
             us = {}
-            for k, nm in ((0, "stage1_us"), (1, "mix_sum_us"), (2, "stage3_us")):
+            for k, nm in ((0, "stage1_us"), (1, "mix_sum_us"), (2, "stage3_us"), (4, "pack_us")):
                 ms, c = ctx.timing(k)
                 us[nm] = round(1e3 * ms / max(c, 1), 1)
             ctx.set_timing(False)
@@ -267,6 +269,8 @@ def forms_table(dev, gaussian):
             h.close()
         ctx.set_option(dev.OPT_LP_PACKED, 1)
         ctx.set_option(dev.OPT_LP_TILE, 0)
+        ctx.set_option(dev.OPT_PACKED_FP64, 1)
+        ctx.set_option(dev.OPT_PACKED_FP64_TILE, 0)
     ctx.close()
 
 
@@ -459,7 +463,7 @@ def main():
 
     # per-kind breakdown on a short pass before the timed region (every launch bracketed by
     # events); it also names the dominant kernel, the GEMM stage with the larger time
-    names = {0: "stage1_env_x_site", 1: "mix_mpo", 2: "stage3_heff"}
+    names = {0: "stage1_env_x_site", 1: "mix_mpo", 2: "stage3_heff", 4: "pack_operands"}
     ctx.set_timing(True)
     for _ in range(min(args.steps, 50)):
         h.apply_device(xp, yp)

@@ -65,7 +65,8 @@ int64_t mpsd_ctx_device_bytes(const mpsd_ctx* ctx);
 #define MPSD_KERNEL_MIX 1         /* MPO contraction of the middle stage */
 #define MPSD_KERNEL_HEFF_STAGE3 2 /* last stage of the matvec */
 #define MPSD_KERNEL_ENV_STAGE3 3  /* last stage of an environment update */
-#define MPSD_KERNEL_KINDS 4
+#define MPSD_KERNEL_PACK 4        /* packing of matvec operands (environments once per handle, x per call) */
+#define MPSD_KERNEL_KINDS 5
 int mpsd_ctx_set_timing(mpsd_ctx* ctx, int on);
 int mpsd_ctx_timing(mpsd_ctx* ctx, int kind, double* total_ms, int64_t* launches);
 /* Restrict timing to the kinds whose bit (1 << MPSD_KERNEL_*) is set in mask (default: all).
@@ -93,6 +94,10 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
                                    operands converted while each tile is staged */
 #define MPSD_OPT_LP_TILE 10     /* packed-operand BRGEMM tile: 0 automatic (default), 1 32x32, 2 64x32,
                                    3 32x64, 4 64x64 */
+#define MPSD_OPT_PACKED_FP64 11 /* FP64 matvec: 1 (default) L and R packed once per handle, x per call,
+                                   where the last stage is split over the MPO bond (same bits as 0);
+                                   0 operands read in place */
+#define MPSD_OPT_PACKED_FP64_TILE 12 /* its BRGEMM tile, as MPSD_OPT_LP_TILE */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
 int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option);
 

@@ -427,6 +427,8 @@ int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option) {
     case MPSD_OPT_KRYLOV: return ctx->krylov;
     case MPSD_OPT_LP_PACKED: return ctx->lp_packed;
     case MPSD_OPT_LP_TILE: return ctx->lp_tile;
+    case MPSD_OPT_PACKED_FP64: return ctx->pk64;
+    case MPSD_OPT_PACKED_FP64_TILE: return ctx->pk64_tile;
     default: return MPSD_ERR_ARG;
   }
 }
@@ -467,6 +469,13 @@ int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value) {
     case MPSD_OPT_LP_TILE:
       if (value < 0 || value > 4) return set_err(ctx, MPSD_ERR_ARG, "set_option: packed tile must be 0..4");
       ctx->lp_tile = value;
+      return MPSD_OK;
+    case MPSD_OPT_PACKED_FP64:
+      ctx->pk64 = value != 0;
+      return MPSD_OK;
+    case MPSD_OPT_PACKED_FP64_TILE:
+      if (value < 0 || value > 4) return set_err(ctx, MPSD_ERR_ARG, "set_option: packed tile must be 0..4");
+      ctx->pk64_tile = value;
       return MPSD_OK;
     case MPSD_OPT_BATCH_GROUP:
       if (value < 0 || value > 2) return set_err(ctx, MPSD_ERR_ARG, "set_option: batch group must be 0, 1 or 2");
@@ -936,20 +945,26 @@ int mpsd_heff_set_precision(mpsd_heff* h, int precision) {
   return MPSD_OK;
 }
 
-// Precision form on packed operands: L and R packed once per handle stamp (scratch slots 9,
-// 10), x packed per call (slot 11), T1 in FP32 (slot 0), T2 packed by the mix (slot 1), the
-// per-MPO-bond partial products in FP32 (slot 0 again, T1 being dead by then), summed into y
-// in FP64 with the offset term.
-static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offset) {
+// Matvec on packed operands: L and R packed once per handle stamp (scratch slots 9, 10), x
+// packed per call (slot 11), T1 (slot 0), T2 packed row-interleaved by the mix (slot 1), the per-MPO-bond partial
+// products (slot 0 again, T1 being dead by then), summed into y with the offset term.
+//  - FP64 (taken only where the plain path splits its last stage, so the sums are the same):
+//    packed FP64 rows in the k grouping of zbrgemm_f64's MFMAs, T1 and partials in FP64, the mix
+//    and the part sum with zmix_f64's and zsum_parts_f64's arithmetic: bit-identical to the
+//    plain path.
+//  - FP32 / split-BF16: packed 8-byte values, T1 and partials in FP32, summed in FP64.
+static int heff_apply_packed(mpsd_heff* h, const double2* x, double2* y, double offset) {
   mpsd_ctx* ctx = h->ctx;
   const int prec = h->precision;
+  const bool f64 = prec == MPSD_PREC_FP64;
   const int d = h->d, Db = h->Db, wl = h->wl, Dk = h->Dk, Dbr = h->Dbr, wr = h->wr, Dkr = h->Dkr;
   const int M = Db * wl;
-  const int Kp1 = lp_pad(Dk), Kp3 = lp_pad(Dkr);
+  const int Kp1 = lp_pad(Dk, prec), Kp3 = lp_pad(Dkr, prec);
   const int64_t ny = (int64_t)d * Db * Dbr;
-  const size_t bL = 8 * (size_t)M * Kp1, bR = 8 * (size_t)wr * Dbr * Kp3, bX = 8 * (size_t)d * Dkr * Kp1;
-  const size_t bT1 = sizeof(float2) * (size_t)M * d * Dkr, bP = sizeof(float2) * (size_t)wr * ny;
-  const size_t bT2 = 8 * (size_t)d * wr * Db * Kp3;
+  const size_t es = f64 ? 16 : 8;  // bytes per packed value, and per T1 / partial value
+  const size_t bL = es * M * Kp1, bR = es * wr * Dbr * Kp3, bX = es * d * Dkr * Kp1;
+  const size_t bT1 = es * M * d * Dkr, bP = es * wr * ny;
+  const size_t bT2 = es * d * wr * Db * Kp3;
   if (ctx->scratch_bytes[9] < bL || ctx->scratch_bytes[10] < bR) ctx->lp_stamp = 0;  // slots reallocated
   int rc = ensure_scratch(ctx, 0, std::max(bT1, bP));
   if (!rc) rc = ensure_scratch(ctx, 1, bT2);
@@ -979,7 +994,7 @@ static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offs
     pr.sb = Dbr;
     pr.srow = 1;
     pr.sk = (int64_t)Dbr * wr;
-    e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {
+    e = timed(ctx, MPSD_KERNEL_PACK, [&] {
       hipError_t r = launch_lp_pack(pl, prec, ctx->stream);
       return r == hipSuccess ? launch_lp_pack(pr, prec, ctx->stream) : r;
     });
@@ -1003,6 +1018,7 @@ static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offs
   g1.A = ctx->scratch[9];
   g1.B = ctx->scratch[11];
   g1.C = (float2*)ctx->scratch[0];
+  g1.Cd = (double2*)ctx->scratch[0];
   g1.M = M;
   g1.N = Dkr;
   g1.Kp = Kp1;
@@ -1013,22 +1029,25 @@ static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offs
   g1.sCm = 1;
   g1.sCn = (int64_t)M * d;
   g1.sC0 = M;
-  e = timed(ctx, MPSD_KERNEL_STAGE1, [&] {
-    hipError_t r = launch_lp_pack(px, prec, ctx->stream);
-    return r == hipSuccess ? launch_lp_brgemm(g1, ctx->lp_tile, ctx->stream) : r;
-  });
-  if (e != hipSuccess) return hip_err(ctx, e, "heff lp stage1");
+  const int tile = f64 ? ctx->pk64_tile : ctx->lp_tile;
+  e = timed(ctx, MPSD_KERNEL_PACK, [&] { return launch_lp_pack(px, prec, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff pack x");
+  e = timed(ctx, MPSD_KERNEL_STAGE1, [&] { return launch_lp_brgemm(g1, tile, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff packed stage1");
   // T2[o = s' + d be][b][a_r] = sum_{al, s} W[s', al, s, be] T1[b, al, s, a_r], packed
   e = timed(ctx, MPSD_KERNEL_MIX, [&] {
-    return launch_lp_mix((const float2*)ctx->scratch[0], ctx->scratch[1], h->mix, Db, wl * d, d * wr, Dkr, Kp3, prec,
-                         ctx->stream);
+    return f64 ? launch_pk64_mix((const double2*)ctx->scratch[0], (double2*)ctx->scratch[1], h->mix, Db, wl * d,
+                                 d * wr, Dkr, Kp3, ctx->stream)
+               : launch_lp_mix((const float2*)ctx->scratch[0], ctx->scratch[1], h->mix, Db, wl * d, d * wr, Dkr, Kp3,
+                               prec, ctx->stream);
   });
-  if (e != hipSuccess) return hip_err(ctx, e, "heff lp mix");
+  if (e != hipSuccess) return hip_err(ctx, e, "heff packed mix");
   // P[be][s' + d b + d Db c] = sum_{a_r} T2[s' + d be][b][a_r] R[c, be, a_r]
   LpDesc g3;
   g3.A = ctx->scratch[1];
   g3.B = ctx->scratch[10];
   g3.C = (float2*)ctx->scratch[0];
+  g3.Cd = (double2*)ctx->scratch[0];
   g3.M = Db;
   g3.N = Dbr;
   g3.Kp = Kp3;
@@ -1042,13 +1061,16 @@ static int heff_apply_lp(mpsd_heff* h, const double2* x, double2* y, double offs
   g3.sCn = (int64_t)d * Db;
   g3.sC0 = 1;
   g3.sC1 = ny;
-  e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3, [&] { return launch_lp_brgemm(g3, ctx->lp_tile, ctx->stream); });
-  if (e != hipSuccess) return hip_err(ctx, e, "heff lp stage3");
+  g3.a_il = 1;  // as the mix writes it
+  e = timed(ctx, MPSD_KERNEL_HEFF_STAGE3, [&] { return launch_lp_brgemm(g3, tile, ctx->stream); });
+  if (e != hipSuccess) return hip_err(ctx, e, "heff packed stage3");
   e = timed(ctx, MPSD_KERNEL_MIX, [&] {
-    return launch_lp_sum((const float2*)ctx->scratch[0], wr, ny, offset != 0.0 ? x : nullptr, offset, y, ny,
-                         ctx->stream);
+    return f64 ? launch_zsum_parts_f64((const double2*)ctx->scratch[0], wr, ny, offset != 0.0 ? x : nullptr, offset,
+                                       y, ny, ctx->stream)
+               : launch_lp_sum((const float2*)ctx->scratch[0], wr, ny, offset != 0.0 ? x : nullptr, offset, y, ny,
+                               ctx->stream);
   });
-  return hip_err(ctx, e, "heff lp sum");
+  return hip_err(ctx, e, "heff packed sum");
 }
 
 // One device matvec y = Heff x (+ offset x) on device pointers.  With s0, s1 set, T1 and the
@@ -1062,7 +1084,9 @@ static int heff_apply_core(mpsd_heff* h, const double2* x, double2* y, double of
   int rc = MPSD_OK;
   if (!fixed) rc = get_mix(ctx, h->W, 0, h->d, h->wl, h->wr, &h->mix);  // pointer check when cached
   if (rc) return rc;
-  if (!fixed && h->precision != MPSD_PREC_FP64 && ctx->lp_packed) return heff_apply_lp(h, x, y, offset);
+  if (!fixed && h->precision != MPSD_PREC_FP64 && ctx->lp_packed) return heff_apply_packed(h, x, y, offset);
+  if (!fixed && h->precision == MPSD_PREC_FP64 && ctx->pk64 && stage3_split(h, ctx))
+    return heff_apply_packed(h, x, y, offset);
   const int arith = arith_of(h->precision);
   rc = left_stages(ctx, h->L->ptr, h->Db, h->wl, h->Dk, x, h->d, h->Dkr, h->mix, h->wr, arith, nullptr, s0, s1);
   if (rc) return rc;

@@ -135,7 +135,8 @@ void Contractor::findGroundState(const MPO& ops, int D, double* lambda, MPS& ini
   };
   auto hs_start = std::chrono::steady_clock::now();
   int64_t hs_mv0 = 0;
-  const unsigned mv_kinds = (1u << MPSD_KERNEL_STAGE1) | (1u << MPSD_KERNEL_MIX) | (1u << MPSD_KERNEL_HEFF_STAGE3);
+  const unsigned mv_kinds = (1u << MPSD_KERNEL_STAGE1) | (1u << MPSD_KERNEL_MIX) | (1u << MPSD_KERNEL_HEFF_STAGE3) |
+                            (1u << MPSD_KERNEL_PACK);
   const int krylov_opt = mpsd_ctx_get_option(ctx_, MPSD_OPT_KRYLOV);
   const int krylov = krylov_opt > 0 ? krylov_opt : 30;
   while (!done) {

@@ -134,9 +134,11 @@ hipError_t launch_zmix_f64(const double2* in, double2* out, const double2* mix, 
 struct LpDesc {
   const void* A = nullptr;
   const void* B = nullptr;
-  float2* C = nullptr;
+  float2* C = nullptr;    // FP32 and split-BF16 output
+  double2* Cd = nullptr;  // FP64 output
   int M = 0, N = 0, Kp = 0, NB = 1, zdiv = 1;
   int prec = MPSD_PREC_FP32;
+  int a_il = 0;  // A row-interleaved: 16-byte chunk c of group G (4 chunks) of row m at (G M + m) 4 + c
   int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0;  // batch strides (packed elements)
   int64_t sCm = 0, sCn = 0, sC0 = 0, sC1 = 0;  // output strides (float2 elements)
 };
@@ -147,12 +149,19 @@ struct LpPackDesc {
   int nb = 1, rows = 0, K = 0, Kp = 0;
   int64_t sb = 0, srow = 0, sk = 0;
 };
-inline int lp_pad(int k) { return (k + 31) / 32 * 32; }
+// k per packed stage: 16 for FP64 (16-byte values), 32 for the low-precision forms (8 bytes)
+inline int lp_stage(int prec) { return prec == MPSD_PREC_FP64 ? 16 : 32; }
+inline int lp_pad(int k, int prec) { return (k + lp_stage(prec) - 1) / lp_stage(prec) * lp_stage(prec); }
+// Packed FP64 (prec = MPSD_PREC_FP64): 16-byte values, within each 16-deep stage k = 16 t + 4 u + q
+// in slot 4 q + u, the k grouping of zbrgemm_f64's MFMAs, so its products are the same bits.
+hipError_t launch_pk64_mix(const double2* T1, double2* T2p, const double2* mix, int Db, int nin, int nout, int Dkr,
+                           int Kp, hipStream_t s);
 // tile: 1 = 32 x 32, 2 = 64 x 32, 3 = 32 x 64, 4 = 64 x 64, 0 = lp_auto_tile
 int lp_auto_tile(const LpDesc& p);
 hipError_t launch_lp_brgemm(const LpDesc& p, int tile, hipStream_t s);
 hipError_t launch_lp_pack(const LpPackDesc& p, int prec, hipStream_t s);
-// T2p[o][b][a_r packed] = sum_i mix[o + nout i] T1[b + Db i + Db nin a_r] (T1 in FP32)
+// T2p[o] (rows b, k = a_r, packed row-interleaved) = sum_i mix[o + nout i] T1[b + Db i + Db nin a_r]
+// (T1 in FP32)
 hipError_t launch_lp_mix(const float2* T1, void* T2p, const double2* mix, int Db, int nin, int nout, int Dkr,
                          int Kp, int prec, hipStream_t s);
 // y[i] = sum_{q < np} P[q stride + i] (FP32 partials, FP64 sum) + beta x[i] (x may be null)
@@ -184,6 +193,8 @@ struct mpsd_ctx {
   int lp_prec = 0;
   int lp_packed = 1;  // precision forms on packed operands (1) or on FP64 operands converted per tile (0)
   int lp_tile = 0;    // packed BRGEMM tile: 0 automatic, 1 32x32, 2 64x32, 3 32x64, 4 64x64
+  int pk64 = 1;       // FP64 matvec on packed operands where its last stage is split (same bits)
+  int pk64_tile = 0;  // its tile, as lp_tile
   // Problem tables of grouped launches: device buffer + pinned host staging, reused after
   // the event of the previous upload has completed.
   void* gtab = nullptr;

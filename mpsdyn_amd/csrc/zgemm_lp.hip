@@ -33,6 +33,7 @@
 namespace mpsd {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned u4v __attribute__((ext_vector_type(4)));
 
@@ -83,11 +84,12 @@ template <int PREC>
 __global__ void __launch_bounds__(kT) lp_pack_kernel(const LpPackDesc p) {
   const int64_t ng = p.Kp / 8;
   const int64_t total = (int64_t)p.nb * p.rows * ng;
+  const bool row_fast = p.srow == 1;
   for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
-    const int row = (int)(t % p.rows);
-    const int64_t rest = t / p.rows;
-    const int g = (int)(rest % ng);
-    const int b = (int)(rest / ng);
+    const int b = (int)(t / ((int64_t)p.rows * ng));
+    const int64_t r = t - (int64_t)b * p.rows * ng;
+    const int row = row_fast ? (int)(r % p.rows) : (int)(r / ng);
+    const int g = row_fast ? (int)(r / p.rows) : (int)(r % ng);
     const double2* src = p.src + (int64_t)b * p.sb + (int64_t)row * p.srow;
     float re[8], im[8];
 #pragma unroll
@@ -149,7 +151,8 @@ __global__ void __launch_bounds__(kT) lp_mix_kernel(const float2* __restrict__ T
       if (o == 1 && !two) break;
       uint4 out[4];
       pack8<PREC>(re[o], im[o], out);
-      uint4* dst = reinterpret_cast<uint4*>(T2p) + (((int64_t)(o0 + o) * Db + b) * Kp + 8 * g) / 2;
+      // row-interleaved: group g of row b at (g Db + b) 4 (16-byte units) within output o
+      uint4* dst = reinterpret_cast<uint4*>(T2p) + (int64_t)(o0 + o) * Db * Kp / 2 + ((int64_t)g * Db + b) * 4;
 #pragma unroll
       for (int u = 0; u < 4; ++u) dst[u] = out[u];
     }
@@ -219,6 +222,11 @@ __global__ void __launch_bounds__(kT) lp_brgemm_kernel(const LpDesc p, int gm, i
   const int64_t rowq = p.Kp / 2;  // uint4 per packed row
   const int c16 = tid & 15;       // 16-byte chunk of a row's stage
   const int r16 = tid >> 4;       // row within a group of 16
+  // A row-interleaved (a_il): the 4-chunk groups G of all rows in turn, at (G M + row) 4 + chunk;
+  // a wave loads one group of 16 rows, 1 KB contiguous
+  const int ar16 = p.a_il ? (tid >> 2) & 15 : r16;
+  const int ac16 = p.a_il ? 4 * (tid >> 6) + (tid & 3) : c16;
+  const int64_t astep = p.a_il ? 16 * (int64_t)p.M : 16;
 
   f4 acc[NACC][BM][BN];
 #pragma unroll
@@ -232,19 +240,22 @@ __global__ void __launch_bounds__(kT) lp_brgemm_kernel(const LpDesc p, int gm, i
   const u4v* arow[AL];
   const u4v* brow[BL];
 #pragma unroll
-  for (int e = 0; e < AL; ++e) arow[e] = Ab + (int64_t)min(m0 + r16 + 16 * e, p.M - 1) * rowq + c16;
+  for (int e = 0; e < AL; ++e) {
+    const int64_t row = min(m0 + ar16 + 16 * e, p.M - 1);
+    arow[e] = Ab + (p.a_il ? ((ac16 >> 2) * (int64_t)p.M + row) * 4 + (ac16 & 3) : row * rowq + ac16);
+  }
 #pragma unroll
   for (int e = 0; e < BL; ++e) brow[e] = Bb + (int64_t)min(n0 + r16 + 16 * e, p.N - 1) * rowq + c16;
   u4v ra[AL], rb[BL];
   auto load = [&](int t) {
 #pragma unroll
-    for (int e = 0; e < AL; ++e) ra[e] = arow[e][16 * t];
+    for (int e = 0; e < AL; ++e) ra[e] = arow[e][astep * t];
 #pragma unroll
     for (int e = 0; e < BL; ++e) rb[e] = brow[e][16 * t];
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < AL; ++e) As[buf][r16 + 16 * e][c16] = ra[e];
+    for (int e = 0; e < AL; ++e) As[buf][ar16 + 16 * e][ac16] = ra[e];
 #pragma unroll
     for (int e = 0; e < BL; ++e) Bs[buf][r16 + 16 * e][c16] = rb[e];
   };
@@ -353,6 +364,197 @@ __global__ void __launch_bounds__(kT) lp_brgemm_kernel(const LpDesc p, int gm, i
     }
 }
 
+// ---------------------------------------------------------------- FP64 on packed operands
+//
+// The FP64 matvec on the same packed rows, 16 bytes per complex value and 16 k per stage:
+// within each stage the value k = 16 t + 4 u + q sits in slot 4 q + u, so lane group q reads
+// the four values it feeds to MFMA steps u = 0 .. 3 as four contiguous 16-byte reads.  Step u
+// of every 16 x 16 block is v_mfma_f64_16x16x4f64 over k = 16 t + 4 u .. 16 t + 4 u + 3 with
+// lane group q on k = 16 t + 4 u + q, in the three-product form, in ascending k: exactly the
+// MFMA sequence of zbrgemm_f64 and of the one-workgroup chain, so the results are the same bits.
+
+// FP64 strided source -> packed FP64 (permuted within each 16-deep stage).  One thread per
+// (batch, row, stage, lane group); rows fastest when they are contiguous in the source (the
+// environments), the (stage, lane group) fastest otherwise (x, whose k is the fast index).
+__global__ void __launch_bounds__(kT) pk64_pack_kernel(const LpPackDesc p) {
+  const int64_t nq = p.Kp / 4;  // (stage, lane group) pairs per row
+  const int64_t total = (int64_t)p.nb * p.rows * nq;
+  const bool row_fast = p.srow == 1;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int b = (int)(t / ((int64_t)p.rows * nq));
+    const int64_t r = t - (int64_t)b * p.rows * nq;
+    const int row = row_fast ? (int)(r % p.rows) : (int)(r / nq);
+    const int sq = row_fast ? (int)(r / p.rows) : (int)(r % nq);
+    const int st = sq >> 2, q = sq & 3;
+    const double2* src = p.src + (int64_t)b * p.sb + (int64_t)row * p.srow;
+    double2* dst = reinterpret_cast<double2*>(p.dst) + ((int64_t)b * p.rows + row) * p.Kp + 16 * st + 4 * q;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = 16 * st + 4 * u + q;
+      dst[u] = k < p.K ? src[(int64_t)k * p.sk] : double2{0.0, 0.0};
+    }
+  }
+}
+
+// T2 = MPO mix of T1 (FP64, zmix_f64's arithmetic element by element), packed FP64 row-interleaved
+// as the last stage's A operand.  One thread per T2 element of every output o (as zmix_f64),
+// ordered (slot u, row b, stage group) so that a wave reads 16 consecutive b of four a_r and
+// writes 1 KB contiguous per output.
+constexpr int kMixMax = 16;
+__global__ void __launch_bounds__(kT) pk64_mix_kernel(const double2* __restrict__ T1, double2* __restrict__ T2p,
+                                                      const double2* __restrict__ mix, int Db, int nin, int nout,
+                                                      int Dkr, int Kp) {
+  const int64_t total = (int64_t)Db * Kp;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int u = (int)(t & 3);
+    const int64_t rest = t >> 2;
+    const int b = (int)(rest % Db);
+    const int G = (int)(rest / Db);  // group 4 st + q of the packed row
+    const int ar = 16 * (G >> 2) + 4 * u + (G & 3);
+    double2 acc[kMixMax];
+#pragma unroll
+    for (int o = 0; o < kMixMax; ++o) acc[o] = double2{0.0, 0.0};
+    if (ar < Dkr) {
+      for (int i = 0; i < nin; ++i) {
+        const double2 v = T1[b + (int64_t)Db * i + (int64_t)Db * nin * ar];
+#pragma unroll
+        for (int o = 0; o < kMixMax; ++o) {
+          if (o < nout) {
+            const double2 w = mix[o + nout * i];
+            acc[o].x = fma(w.x, v.x, fma(-w.y, v.y, acc[o].x));
+            acc[o].y = fma(w.x, v.y, fma(w.y, v.x, acc[o].y));
+          }
+        }
+      }
+    }
+    double2* dst = T2p + ((int64_t)G * Db + b) * 4 + u;
+#pragma unroll
+    for (int o = 0; o < kMixMax; ++o)
+      if (o < nout) dst[(int64_t)o * Db * Kp] = acc[o];
+  }
+}
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(kT) pk64_brgemm_kernel(const LpDesc p, int gm, int gn) {
+  constexpr int TM = 32 * BM, TN = 32 * BN;
+  constexpr int AL = TM * 16 / kT;
+  constexpr int BL = TN * 16 / kT;
+  __shared__ u4v As[2][TM][kPitch];
+  __shared__ u4v Bs[2][TN][kPitch];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  int tm_i, tn_i, z;
+  lp_tile_of(gm, gn, tm_i, tn_i, z);
+  const int m0 = tm_i * TM, n0 = tn_i * TN;
+  const int z0 = z % p.zdiv, z1 = z / p.zdiv;
+  const u4v* Ab = reinterpret_cast<const u4v*>(p.A) + (z0 * p.sA0 + z1 * p.sA1);
+  const u4v* Bb = reinterpret_cast<const u4v*>(p.B) + (z0 * p.sB0 + z1 * p.sB1);
+  const int64_t rowq = p.Kp;  // 16-byte values per packed row
+  const int c16 = tid & 15, r16 = tid >> 4;
+  const int ar16 = p.a_il ? (tid >> 2) & 15 : r16;
+  const int ac16 = p.a_il ? 4 * (tid >> 6) + (tid & 3) : c16;
+  const int64_t astep = p.a_il ? 16 * (int64_t)p.M : 16;
+
+  d4 acc[3][BM][BN];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int j = 0; j < BN; ++j) acc[c][i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const u4v* arow[AL];
+  const u4v* brow[BL];
+#pragma unroll
+  for (int e = 0; e < AL; ++e) {
+    const int64_t row = min(m0 + ar16 + 16 * e, p.M - 1);
+    arow[e] = Ab + (p.a_il ? ((ac16 >> 2) * (int64_t)p.M + row) * 4 + (ac16 & 3) : row * rowq + ac16);
+  }
+#pragma unroll
+  for (int e = 0; e < BL; ++e) brow[e] = Bb + (int64_t)min(n0 + r16 + 16 * e, p.N - 1) * rowq + c16;
+  u4v ra[AL], rb[BL];
+  auto load = [&](int t) {
+#pragma unroll
+    for (int e = 0; e < AL; ++e) ra[e] = arow[e][astep * t];
+#pragma unroll
+    for (int e = 0; e < BL; ++e) rb[e] = brow[e][16 * t];
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < AL; ++e) As[buf][ar16 + 16 * e][ac16] = ra[e];
+#pragma unroll
+    for (int e = 0; e < BL; ++e) Bs[buf][r16 + 16 * e][c16] = rb[e];
+  };
+
+  const int nst = p.Kp / 16;
+  const int lr = lane & 15, q = lane >> 4;
+  if (nst > 0) {
+    load(0);
+    store(0);
+    __syncthreads();
+  }
+  for (int t = 0; t < nst; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < nst) load(t + 1);
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double ar[BM], ai[BM], a3[BM], br[BN], bi[BN], b3[BN];
+#pragma unroll
+      for (int i = 0; i < BM; ++i) {
+        const double2 v = __builtin_bit_cast(double2, As[cur][wm * (TM / 2) + i * 16 + lr][4 * q + u]);
+        ar[i] = v.x;
+        ai[i] = v.y;
+        a3[i] = v.x + v.y;
+      }
+#pragma unroll
+      for (int j = 0; j < BN; ++j) {
+        const double2 v = __builtin_bit_cast(double2, Bs[cur][wn * (TN / 2) + j * 16 + lr][4 * q + u]);
+        br[j] = v.x;
+        bi[j] = v.y;
+        b3[j] = v.x + v.y;
+      }
+#pragma unroll
+      for (int i = 0; i < BM; ++i)
+#pragma unroll
+        for (int j = 0; j < BN; ++j) {
+          acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acc[0][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acc[1][i][j], 0, 0, 0);
+          acc[2][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3[i], b3[j], acc[2][i][j], 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_setprio(0);
+    if (t + 1 < nst) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  double2* Cb = p.Cd + z0 * p.sC0 + z1 * p.sC1;
+#pragma unroll
+  for (int i = 0; i < BM; ++i)
+#pragma unroll
+    for (int j = 0; j < BN; ++j) {
+      const int n = n0 + wn * (TN / 2) + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * (TM / 2) + i * 16 + q + 4 * r;
+        if (m < p.M && n < p.N) {
+          const double p1 = acc[0][i][j][r], p2 = acc[1][i][j][r], p3 = acc[2][i][j][r];
+          Cb[m * p.sCm + n * p.sCn] = double2{p1 - p2, p3 - (p1 + p2)};
+        }
+      }
+    }
+}
+
+template <int BM, int BN>
+hipError_t pk64_launch(const LpDesc& p, hipStream_t s) {
+  const int gm = (p.M + 32 * BM - 1) / (32 * BM), gn = (p.N + 32 * BN - 1) / (32 * BN);
+  const int64_t total = (int64_t)gm * gn * p.NB;
+  if (total > 0x7fffffff) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL((pk64_brgemm_kernel<BM, BN>), dim3((unsigned)total), dim3(kT), 0, s, p, gm, gn);
+  return hipGetLastError();
+}
+
 template <int PREC, int BM, int BN>
 hipError_t lp_launch(const LpDesc& p, hipStream_t s) {
   const int gm = (p.M + 32 * BM - 1) / (32 * BM), gn = (p.N + 32 * BN - 1) / (32 * BN);
@@ -377,10 +579,13 @@ unsigned grid_for(int64_t total) { return (unsigned)std::min<int64_t>(8192, std:
 }  // namespace
 
 int lp_auto_tile(const LpDesc& p) {
-  // the largest tile that still gives two workgroups per CU
+  // the largest tile that still gives two workgroups per CU.  FP64 stops at two blocks per wave
+  // (the 2 x 2 form holds two waves per SIMD, too few for the FP64 MFMA) and prefers 32 x 64
+  // (measured at D = 512: stage 1 87.8 us against 91.7 us for 64 x 32)
   auto wgs = [&](int bm, int bn) {
     return (int64_t)((p.M + 32 * bm - 1) / (32 * bm)) * ((p.N + 32 * bn - 1) / (32 * bn)) * p.NB;
   };
+  if (p.prec == MPSD_PREC_FP64) return wgs(1, 2) >= 512 ? 3 : wgs(2, 1) >= 512 ? 2 : 1;
   if (wgs(2, 2) >= 512) return 4;
   if (p.M >= p.N && wgs(2, 1) >= 512) return 2;
   if (wgs(1, 2) >= 512) return 3;
@@ -390,18 +595,28 @@ int lp_auto_tile(const LpDesc& p) {
 
 hipError_t launch_lp_brgemm(const LpDesc& p, int tile, hipStream_t s) {
   if (p.M <= 0 || p.N <= 0 || p.NB <= 0) return hipSuccess;
-  if (p.Kp % 32 != 0 || p.zdiv <= 0) return hipErrorInvalidValue;
+  if (p.Kp % lp_stage(p.prec) != 0 || p.zdiv <= 0) return hipErrorInvalidValue;
   if (tile <= 0) tile = lp_auto_tile(p);
+  if (p.prec == MPSD_PREC_FP64) {
+    switch (tile) {
+      case 2: return pk64_launch<2, 1>(p, s);
+      case 3: return pk64_launch<1, 2>(p, s);
+      case 4: return pk64_launch<2, 2>(p, s);
+      default: return pk64_launch<1, 1>(p, s);
+    }
+  }
   if (p.prec == MPSD_PREC_BF16X3) return lp_launch_tile<MPSD_PREC_BF16X3>(p, tile, s);
   if (p.prec == MPSD_PREC_FP32) return lp_launch_tile<MPSD_PREC_FP32>(p, tile, s);
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_lp_pack(const LpPackDesc& p, int prec, hipStream_t s) {
-  const int64_t total = (int64_t)p.nb * p.rows * (p.Kp / 8);
+  const int64_t total = (int64_t)p.nb * p.rows * (p.Kp / (prec == MPSD_PREC_FP64 ? 4 : 8));
   if (total <= 0) return hipSuccess;
-  if (p.Kp % 32 != 0) return hipErrorInvalidValue;
-  if (prec == MPSD_PREC_BF16X3)
+  if (p.Kp % lp_stage(prec) != 0) return hipErrorInvalidValue;
+  if (prec == MPSD_PREC_FP64)
+    hipLaunchKernelGGL(pk64_pack_kernel, dim3(grid_for(total)), dim3(kT), 0, s, p);
+  else if (prec == MPSD_PREC_BF16X3)
     hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_BF16X3>, dim3(grid_for(total)), dim3(kT), 0, s, p);
   else if (prec == MPSD_PREC_FP32)
     hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_FP32>, dim3(grid_for(total)), dim3(kT), 0, s, p);
@@ -423,6 +638,15 @@ hipError_t launch_lp_mix(const float2* T1, void* T2p, const double2* mix, int Db
                        nout, Dkr, Kp);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_pk64_mix(const double2* T1, double2* T2p, const double2* mix, int Db, int nin, int nout, int Dkr,
+                           int Kp, hipStream_t s) {
+  const int64_t total = (int64_t)Db * Kp;
+  if (total <= 0) return hipSuccess;
+  if (Kp % 16 != 0 || nout > kMixMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pk64_mix_kernel, dim3(grid_for(total)), dim3(kT), 0, s, T1, T2p, mix, Db, nin, nout, Dkr, Kp);
   return hipGetLastError();
 }
 

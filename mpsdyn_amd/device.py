@@ -33,6 +33,8 @@ OPT_ENV_CHAIN = 7
 OPT_KRYLOV = 8
 OPT_LP_PACKED = 9
 OPT_LP_TILE = 10
+OPT_PACKED_FP64 = 11
+OPT_PACKED_FP64_TILE = 12
 
 _lib = None
 
