@@ -179,13 +179,21 @@ __global__ void __launch_bounds__(kT) lp_sum_kernel(const float2* __restrict__ P
   y[i] = double2{re, im};
 }
 
-__device__ __forceinline__ void lp_tile_of(int gm, int gn, int& tm_i, int& tn_i, int& z) {
+// Workgroup -> (tile, batch).  Ids are regrouped so each XCD owns a contiguous range; within it
+// the first batch index (z % zdiv: the physical index s or s' of the stages) is fastest, so the
+// zdiv workgroups of one tile run side by side on one XCD: they share the common operand's
+// panel in its L2 and write the interleaved halves of the same output lines.  Tiles are walked
+// in groups of kGroupM row-tiles; the second batch index (z / zdiv) is slowest.
+__device__ __forceinline__ void lp_tile_of(int gm, int gn, int zdiv, int& tm_i, int& tn_i, int& z) {
   const int total = gridDim.x;
   int id = blockIdx.x;
   if ((total & 7) == 0) id = (id & 7) * (total >> 3) + (id >> 3);  // contiguous range per XCD
+  const int z0 = id % zdiv;
+  const int rest = id / zdiv;
   const int per_z = gm * gn;
-  z = id / per_z;
-  const int r = id - z * per_z;
+  const int z1 = rest / per_z;
+  const int r = rest - z1 * per_z;
+  z = z0 + zdiv * z1;
   const int in_group = kGroupM * gn;
   const int g = r / in_group;
   const int first_m = g * kGroupM;
@@ -214,7 +222,7 @@ __global__ void __launch_bounds__(kT) lp_brgemm_kernel(const LpDesc p, int gm, i
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   int tm_i, tn_i, z;
-  lp_tile_of(gm, gn, tm_i, tn_i, z);
+  lp_tile_of(gm, gn, p.zdiv, tm_i, tn_i, z);
   const int m0 = tm_i * TM, n0 = tn_i * TN;
   const int z0 = z % p.zdiv, z1 = z / p.zdiv;
   const u4v* Ab = reinterpret_cast<const u4v*>(p.A) + (z0 * p.sA0 + z1 * p.sA1) / 2;
@@ -445,7 +453,7 @@ __global__ void __launch_bounds__(kT) pk64_brgemm_kernel(const LpDesc p, int gm,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   int tm_i, tn_i, z;
-  lp_tile_of(gm, gn, tm_i, tn_i, z);
+  lp_tile_of(gm, gn, p.zdiv, tm_i, tn_i, z);
   const int m0 = tm_i * TM, n0 = tn_i * TN;
   const int z0 = z % p.zdiv, z1 = z / p.zdiv;
   const u4v* Ab = reinterpret_cast<const u4v*>(p.A) + (z0 * p.sA0 + z1 * p.sA1);
@@ -595,7 +603,7 @@ int lp_auto_tile(const LpDesc& p) {
 
 hipError_t launch_lp_brgemm(const LpDesc& p, int tile, hipStream_t s) {
   if (p.M <= 0 || p.N <= 0 || p.NB <= 0) return hipSuccess;
-  if (p.Kp % lp_stage(p.prec) != 0 || p.zdiv <= 0) return hipErrorInvalidValue;
+  if (p.Kp % lp_stage(p.prec) != 0 || p.zdiv <= 0 || p.NB % p.zdiv != 0) return hipErrorInvalidValue;
   if (tile <= 0) tile = lp_auto_tile(p);
   if (p.prec == MPSD_PREC_FP64) {
     switch (tile) {
