@@ -480,8 +480,10 @@ def main():
         ms, n = ctx.timing(k)
         kern[nm] = {"avg_us": 1e3 * ms / max(n, 1), "launches": n}
     ctx.set_timing(False)
-    # the packed-operand path (pk64_brgemm) runs when its pack kernels ran in the pre-pass
-    packed = kern["pack_operands"]["launches"] > 0
+    # the packed-operand path (pk64_brgemm) runs when it is on and the last stage is split over
+    # the MPO bond (d = 2, w >= 2, fewer than 512 32x32 tiles of the unsplit last stage)
+    packed = (ctx.get_option(dev.OPT_PACKED_FP64) == 1 and d == 2 and w >= 2
+              and ((D + 31) // 32) ** 2 < 512)
     kname = "pk64_brgemm" if packed else "zbrgemm_f64"
     if kern["stage1_env_x_site"]["avg_us"] >= kern["stage3_heff"]["avg_us"]:
         dom, dom_kind, dom_flops = f"{kname} stage1 (env x site)", 0, s1

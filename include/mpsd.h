@@ -98,6 +98,8 @@ int mpsd_ctx_set_tile(mpsd_ctx* ctx, int tm, int tn);
                                    where the last stage is split over the MPO bond (same bits as 0);
                                    0 operands read in place */
 #define MPSD_OPT_PACKED_FP64_TILE 12 /* its BRGEMM tile, as MPSD_OPT_LP_TILE */
+#define MPSD_OPT_PACKED_FP64_X 13 /* its first stage on x: -1 automatic (default), 0 x read in place,
+                                     1 x packed per call (same bits either way) */
 int mpsd_ctx_set_option(mpsd_ctx* ctx, int option, int value);
 int mpsd_ctx_get_option(const mpsd_ctx* ctx, int option);
 

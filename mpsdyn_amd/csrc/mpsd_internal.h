@@ -139,6 +139,10 @@ struct LpDesc {
   int M = 0, N = 0, Kp = 0, NB = 1, zdiv = 1;
   int prec = MPSD_PREC_FP32;
   int a_il = 0;  // A row-interleaved: 16-byte chunk c of group G (4 chunks) of row m at (G M + m) 4 + c
+  // FP64 only: B read in place, row n and contraction index k at n sBn + k sBk (16-byte values),
+  // zero for k >= Kb
+  int b_ip = 0, Kb = 0;
+  int64_t sBk = 0, sBn = 0;
   int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0;  // batch strides (packed elements)
   int64_t sCm = 0, sCn = 0, sC0 = 0, sC1 = 0;  // output strides (float2 elements)
 };
@@ -158,6 +162,9 @@ hipError_t launch_pk64_mix(const double2* T1, double2* T2p, const double2* mix, 
                            int Kp, hipStream_t s);
 // tile: 1 = 32 x 32, 2 = 64 x 32, 3 = 32 x 64, 4 = 64 x 64, 0 = lp_auto_tile
 int lp_auto_tile(const LpDesc& p);
+// The tile of a packed launch: the tuning candidate, the user's option, the tuned table or 0.
+// Call it where the launch is issued (inside timed), so a tuning pass records its kind.
+GemmKey lp_key(const LpDesc& g);
 hipError_t launch_lp_brgemm(const LpDesc& p, int tile, hipStream_t s);
 hipError_t launch_lp_pack(const LpPackDesc& p, int prec, hipStream_t s);
 // T2p[o] (rows b, k = a_r, packed row-interleaved) = sum_i mix[o + nout i] T1[b + Db i + Db nin a_r]
@@ -195,6 +202,7 @@ struct mpsd_ctx {
   int lp_tile = 0;    // packed BRGEMM tile: 0 automatic, 1 32x32, 2 64x32, 3 32x64, 4 64x64
   int pk64 = 1;       // FP64 matvec on packed operands where its last stage is split (same bits)
   int pk64_tile = 0;  // its tile, as lp_tile
+  int pk64_x = -1;    // its stage 1 on x: -1 automatic, 0 read in place, 1 packed per call
   // Problem tables of grouped launches: device buffer + pinned host staging, reused after
   // the event of the previous upload has completed.
   void* gtab = nullptr;
@@ -346,6 +354,7 @@ bool stage3_split(const mpsd_heff* h, const mpsd_ctx* ctx);
 // explicit context override, else the tuned entry of g's shape class, else built-in.  Every
 // configuration gives bit-identical results, so the choice only moves time.
 TileCfg tile_for(mpsd_ctx* ctx, const ZgemmDesc& g);
+int lp_tile_for(mpsd_ctx* ctx, const LpDesc& g);
 // Run `launch` on the context's stream, bracketed by HIP events when timing is on.
 hipError_t timed(mpsd_ctx* ctx, int kind, const std::function<hipError_t()>& launch);
 int flush_timing(mpsd_ctx* ctx);

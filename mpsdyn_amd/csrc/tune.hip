@@ -41,6 +41,43 @@ TileCfg tile_for(mpsd_ctx* ctx, const ZgemmDesc& g) {
   return TileCfg{};
 }
 
+// Packed-operand BRGEMMs (zgemm_lp.hip) share the tuner: their shape class carries bit 12 and
+// their arithmetic, the candidate's tile maps onto the packed tiles (other candidates: the
+// automatic choice), and every packed tile gives the same bits.
+GemmKey lp_key(const LpDesc& g) {
+  GemmKey k;
+  k.M = g.M;
+  k.N = g.N;
+  k.K = g.Kp;
+  k.R = 0;
+  k.NB = g.NB;
+  k.flags = 4096 | ((g.prec & 3) << 4) | (g.a_il ? 8192 : 0) | (g.b_ip ? 16384 : 0);
+  return k;
+}
+
+// (a candidate with no packed counterpart, such as 16 x 16, times the automatic choice again)
+static int lp_tile_of_cfg(const TileCfg& c) {
+  if (c.tm == 32 && c.tn == 32) return 1;
+  if (c.tm == 64 && c.tn == 32) return 2;
+  if (c.tm == 32 && c.tn == 64) return 3;
+  if (c.tm == 64 && c.tn == 64) return 4;
+  return 0;
+}
+
+int lp_tile_for(mpsd_ctx* ctx, const LpDesc& g) {
+  if (ctx->tune_cand) {
+    ctx->tune_seen.push_back({ctx->cur_kind, lp_key(g)});
+    return lp_tile_of_cfg(*ctx->tune_cand);
+  }
+  const int user = g.prec == MPSD_PREC_FP64 ? ctx->pk64_tile : ctx->lp_tile;
+  if (user) return user;
+  if (!ctx->tuned.empty()) {
+    auto it = ctx->tuned.find(lp_key(g));
+    if (it != ctx->tuned.end()) return lp_tile_of_cfg(it->second);
+  }
+  return 0;
+}
+
 namespace {
 
 // Candidates: the built-in choice first (kept unless another is clearly faster).

@@ -479,14 +479,22 @@ __global__ void __launch_bounds__(kT) pk64_brgemm_kernel(const LpDesc p, int gm,
     const int64_t row = min(m0 + ar16 + 16 * e, p.M - 1);
     arow[e] = Ab + (p.a_il ? ((ac16 >> 2) * (int64_t)p.M + row) * 4 + (ac16 & 3) : row * rowq + ac16);
   }
+  // B read in place (b_ip: rows at sBn, k at sBk, the value of slot c = 4 q + u being
+  // k = 16 t + 4 u + q, zero past Kb) or packed
+  const int bk = 4 * (c16 & 3) + (c16 >> 2);
+  const int64_t bstep = p.b_ip ? 16 * p.sBk : 16;
 #pragma unroll
-  for (int e = 0; e < BL; ++e) brow[e] = Bb + (int64_t)min(n0 + r16 + 16 * e, p.N - 1) * rowq + c16;
+  for (int e = 0; e < BL; ++e) {
+    const int64_t row = min(n0 + r16 + 16 * e, p.N - 1);
+    brow[e] = Bb + (p.b_ip ? row * p.sBn + bk * p.sBk : row * rowq + c16);
+  }
   u4v ra[AL], rb[BL];
   auto load = [&](int t) {
 #pragma unroll
     for (int e = 0; e < AL; ++e) ra[e] = arow[e][astep * t];
+    const bool bin = !p.b_ip || 16 * t + bk < p.Kb;
 #pragma unroll
-    for (int e = 0; e < BL; ++e) rb[e] = brow[e][16 * t];
+    for (int e = 0; e < BL; ++e) rb[e] = bin ? brow[e][bstep * t] : u4v{0u, 0u, 0u, 0u};
   };
   auto store = [&](int buf) {
 #pragma unroll

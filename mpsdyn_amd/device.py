@@ -35,6 +35,7 @@ OPT_LP_PACKED = 9
 OPT_LP_TILE = 10
 OPT_PACKED_FP64 = 11
 OPT_PACKED_FP64_TILE = 12
+OPT_PACKED_FP64_X = 13
 
 _lib = None
 
@@ -453,6 +454,9 @@ class Context:
 
     def set_option(self, option, value):
         self.check(self.lib.mpsd_ctx_set_option(self._h, option, value))
+
+    def get_option(self, option):
+        return self.lib.mpsd_ctx_get_option(self._h, option)
 
     # tile autotuner table (mpsd_tune_*)
     def tune_env(self, side, env, ket, W, bra, out, reps=5):

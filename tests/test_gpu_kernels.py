@@ -224,9 +224,9 @@ def test_env_chain_bit_identical(ctx, d, Dl, Dr, wl, wr):
 @pytest.mark.parametrize("Db,Dbr,wl,wr", [(20, 20, 3, 3), (37, 50, 5, 5), (1, 4, 1, 3), (4, 1, 3, 1), (3, 2, 3, 2),
                                           (100, 70, 3, 5), (128, 128, 5, 5), (512, 512, 3, 3), (400, 400, 5, 5)])
 def test_packed_fp64_matvec_same_bits(ctx, Db, Dbr, wl, wr):
-    """The FP64 matvec on packed operands (L, R packed once per handle, x per call) equals the
-    matvec on operands read in place bit for bit, for every BRGEMM tile, with the offset term;
-    both match the oracle to 1e-12."""
+    """The FP64 matvec on packed operands (L, R packed once per handle; x packed per call or read
+    in place by the first stage) equals the matvec on operands read in place bit for bit, for
+    every BRGEMM tile, with the offset term; both match the oracle to 1e-12."""
     from mpsdyn_amd import device as dev
     print(f"Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
     L = orc.complex_gaussian(11, (Db, wl, Db))
@@ -239,12 +239,15 @@ def test_packed_fp64_matvec_same_bits(ctx, Db, Dbr, wl, wr):
         ctx.set_option(dev.OPT_PACKED_FP64, 0)
         plain = h.apply(x)
         ctx.set_option(dev.OPT_PACKED_FP64, 1)
-        for tile in (0, 1, 2, 3, 4):
-            ctx.set_option(dev.OPT_PACKED_FP64_TILE, tile)
-            outs.append(h.apply(x))
+        for xin in (0, 1):  # x read in place by the first stage, or packed first
+            ctx.set_option(dev.OPT_PACKED_FP64_X, xin)
+            for tile in (0, 1, 2, 3, 4):
+                ctx.set_option(dev.OPT_PACKED_FP64_TILE, tile)
+                outs.append(h.apply(x))
     finally:
         ctx.set_option(dev.OPT_PACKED_FP64, 1)
         ctx.set_option(dev.OPT_PACKED_FP64_TILE, 0)
+        ctx.set_option(dev.OPT_PACKED_FP64_X, -1)
     for k, y in enumerate(outs):
         assert np.array_equal(y, plain), f"tile option {k} differs from the in-place matvec"
     ref = orc.heff_product(L, W, R, x, offset=0.45)
