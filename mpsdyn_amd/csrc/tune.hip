@@ -136,9 +136,12 @@ int tune_op(mpsd_ctx* ctx, int reps, const std::function<int()>& op, mpsd_tune_r
   for (auto& kv : stats) {
     const Stat& st = kv.second;
     if (((kv.first.flags >> 4) & 3) == 2) continue;  // split-BF16 form: fixed tiles
+    const bool packed = (kv.first.flags & 4096) != 0;
     int best = 0;
-    for (int c = 1; c < kNumCandidates; ++c)
+    for (int c = 1; c < kNumCandidates; ++c) {
+      if (packed && lp_tile_of_cfg(kCandidates[c]) == 0) continue;  // the automatic tile again
       if (st.us[c] > 0.0 && st.us[c] < st.us[best] * (best == 0 ? kKeepBuiltin : 1.0)) best = c;
+    }
     ctx->tuned[kv.first] = kCandidates[best];
     if (out && rows < max_out) {
       mpsd_tune_result& r = out[rows];
