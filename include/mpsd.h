@@ -314,6 +314,12 @@ int mpsd_vec_copy(mpsd_ctx* ctx, const double* x, double* y, int64_t n);
 int mpsd_vec_zero(mpsd_ctx* ctx, double* x, int64_t n);
 /* Copy `bytes` from device memory to host after the context's queued work (synchronises). */
 int mpsd_copy_to_host(mpsd_ctx* ctx, const void* dev, void* host, int64_t bytes);
+/* The same in two halves: mpsd_fetch_async queues the copy of up to 64 KiB into pinned landing
+ * slot 0 or 1 behind the context's queued work and returns; mpsd_fetch_wait waits for that copy
+ * and moves the bytes to host.  Work queued after the fetch runs on meanwhile (the eigensolver
+ * queues its next chunk of steps before it reads the coefficients of the last one). */
+int mpsd_fetch_async(mpsd_ctx* ctx, const void* dev, int64_t bytes, int slot);
+int mpsd_fetch_wait(mpsd_ctx* ctx, int slot, void* host, int64_t bytes);
 /* out = sum |x_i|^2 (synchronises the stream). */
 int mpsd_vec_norm2(mpsd_ctx* ctx, const double* x, int64_t n, double* out);
 

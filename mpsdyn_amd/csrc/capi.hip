@@ -357,6 +357,9 @@ int mpsd_ctx_destroy(mpsd_ctx* ctx) {
   if (ctx->gtab_ev) (void)hipEventDestroy(ctx->gtab_ev);
   if (ctx->coef) (void)hipFree(ctx->coef);
   if (ctx->coef_host) (void)hipHostFree(ctx->coef_host);
+  if (ctx->fetch_host) (void)hipHostFree(ctx->fetch_host);
+  for (hipEvent_t e : ctx->fetch_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& kv : ctx->pool) (void)hipFree(kv.second);
   ctx->pool.clear();
   if (ctx->up_host) (void)hipHostFree(ctx->up_host);

@@ -288,28 +288,48 @@ EigStats LanczosSolver::lowest(Multiplier& H, double* x, double tol, int max_kry
   double* ab = h2 + 2 * (m + 2);  // |x|^2, 0, then alpha_j, beta_j
   std::vector<double> abh(2 * m + 4), s;
   bool seeded = false;
+  // steps jj0 .. jj1 - 1 of the basis, then the coefficients so far into fetch slot `slot`
+  auto run_chunk = [&](int jj0, int jj1, int slot) {
+    for (int jj = jj0; jj < jj1; ++jj) {
+      double* w = vec(jj + 1);
+      H.product_device(vec(jj), w);
+      ++st.matvecs;
+      if (jj + 1 <= kFusedGsCols) {
+        check(ctx_, mpsd_lanczos_gs_step_dev(ctx_, V, ldv, jj, w, n, h1, h2, ab + 2), "Gram-Schmidt step");
+      } else {
+        check(ctx_, mpsd_lanczos_gs1_dev(ctx_, V, ldv, jj + 1, w, n, h1), "Gram-Schmidt pass 1");
+        check(ctx_, mpsd_lanczos_gs2_dev(ctx_, V, ldv, jj, h1, h2, ab + 2, w, n), "Gram-Schmidt pass 2 + coef");
+      }
+    }
+    check(ctx_, mpsd_fetch_async(ctx_, ab, (int64_t)sizeof(double) * (2 + 2 * jj1), slot), "coef fetch");
+  };
   for (int restart = 0; restart <= max_restarts; ++restart) {
     st.restarts = restart;
     check(ctx_, mpsd_vec_normalize_dev(ctx_, x, vec(0), n, ab), "normalise");
     int K = 0;  // size of the Krylov basis used for the Ritz vector
     bool stop = false;
     double theta = 0.0, res = 0.0;
+    int slot = 0;
+    bool queued = false;  // the chunk starting at j was queued ahead, its coefficients in `slot`
     for (int j = 0; j < m && !stop;) {
       const int k = lanczos_chunk(j);
       if (j + k > m && j > 0) break;  // the tail shorter than a chunk ends this restart
       const int jend = std::min(m, j + k);
-      for (int jj = j; jj < jend; ++jj) {
-        double* w = vec(jj + 1);
-        H.product_device(vec(jj), w);
-        ++st.matvecs;
-        if (jj + 1 <= kFusedGsCols) {
-          check(ctx_, mpsd_lanczos_gs_step_dev(ctx_, V, ldv, jj, w, n, h1, h2, ab + 2), "Gram-Schmidt step");
-        } else {
-          check(ctx_, mpsd_lanczos_gs1_dev(ctx_, V, ldv, jj + 1, w, n, h1), "Gram-Schmidt pass 1");
-          check(ctx_, mpsd_lanczos_gs2_dev(ctx_, V, ldv, jj, h1, h2, ab + 2, w, n), "Gram-Schmidt pass 2 + coef");
+      if (!queued) run_chunk(j, jend, slot);
+      // Queue the next chunk before reading this one's coefficients, so the device works through
+      // the host's round trip.  Not from the first chunk, where late sweeps converge: an unused
+      // chunk costs its matvecs.  Its steps only extend the basis past what this chunk's test
+      // can keep, so the basis, the Ritz vector and the value are the same either way.
+      queued = false;
+      if (j > 0 && jend < m) {
+        const int k2 = lanczos_chunk(jend);
+        if (jend + k2 <= m) {
+          run_chunk(jend, jend + k2, slot ^ 1);
+          queued = true;
         }
       }
-      check(ctx_, mpsd_copy_to_host(ctx_, ab, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef download");
+      check(ctx_, mpsd_fetch_wait(ctx_, slot, abh.data(), (int64_t)sizeof(double) * (2 + 2 * jend)), "coef read");
+      slot ^= 1;
       if (j == 0) {
         const double nx = abh[0];
         if (!(nx > 0.0) || !std::isfinite(nx)) {

@@ -803,6 +803,29 @@ int mpsd_vec_axpby(mpsd_ctx* ctx, const double* a, const double* x, const double
   return hip_err(ctx, hipGetLastError(), "vec_axpby");
 }
 
+int mpsd_fetch_async(mpsd_ctx* ctx, const void* dev, int64_t bytes, int slot) {
+  if (!ctx || !dev || bytes < 0 || bytes > mpsd_ctx::kFetchBytes || slot < 0 || slot > 1) return MPSD_ERR_ARG;
+  if (!ctx->fetch_host) {
+    hipError_t e = hipHostMalloc(&ctx->fetch_host, 2 * mpsd_ctx::kFetchBytes, hipHostMallocDefault);
+    for (int i = 0; e == hipSuccess && i < 2; ++i) e = hipEventCreateWithFlags(&ctx->fetch_ev[i], hipEventDisableTiming);
+    if (e != hipSuccess) return hip_err(ctx, e, "fetch alloc");
+  }
+  // the slot's previous copy has been waited for by its reader (mpsd_fetch_wait)
+  char* h = (char*)ctx->fetch_host + slot * mpsd_ctx::kFetchBytes;
+  hipError_t e = bytes ? hipMemcpyAsync(h, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream) : hipSuccess;
+  if (e == hipSuccess) e = hipEventRecord(ctx->fetch_ev[slot], ctx->stream);
+  return hip_err(ctx, e, "fetch_async");
+}
+
+int mpsd_fetch_wait(mpsd_ctx* ctx, int slot, void* host, int64_t bytes) {
+  if (!ctx || !host || bytes < 0 || bytes > mpsd_ctx::kFetchBytes || slot < 0 || slot > 1 || !ctx->fetch_host)
+    return MPSD_ERR_ARG;
+  hipError_t e = hipEventSynchronize(ctx->fetch_ev[slot]);
+  if (e != hipSuccess) return hip_err(ctx, e, "fetch_wait");
+  memcpy(host, (const char*)ctx->fetch_host + slot * mpsd_ctx::kFetchBytes, (size_t)bytes);
+  return MPSD_OK;
+}
+
 int mpsd_copy_to_host(mpsd_ctx* ctx, const void* dev, void* host, int64_t bytes) {
   if (!ctx || !dev || !host || bytes < 0) return MPSD_ERR_ARG;
   if (bytes == 0) return MPSD_OK;

@@ -219,6 +219,10 @@ struct mpsd_ctx {
   void* coef_host = nullptr;  // pinned host ring, same layout
   hipEvent_t coef_ev[8] = {nullptr};
   unsigned coef_slot = 0;
+  // Two pinned landing slots for asynchronous reads of small device results (mpsd_fetch_async).
+  static constexpr int64_t kFetchBytes = 64 << 10;
+  void* fetch_host = nullptr;
+  hipEvent_t fetch_ev[2] = {nullptr, nullptr};
   // Staging for host-buffer matvecs.
   void* stage[2] = {nullptr, nullptr};
   size_t stage_bytes[2] = {0, 0};
