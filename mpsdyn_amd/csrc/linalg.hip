@@ -155,15 +155,19 @@ inline int gs_blocks(int64_t n) {
   return b < 1 ? 1 : (b > kGsMaxBlocks ? kGsMaxBlocks : (int)b);
 }
 
-// w1[t] = w[t] - sum_{i<m1} V_i[t] c[i], gemv_elem's operations; the columns are left in v.
+// Columns V_i[t] (i < m1) into v and w[t] into wt.
 template <int M>
-__device__ __forceinline__ double2 gs_w1(const double2* __restrict__ V, int64_t ldv, int m1,
-                                         const double2* c, const double2* __restrict__ w, int64_t t,
-                                         double2 (&v)[M]) {
+__device__ __forceinline__ void gs_load(const double2* __restrict__ V, int64_t ldv, int m1,
+                                        const double2* __restrict__ w, int64_t t, double2 (&v)[M], double2& wt) {
 #pragma unroll
   for (int i = 0; i < M; ++i)
     if (i < m1) v[i] = V[(int64_t)i * ldv + t];
-  const double2 wi = w[t];
+  wt = w[t];
+}
+
+// w1 = wi - sum_{i<m1} v[i] c[i], gemv_elem's operations on the loaded columns.
+template <int M>
+__device__ __forceinline__ double2 gs_w1(int m1, const double2* c, double2 wi, const double2 (&v)[M]) {
   double2 acc{0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < M; ++i)
@@ -257,6 +261,13 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
   __shared__ double2 c[M];
   __shared__ double2 red[kT / 64][M + 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t chunk = (n + nb2 - 1) / nb2;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = min(n, lo + chunk);
+  // the first element's columns and w are loaded before h1 is folded, so the two memory
+  // latencies overlap (they do not depend on each other)
+  double2 v0[M], w0{0.0, 0.0};
+  if (lo + threadIdx.x < hi) gs_load<M>(V, ldv, m1, w, lo + threadIdx.x, v0, w0);
   gs_fold_waves<M>(p1, kDotBlocks, nb1, m1, [&](int i, double2 v) {
     c[i] = v;
     if (blockIdx.x == 0) h1[i] = v;
@@ -267,15 +278,19 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
 #pragma unroll
   for (int i = 0; i < S; ++i) acc[i] = double2{0.0, 0.0};
   double nrm = 0.0;
-  const int64_t chunk = (n + nb2 - 1) / nb2;
-  const int64_t lo = (int64_t)blockIdx.x * chunk;
-  const int64_t hi = min(n, lo + chunk);
   for (int64_t t = lo + threadIdx.x; t < hi; t += kT) {
     // h1 is read from LDS at its use: kept in registers across the loop it would cost 4 M
     // VGPRs on top of the columns and the accumulators
     __asm__ volatile("" ::: "memory");
-    double2 v[M];
-    const double2 w1 = gs_w1<M>(V, ldv, m1, c, w, t, v);
+    double2 v[M], wt;
+    if (t == lo + threadIdx.x) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) v[i] = v0[i];
+      wt = w0;
+    } else {
+      gs_load<M>(V, ldv, m1, w, t, v, wt);
+    }
+    const double2 w1 = gs_w1<M>(m1, c, wt, v);
 #pragma unroll
     for (int i = 0; i < M; ++i)
       if (i < m1) {  // dot_block's operations
@@ -338,6 +353,10 @@ __global__ void __launch_bounds__(kT) gs_update_kernel(const double2* __restrict
                                                        double2* __restrict__ w, int64_t n) {
   __shared__ double2 c1[M];
   __shared__ double2 c2[M + 1];
+  // the element's columns and w first: their latency overlaps the coefficient fold's
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  double2 v[M], wt{0.0, 0.0};
+  if (t < n) gs_load<M>(V, ldv, m1, w, t, v, wt);
   if ((int)threadIdx.x < m1) c1[threadIdx.x] = h1[threadIdx.x];
   if (fold) {
     gs_fold_waves<M + 1>(p2, kGsMaxBlocks, nb2, m1 + 1, [&](int i, double2 v) {
@@ -349,10 +368,8 @@ __global__ void __launch_bounds__(kT) gs_update_kernel(const double2* __restrict
   }
   __syncthreads();
   const int j = m1 - 1;
-  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (t < n) {
-    double2 v[M];
-    const double2 w1 = gs_w1<M>(V, ldv, m1, c1, w, t, v);
+    const double2 w1 = gs_w1<M>(m1, c1, wt, v);
     double2 acc{0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < M; ++i)

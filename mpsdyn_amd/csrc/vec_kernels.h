@@ -23,10 +23,23 @@ __device__ inline double2 dot_block(const double2* __restrict__ v, const double2
   const int64_t lo = blk * chunk;
   const int64_t hi = min(n, lo + chunk);
   double2 acc{0.0, 0.0};
-  for (int64_t t = lo + threadIdx.x; t < hi; t += kVecThreads) {
-    const double2 a = v[t], b = w[t];
-    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
-    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
+  // the thread's elements in ascending order, loads issued 8 at a time ahead of their sums
+  for (int64_t t0 = lo + threadIdx.x; t0 < hi; t0 += 8 * kVecThreads) {
+    double2 a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int64_t t = t0 + (int64_t)k * kVecThreads;
+      if (t < hi) {
+        a[k] = v[t];
+        b[k] = w[t];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (t0 + (int64_t)k * kVecThreads < hi) {
+        acc.x = fma(a[k].x, b[k].x, fma(a[k].y, b[k].y, acc.x));
+        acc.y = fma(a[k].x, b[k].y, fma(-a[k].y, b[k].x, acc.y));
+      }
   }
   __shared__ double2 red[kVecThreads];
   red[threadIdx.x] = acc;
