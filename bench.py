@@ -455,6 +455,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner (built-in tiles)")
     ap.add_argument("--forms-table", action="store_true",
                     help="only the per-precision-form matvec table (one JSON line per shape and form)")
+    ap.add_argument("--sweeps-only", action="store_true",
+                    help="only the L=100 sweep leg (--sweep-D, --sweep-heisenberg-D): one JSON line per case")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 256 MiB device copy (3x) and 5 matvecs, for rocprofv3 --pmc counter runs")
     args = ap.parse_args()
@@ -467,6 +469,17 @@ def main():
     from mpsdyn_amd import device as dev
     from mpsdyn_amd.synth import complex_gaussian
 
+    if args.sweeps_only:
+        ints = lambda txt: [int(v) for v in txt.split(",") if v.strip()]
+        for D in ints(args.sweep_D):
+            for r in sweep_leg(dev, [(f"ising_L100_g1_h0_D{D}", dev.MODEL_ISING, D, [-1.0, -1.0, 0.0])],
+                               args.sweep_halfsweeps):
+                print(json.dumps(r), flush=True)
+        for D in ints(args.sweep_heisenberg_D):
+            for r in sweep_leg(dev, [(f"heisenberg_xxz_L100_delta1_D{D}", dev.MODEL_HEISENBERG, D,
+                                      [1.0, 1.0, 1.0, 0.0])], args.sweep_halfsweeps):
+                print(json.dumps(r), flush=True)
+        return
     if args.traffic_probe:
         traffic_probe(dev, complex_gaussian, args.D, args.w)
         return
