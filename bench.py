@@ -437,7 +437,7 @@ def main():
     ap.add_argument("--w", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--gs-solves", type=int, default=8)
+    ap.add_argument("--gs-solves", type=int, default=100)
     ap.add_argument("--batched", type=int, default=64)
     ap.add_argument("--served", type=int, default=24)
     ap.add_argument("--rate", type=float, default=1.0)
