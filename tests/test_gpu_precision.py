@@ -43,6 +43,25 @@ def test_lowp_matvec_accuracy(ctx, prec, tol, Db, Dbr, wl, wr):
     h.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6), (2, 1e-4)])
+@pytest.mark.parametrize("d,Db,Dbr,wl,wr", [(3, 24, 17, 3, 5), (1, 40, 33, 4, 4), (4, 16, 16, 2, 3),
+                                            (3, 96, 96, 5, 5)])
+def test_matvec_other_physical_dimensions(ctx, prec, tol, d, Db, Dbr, wl, wr):
+    """Physical dimension d != 2 (spin-1 chains, d = 1 and 4 as well) in every arithmetic form:
+    the packed precision forms take any d, the FP64 form its in-place path, all against the
+    oracle's product."""
+    from mpsdyn_amd import device as dev
+    print(f"prec={prec} d={d} Db={Db} Dbr={Dbr} wl={wl} wr={wr}", flush=True)
+    L = orc.complex_gaussian(61, (Db, wl, Db))
+    W = orc.complex_gaussian(62, (d, wl, d, wr))
+    R = orc.complex_gaussian(63, (Dbr, wr, Dbr))
+    x = orc.complex_gaussian(64, (d * Db * Dbr,))
+    h = dev.Heff(ctx, ctx.tensor(L), ctx.tensor(W), ctx.tensor(R), offset=-0.4, precision=prec)
+    err = _rel(h.apply(x), orc.heff_product(L, W, R, x, offset=-0.4))
+    assert err <= tol, err
+    h.close()
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_mixed_precision_schwinger_energy(ctx, prec):
     """Schwinger N=12, x=100, m/g=0.125: mixed-precision energy within 1e-10 of the FP64
