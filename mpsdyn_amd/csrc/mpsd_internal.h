@@ -145,6 +145,7 @@ struct LpDesc {
   int64_t sBk = 0, sBn = 0;
   int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0;  // batch strides (packed elements)
   int64_t sCm = 0, sCn = 0, sC0 = 0, sC1 = 0;  // output strides (float2 elements)
+  int group = 0;  // row-tiles per tile group of the workgroup order (0: by the launcher)
 };
 // Pack src(b, row, k) = src[b*sb + row*srow + k*sk] (k < K, FP64) into dst (nb x rows x Kp).
 struct LpPackDesc {

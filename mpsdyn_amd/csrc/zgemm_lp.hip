@@ -41,7 +41,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kPitch = 17;  // uint4 per LDS row: 16 for the 32-deep stage + 1 of padding
-constexpr int kGroupM = 8;
+constexpr int kGroupM = 8;  // row-tiles per tile group (split-BF16 and FP32 launches)
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
   unsigned u = __float_as_uint(f);
@@ -183,8 +183,8 @@ __global__ void __launch_bounds__(kT) lp_sum_kernel(const float2* __restrict__ P
 // the first batch index (z % zdiv: the physical index s or s' of the stages) is fastest, so the
 // zdiv workgroups of one tile run side by side on one XCD: they share the common operand's
 // panel in its L2 and write the interleaved halves of the same output lines.  Tiles are walked
-// in groups of kGroupM row-tiles; the second batch index (z / zdiv) is slowest.
-__device__ __forceinline__ void lp_tile_of(int gm, int gn, int zdiv, int& tm_i, int& tn_i, int& z) {
+// in groups of `group` row-tiles; the second batch index (z / zdiv) is slowest.
+__device__ __forceinline__ void lp_tile_of(int gm, int gn, int zdiv, int group, int& tm_i, int& tn_i, int& z) {
   const int total = gridDim.x;
   int id = blockIdx.x;
   if ((total & 7) == 0) id = (id & 7) * (total >> 3) + (id >> 3);  // contiguous range per XCD
@@ -194,10 +194,10 @@ __device__ __forceinline__ void lp_tile_of(int gm, int gn, int zdiv, int& tm_i, 
   const int z1 = rest / per_z;
   const int r = rest - z1 * per_z;
   z = z0 + zdiv * z1;
-  const int in_group = kGroupM * gn;
+  const int in_group = group * gn;
   const int g = r / in_group;
-  const int first_m = g * kGroupM;
-  const int gsize = min(gm - first_m, kGroupM);
+  const int first_m = g * group;
+  const int gsize = min(gm - first_m, group);
   const int rr = r - g * in_group;
   tm_i = first_m + rr % gsize;
   tn_i = rr / gsize;
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kT) lp_brgemm_kernel(const LpDesc p, int gm, i
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   int tm_i, tn_i, z;
-  lp_tile_of(gm, gn, p.zdiv, tm_i, tn_i, z);
+  lp_tile_of(gm, gn, p.zdiv, p.group > 0 ? p.group : kGroupM, tm_i, tn_i, z);
   const int m0 = tm_i * TM, n0 = tn_i * TN;
   const int z0 = z % p.zdiv, z1 = z / p.zdiv;
   const u4v* Ab = reinterpret_cast<const u4v*>(p.A) + (z0 * p.sA0 + z1 * p.sA1) / 2;
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(kT) pk64_brgemm_kernel(const LpDesc p, int gm,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   int tm_i, tn_i, z;
-  lp_tile_of(gm, gn, p.zdiv, tm_i, tn_i, z);
+  lp_tile_of(gm, gn, p.zdiv, p.group > 0 ? p.group : kGroupM, tm_i, tn_i, z);
   const int m0 = tm_i * TM, n0 = tn_i * TN;
   const int z0 = z % p.zdiv, z1 = z / p.zdiv;
   const u4v* Ab = reinterpret_cast<const u4v*>(p.A) + (z0 * p.sA0 + z1 * p.sA1);
@@ -563,10 +563,15 @@ __global__ void __launch_bounds__(kT) pk64_brgemm_kernel(const LpDesc p, int gm,
 }
 
 template <int BM, int BN>
-hipError_t pk64_launch(const LpDesc& p, hipStream_t s) {
-  const int gm = (p.M + 32 * BM - 1) / (32 * BM), gn = (p.N + 32 * BN - 1) / (32 * BN);
-  const int64_t total = (int64_t)gm * gn * p.NB;
+hipError_t pk64_launch(const LpDesc& p_in, hipStream_t s) {
+  const int gm = (p_in.M + 32 * BM - 1) / (32 * BM), gn = (p_in.N + 32 * BN - 1) / (32 * BN);
+  const int64_t total = (int64_t)gm * gn * p_in.NB;
   if (total > 0x7fffffff) return hipErrorInvalidConfiguration;
+  // Row-tiles per group: tall grids (the first stage: L's rows against x) walk 16 row-tiles per
+  // column panel, square ones (the last stage) 2; measured FETCH_SIZE at D = 512, w = 3:
+  // 67 and 67 MB per launch against 76 and 76 MB with 8 for both.  Timing is unchanged.
+  LpDesc p = p_in;
+  if (p.group <= 0) p.group = gm >= 4 * gn ? 16 : 2;
   hipLaunchKernelGGL((pk64_brgemm_kernel<BM, BN>), dim3((unsigned)total), dim3(kT), 0, s, p, gm, gn);
   return hipGetLastError();
 }
