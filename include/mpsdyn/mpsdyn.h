@@ -442,7 +442,9 @@ class Contractor {
   // at max(eigtol, min(1e-3, 1e-2 |dE| / |E| of the previous half-sweep, 1e-2 times the previous
   // half-sweep's tolerance)), 1e-3 at the start.  The precision forms always run the schedule.
   void setEigTolSchedule(bool on) { schedule_ = on; }
-  // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).
+  // Contractor::findGroundState (Contractor.h:255-257, Contractor.cpp:1352-1533).  knr, the number
+  // of local eigenpairs the reference asks its eigensolver for, of which it keeps the lowest
+  // (Contractor.cpp:1407-1416), is accepted: the device eigensolver computes that lowest pair.
   void findGroundState(const MPO& ops, int D, double* lambda, MPS& init, double offset = 0.,
                        int knr = 1, const std::string& tmpfile = "", int freqSv = 200, bool useLM = false);
   // <bra|ops|ket> via a full left-environment sweep (Contractor.cpp:528).
