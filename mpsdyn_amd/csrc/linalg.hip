@@ -264,10 +264,6 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
   const int64_t chunk = (n + nb2 - 1) / nb2;
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = min(n, lo + chunk);
-  // the first element's columns and w are loaded before h1 is folded, so the two memory
-  // latencies overlap (they do not depend on each other)
-  double2 v0[M], w0{0.0, 0.0};
-  if (lo + threadIdx.x < hi) gs_load<M>(V, ldv, m1, w, lo + threadIdx.x, v0, w0);
   gs_fold_waves<M>(p1, kDotBlocks, nb1, m1, [&](int i, double2 v) {
     c[i] = v;
     if (blockIdx.x == 0) h1[i] = v;
@@ -278,18 +274,14 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
 #pragma unroll
   for (int i = 0; i < S; ++i) acc[i] = double2{0.0, 0.0};
   double nrm = 0.0;
+  // (the columns are loaded after the fold: loaded before it, they would stay live across the
+  // fold's registers, and at 16 to 32 columns the kernel spills and runs slower)
   for (int64_t t = lo + threadIdx.x; t < hi; t += kT) {
     // h1 is read from LDS at its use: kept in registers across the loop it would cost 4 M
     // VGPRs on top of the columns and the accumulators
     __asm__ volatile("" ::: "memory");
     double2 v[M], wt;
-    if (t == lo + threadIdx.x) {
-#pragma unroll
-      for (int i = 0; i < M; ++i) v[i] = v0[i];
-      wt = w0;
-    } else {
-      gs_load<M>(V, ldv, m1, w, t, v, wt);
-    }
+    gs_load<M>(V, ldv, m1, w, t, v, wt);
     const double2 w1 = gs_w1<M>(m1, c, wt, v);
 #pragma unroll
     for (int i = 0; i < M; ++i)
