@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(kT) coef_fold_kernel(const double2* __restrict
 //   dots_partial_kernel : block partials of h1 (dot_block's blocks);
 //   gs_mid_kernel       : h1 folded from those partials, one wave per coefficient; per element w1 in
 //                         registers and its terms of h2 and |w1|^2, reduced per block (nb2
-//                         blocks);
+//                         blocks; the One instance, with no loop, when a block has at most one
+//                         element per thread);
 //   gs_fold_kernel      : h2 from the block partials (one launch, or folded by a wave inside
 //                         gs_update_kernel when nb2 <= kGsFoldMax);
 //   gs_update_kernel    : w1 again, by the same operations (same bits), then w1 - V h2, scaled
@@ -253,7 +254,7 @@ __device__ __forceinline__ double2 gs_wave_scatter(double2 (&acc)[M]) {
   return r;
 }
 
-template <int M>
+template <int M, bool One>
 __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ V, int64_t ldv, int m1,
                                                     const double2* __restrict__ p1, int nb1,
                                                     const double2* __restrict__ w, int64_t n, int nb2,
@@ -276,6 +277,23 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
   double nrm = 0.0;
   // (the columns are loaded after the fold: loaded before it, they would stay live across the
   // fold's registers, and at 16 to 32 columns the kernel spills and runs slower)
+  if constexpr (One) {
+    // at most one element per thread (chunk <= kT): straight-line, so each column's register
+    // is free once its product is accumulated
+    const int64_t t = lo + threadIdx.x;
+    if (t < hi) {
+      double2 v[M], wt;
+      gs_load<M>(V, ldv, m1, w, t, v, wt);
+      const double2 w1 = gs_w1<M>(m1, c, wt, v);
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < m1) {  // dot_block's operations, from zero
+          acc[i].x = fma(v[i].x, w1.x, fma(v[i].y, w1.y, acc[i].x));
+          acc[i].y = fma(v[i].x, w1.y, fma(-v[i].y, w1.x, acc[i].y));
+        }
+      nrm = fma(w1.x, w1.x, fma(w1.y, w1.y, nrm));
+    }
+  } else {
   for (int64_t t = lo + threadIdx.x; t < hi; t += kT) {
     // h1 is read from LDS at its use: kept in registers across the loop it would cost 4 M
     // VGPRs on top of the columns and the accumulators
@@ -290,6 +308,7 @@ __global__ void __launch_bounds__(kT) gs_mid_kernel(const double2* __restrict__ 
         acc[i].y = fma(v[i].x, w1.y, fma(-v[i].y, w1.x, acc[i].y));
       }
     nrm = fma(w1.x, w1.x, fma(w1.y, w1.y, nrm));
+  }
   }
   const double2 r = gs_wave_scatter<S>(acc);
   constexpr int share = 64 / S;  // lanes holding one coefficient
@@ -777,21 +796,22 @@ int mpsd_lanczos_gs_step_dev(mpsd_ctx* ctx, const double* V, int64_t ldv, int j,
   const int nb1 = dot_blocks(n), nb2 = gs_blocks(n);
   const int fold = nb2 <= kGsFoldMax ? 1 : 0;
   hipLaunchKernelGGL(dots_partial_kernel, dim3(nb1, m1), dim3(kT), 0, s, Vc, ldv, (const double2*)w, n, nb1, p1);
-  auto steps = [&](auto mid, auto upd) {
-    hipLaunchKernelGGL(mid, dim3(nb2), dim3(kT), 0, s, Vc, ldv, m1, (const double2*)p1, nb1, (const double2*)w, n,
+  const bool one = (n + nb2 - 1) / nb2 <= kT;  // gs_mid_kernel's chunk fits one pass
+  auto steps = [&](auto mid1, auto mid, auto upd) {
+    hipLaunchKernelGGL(one ? mid1 : mid, dim3(nb2), dim3(kT), 0, s, Vc, ldv, m1, (const double2*)p1, nb1, (const double2*)w, n,
                        nb2, h1, p2);
     if (!fold) hipLaunchKernelGGL(gs_fold_kernel, dim3(m1 + 1), dim3(kT), 0, s, (const double2*)p2, nb2, h2);
     hipLaunchKernelGGL(upd, dim3(nblk(n)), dim3(kT), 0, s, Vc, ldv, m1, (const double2*)h1, (const double2*)p2,
                        nb2, fold, h2, ab_dev, (double2*)w, n);
   };
   if (m1 <= 8)
-    steps(gs_mid_kernel<8>, gs_update_kernel<8>);
+    steps(gs_mid_kernel<8, true>, gs_mid_kernel<8, false>, gs_update_kernel<8>);
   else if (m1 <= 16)
-    steps(gs_mid_kernel<16>, gs_update_kernel<16>);
+    steps(gs_mid_kernel<16, true>, gs_mid_kernel<16, false>, gs_update_kernel<16>);
   else if (m1 <= 24)
-    steps(gs_mid_kernel<24>, gs_update_kernel<24>);
+    steps(gs_mid_kernel<24, true>, gs_mid_kernel<24, false>, gs_update_kernel<24>);
   else
-    steps(gs_mid_kernel<32>, gs_update_kernel<32>);
+    steps(gs_mid_kernel<32, true>, gs_mid_kernel<32, false>, gs_update_kernel<32>);
   return hip_err(ctx, hipGetLastError(), "lanczos_gs_step");
 }
 
