@@ -409,6 +409,7 @@ __global__ void __launch_bounds__(kT) pk64_pack_kernel(const LpPackDesc p) {
 // ordered (slot u, row b, stage group) so that a wave reads 16 consecutive b of four a_r and
 // writes 1 KB contiguous per output.
 constexpr int kMixMax = 16;
+constexpr unsigned kMixSplitBelow = 512;  // one-row grids smaller than this split by output
 __global__ void __launch_bounds__(kT) pk64_mix_kernel(const double2* __restrict__ T1, double2* __restrict__ T2p,
                                                       const double2* __restrict__ mix, int Db, int nin, int nout,
                                                       int Dkr, int Kp) {
@@ -439,6 +440,34 @@ __global__ void __launch_bounds__(kT) pk64_mix_kernel(const double2* __restrict_
 #pragma unroll
     for (int o = 0; o < kMixMax; ++o)
       if (o < nout) dst[(int64_t)o * Db * Kp] = acc[o];
+  }
+}
+
+// The same sums with one output per grid row (o = blockIdx.y): nout times the workgroups and a
+// sixth of the work per thread, for shapes whose one-row grid leaves most CUs idle (the T1
+// values are read once per output, from L2).  Same operations per output, same bits.
+__global__ void __launch_bounds__(kT) pk64_mix_split_kernel(const double2* __restrict__ T1,
+                                                            double2* __restrict__ T2p,
+                                                            const double2* __restrict__ mix, int Db, int nin,
+                                                            int nout, int Dkr, int Kp) {
+  const int o = blockIdx.y;
+  const int64_t total = (int64_t)Db * Kp;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int u = (int)(t & 3);
+    const int64_t rest = t >> 2;
+    const int b = (int)(rest % Db);
+    const int G = (int)(rest / Db);
+    const int ar = 16 * (G >> 2) + 4 * u + (G & 3);
+    double2 acc{0.0, 0.0};
+    if (ar < Dkr) {
+      for (int i = 0; i < nin; ++i) {
+        const double2 v = T1[b + (int64_t)Db * i + (int64_t)Db * nin * ar];
+        const double2 w = mix[o + nout * i];
+        acc.x = fma(w.x, v.x, fma(-w.y, v.y, acc.x));
+        acc.y = fma(w.x, v.y, fma(w.y, v.x, acc.y));
+      }
+    }
+    T2p[(int64_t)o * Db * Kp + ((int64_t)G * Db + b) * 4 + u] = acc;
   }
 }
 
@@ -672,7 +701,11 @@ hipError_t launch_pk64_mix(const double2* T1, double2* T2p, const double2* mix, 
   const int64_t total = (int64_t)Db * Kp;
   if (total <= 0) return hipSuccess;
   if (Kp % 16 != 0 || nout > kMixMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pk64_mix_kernel, dim3(grid_for(total)), dim3(kT), 0, s, T1, T2p, mix, Db, nin, nout, Dkr, Kp);
+  const unsigned g = grid_for(total);
+  if (g < kMixSplitBelow)
+    hipLaunchKernelGGL(pk64_mix_split_kernel, dim3(g, nout), dim3(kT), 0, s, T1, T2p, mix, Db, nin, nout, Dkr, Kp);
+  else
+    hipLaunchKernelGGL(pk64_mix_kernel, dim3(g), dim3(kT), 0, s, T1, T2p, mix, Db, nin, nout, Dkr, Kp);
   return hipGetLastError();
 }
 
