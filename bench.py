@@ -30,7 +30,7 @@ L from [20, 60] and D from {32, 64, 128}.
 
     python bench.py [--steps K] [--warmup W] [--D 512] [--w 3] [--cpu-seconds 15] [--no-cpu]
                     [--gs-solves N] [--batched N] [--served N] [--rate R] [--reduced-serving]
-                    [--served-convtol T] [--served-only]
+                    [--served-convtol T] [--served-only] [--batched-only]
 """
 from __future__ import annotations
 
@@ -371,7 +371,25 @@ def batched_form(dev, n):
                      "failed": sum(1 for r in res if r.status != 0)}
     out["mixed_vs_fp64_max_rel_energy_diff"] = float(max(abs(a - b) / abs(a) for a, b in
                                                           zip(energies["fp64"], energies["mixed_fp32"])))
-    out["note"] = "fp64 with the local-tolerance schedule; each request runs on its own stream (16 slots)"
+    # the same FP64 solves with the slots' matvecs merged into grouped persistent launches, and
+    # those launches replayed from a captured hipGraph (slower here; reported beside the plain
+    # streams, with the energies' agreement)
+    for name, kw in (("fp64_grouped", {"batching": True}), ("fp64_grouped_graph", {"batching": True, "graph": True})):
+        cfgs = [dev.gs_config(dev.MODEL_ISING, L, D, p, convtol=1e-10, seed=k + 1, precision=0, tol_schedule=True)
+                for k, (L, D, p) in enumerate(specs)]
+        srv = dev.Server(slots=16, **kw)
+        t0 = time.perf_counter()
+        res = srv.run_trace(cfgs, [0.0] * n)
+        wall = time.perf_counter() - t0
+        srv.close()
+        mv = sum(r.matvecs for r in res)
+        out[name] = {"solves_per_s": round(n / wall, 3), "matvecs_per_s": round(mv / wall, 1), "wall_s": round(wall, 3),
+                     "failed": sum(1 for r in res if r.status != 0),
+                     "max_rel_energy_diff_vs_fp64": float(max(abs(r.energy - e) / abs(e)
+                                                              for r, e in zip(res, energies["fp64"])))}
+    out["note"] = ("fp64 with the local-tolerance schedule; each request runs on its own stream (16 slots); "
+                   "fp64_grouped: matvecs of the slots merged into grouped persistent launches; "
+                   "fp64_grouped_graph: those launches replayed from a captured hipGraph")
     return out
 
 
@@ -450,6 +468,7 @@ def main():
     ap.add_argument("--served-convtol", type=float, default=1e-12,
                     help="stopping tolerance of the served solves (|dE| < convtol |E| per half-sweep)")
     ap.add_argument("--served-only", action="store_true", help="only the served leg (one JSON line)")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched leg (one JSON line)")
     ap.add_argument("--reduced-serving", action="store_true",
                     help="served mix over L in [20, 60], D in {32, 64, 128} instead of the full ranges")
     ap.add_argument("--sweep-D", type=str, default="128,256",
@@ -488,6 +507,9 @@ def main():
             for r in sweep_leg(dev, [(f"heisenberg_xxz_L100_delta1_D{D}", dev.MODEL_HEISENBERG, D,
                                       [1.0, 1.0, 1.0, 0.0])], args.sweep_halfsweeps):
                 print(json.dumps(r), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched_form(dev, args.batched)), flush=True)
         return
     if args.served_only:
         print(json.dumps(served_form(dev, args.served, args.rate, not args.reduced_serving, args.served_convtol)),
