@@ -159,6 +159,74 @@ __global__ void __launch_bounds__(kT) lp_mix_kernel(const float2* __restrict__ T
   }
 }
 
+// Four values of pack8's layout: a_r = 8 g + 4 h + j (j < 4) of one row, stored at dst (the
+// row's four 16-byte units of group g), the half h of each unit that pack8 would write.
+template <int PREC>
+__device__ __forceinline__ void store_pack4(const float re[4], const float im[4], int h, uint4* dst) {
+  if constexpr (PREC == MPSD_PREC_FP32) {
+    dst[2 * h] = make_uint4(__float_as_uint(re[0]), __float_as_uint(re[1]), __float_as_uint(re[2]),
+                            __float_as_uint(re[3]));
+    dst[2 * h + 1] = make_uint4(__float_as_uint(im[0]), __float_as_uint(im[1]), __float_as_uint(im[2]),
+                                __float_as_uint(im[3]));
+  } else {
+    unsigned rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const unsigned short h0 = f2bf(re[2 * j]), h1 = f2bf(re[2 * j + 1]);
+      const unsigned short l0 = f2bf(re[2 * j] - bf2f(h0)), l1 = f2bf(re[2 * j + 1] - bf2f(h1));
+      const unsigned short g0 = f2bf(im[2 * j]), g1 = f2bf(im[2 * j + 1]);
+      const unsigned short m0 = f2bf(im[2 * j] - bf2f(g0)), m1 = f2bf(im[2 * j + 1] - bf2f(g1));
+      rh[j] = (unsigned)h0 | ((unsigned)h1 << 16);
+      rl[j] = (unsigned)l0 | ((unsigned)l1 << 16);
+      ih[j] = (unsigned)g0 | ((unsigned)g1 << 16);
+      il[j] = (unsigned)m0 | ((unsigned)m1 << 16);
+    }
+    reinterpret_cast<uint2*>(dst + 0)[h] = make_uint2(rh[0], rh[1]);
+    reinterpret_cast<uint2*>(dst + 1)[h] = make_uint2(rl[0], rl[1]);
+    reinterpret_cast<uint2*>(dst + 2)[h] = make_uint2(ih[0], ih[1]);
+    reinterpret_cast<uint2*>(dst + 3)[h] = make_uint2(il[0], il[1]);
+  }
+}
+
+// lp_mix_kernel's sums with one output per grid row (o = blockIdx.y) and half a group of a_r per
+// thread: 2 nout / 2 times its threads, for the small grids where each of its threads would
+// carry 16 sums.  The same fmaf sequence per value, so the same bits.
+template <int PREC>
+__global__ void __launch_bounds__(kT) lp_mix_split_kernel(const float2* __restrict__ T1, void* __restrict__ T2p,
+                                                          const double2* __restrict__ mix, int Db, int nin,
+                                                          int nout, int Dkr, int Kp) {
+  __shared__ float2 wm[256];
+  const int o = blockIdx.y;
+  for (int i = threadIdx.x; i < nin; i += kT) {
+    const double2 v = mix[o + nout * i];
+    wm[i] = float2{(float)v.x, (float)v.y};
+  }
+  __syncthreads();
+  const int ng = Kp / 8;
+  const int64_t total = (int64_t)Db * ng * 2;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int b = (int)(t % Db);
+    const int64_t rest = t / Db;
+    const int h = (int)(rest & 1);
+    const int g = (int)(rest >> 1);
+    float re[4], im[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) re[j] = im[j] = 0.f;
+    for (int i = 0; i < nin; ++i) {
+      const float2 w0 = wm[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int ar = 8 * g + 4 * h + j;
+        const float2 v = ar < Dkr ? T1[b + (int64_t)Db * i + (int64_t)Db * nin * ar] : float2{0.f, 0.f};
+        re[j] = fmaf(w0.x, v.x, fmaf(-w0.y, v.y, re[j]));
+        im[j] = fmaf(w0.x, v.y, fmaf(w0.y, v.x, im[j]));
+      }
+    }
+    uint4* dst = reinterpret_cast<uint4*>(T2p) + (int64_t)o * Db * Kp / 2 + ((int64_t)g * Db + b) * 4;
+    store_pack4<PREC>(re, im, h, dst);
+  }
+}
+
 // y[i] = sum_{q < np} P[q * stride + i] (FP32 partials, summed in FP64 in a fixed order) + beta x[i]
 __global__ void __launch_bounds__(kT) lp_sum_kernel(const float2* __restrict__ P, int np, int64_t stride,
                                                     const double2* __restrict__ x, double beta,
@@ -410,6 +478,7 @@ __global__ void __launch_bounds__(kT) pk64_pack_kernel(const LpPackDesc p) {
 // writes 1 KB contiguous per output.
 constexpr int kMixMax = 16;
 constexpr unsigned kMixSplitBelow = 512;  // one-row grids smaller than this split by output
+constexpr unsigned kLpMixSplitBelow = 128;  // the same for the low-precision mix
 __global__ void __launch_bounds__(kT) pk64_mix_kernel(const double2* __restrict__ T1, double2* __restrict__ T2p,
                                                       const double2* __restrict__ mix, int Db, int nin, int nout,
                                                       int Dkr, int Kp) {
@@ -685,6 +754,20 @@ hipError_t launch_lp_mix(const float2* T1, void* T2p, const double2* mix, int Db
   const int64_t total = (int64_t)Db * (Kp / 8) * ((nout + 1) / 2);
   if (total <= 0) return hipSuccess;
   if (nin * nout > 256 || Kp % 32 != 0) return hipErrorInvalidValue;
+  // grids under kLpMixSplitBelow workgroups (D <= 256 at w = 3) run the split form: measured
+  // 10-18% faster matvecs at D = 64-128, slower ones at D = 400-512 (its loads repeat per output)
+  if (grid_for(total) < kLpMixSplitBelow) {
+    const int64_t total_split = (int64_t)Db * (Kp / 8) * 2;
+    if (prec == MPSD_PREC_BF16X3)
+      hipLaunchKernelGGL(lp_mix_split_kernel<MPSD_PREC_BF16X3>, dim3(grid_for(total_split), nout), dim3(kT), 0, s, T1,
+                         T2p, mix, Db, nin, nout, Dkr, Kp);
+    else if (prec == MPSD_PREC_FP32)
+      hipLaunchKernelGGL(lp_mix_split_kernel<MPSD_PREC_FP32>, dim3(grid_for(total_split), nout), dim3(kT), 0, s, T1,
+                         T2p, mix, Db, nin, nout, Dkr, Kp);
+    else
+      return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   if (prec == MPSD_PREC_BF16X3)
     hipLaunchKernelGGL(lp_mix_kernel<MPSD_PREC_BF16X3>, dim3(grid_for(total)), dim3(kT), 0, s, T1, T2p, mix, Db, nin,
                        nout, Dkr, Kp);
