@@ -78,32 +78,62 @@ __device__ __forceinline__ void pack8(const float re[8], const float im[8], uint
   }
 }
 
-// FP64 strided source -> packed operand.  One thread per (batch, row, group of 8 k), rows
-// fastest (the environments are row-contiguous in HBM).
+// Four values of pack8's layout: a_r = 8 g + 4 h + j (j < 4) of one row, stored at dst (the
+// row's four 16-byte units of group g), the half h of each unit that pack8 would write.
+template <int PREC>
+__device__ __forceinline__ void store_pack4(const float re[4], const float im[4], int h, uint4* dst) {
+  if constexpr (PREC == MPSD_PREC_FP32) {
+    dst[2 * h] = make_uint4(__float_as_uint(re[0]), __float_as_uint(re[1]), __float_as_uint(re[2]),
+                            __float_as_uint(re[3]));
+    dst[2 * h + 1] = make_uint4(__float_as_uint(im[0]), __float_as_uint(im[1]), __float_as_uint(im[2]),
+                                __float_as_uint(im[3]));
+  } else {
+    unsigned rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const unsigned short h0 = f2bf(re[2 * j]), h1 = f2bf(re[2 * j + 1]);
+      const unsigned short l0 = f2bf(re[2 * j] - bf2f(h0)), l1 = f2bf(re[2 * j + 1] - bf2f(h1));
+      const unsigned short g0 = f2bf(im[2 * j]), g1 = f2bf(im[2 * j + 1]);
+      const unsigned short m0 = f2bf(im[2 * j] - bf2f(g0)), m1 = f2bf(im[2 * j + 1] - bf2f(g1));
+      rh[j] = (unsigned)h0 | ((unsigned)h1 << 16);
+      rl[j] = (unsigned)l0 | ((unsigned)l1 << 16);
+      ih[j] = (unsigned)g0 | ((unsigned)g1 << 16);
+      il[j] = (unsigned)m0 | ((unsigned)m1 << 16);
+    }
+    reinterpret_cast<uint2*>(dst + 0)[h] = make_uint2(rh[0], rh[1]);
+    reinterpret_cast<uint2*>(dst + 1)[h] = make_uint2(rl[0], rl[1]);
+    reinterpret_cast<uint2*>(dst + 2)[h] = make_uint2(ih[0], ih[1]);
+    reinterpret_cast<uint2*>(dst + 3)[h] = make_uint2(il[0], il[1]);
+  }
+}
+
+// FP64 strided source -> packed operand.  One thread per (batch, row, group of 8 k, half h of
+// the group), h fastest, then rows (the environments are row-contiguous in HBM).  Half groups
+// rather than whole ones: twice the threads, which made the per-call pack of x 0.7-2.6 us
+// faster at D = 64-512 (the same conversions, the same packed bits).
 template <int PREC>
 __global__ void __launch_bounds__(kT) lp_pack_kernel(const LpPackDesc p) {
   const int64_t ng = p.Kp / 8;
-  const int64_t total = (int64_t)p.nb * p.rows * ng;
+  const int64_t total = (int64_t)p.nb * p.rows * ng * 2;
   const bool row_fast = p.srow == 1;
   for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
-    const int b = (int)(t / ((int64_t)p.rows * ng));
-    const int64_t r = t - (int64_t)b * p.rows * ng;
+    const int64_t t2 = t >> 1;
+    const int h = (int)(t & 1);
+    const int b = (int)(t2 / ((int64_t)p.rows * ng));
+    const int64_t r = t2 - (int64_t)b * p.rows * ng;
     const int row = row_fast ? (int)(r % p.rows) : (int)(r / ng);
     const int g = row_fast ? (int)(r / p.rows) : (int)(r % ng);
     const double2* src = p.src + (int64_t)b * p.sb + (int64_t)row * p.srow;
-    float re[8], im[8];
+    float re[4], im[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 8 * g + j;
+    for (int j = 0; j < 4; ++j) {
+      const int k = 8 * g + 4 * h + j;
       const double2 v = k < p.K ? src[(int64_t)k * p.sk] : double2{0.0, 0.0};
       re[j] = (float)v.x;
       im[j] = (float)v.y;
     }
-    uint4 out[4];
-    pack8<PREC>(re, im, out);
     uint4* dst = reinterpret_cast<uint4*>(p.dst) + (((int64_t)b * p.rows + row) * p.Kp + 8 * g) / 2;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) dst[u] = out[u];
+    store_pack4<PREC>(re, im, h, dst);
   }
 }
 
@@ -156,35 +186,6 @@ __global__ void __launch_bounds__(kT) lp_mix_kernel(const float2* __restrict__ T
 #pragma unroll
       for (int u = 0; u < 4; ++u) dst[u] = out[u];
     }
-  }
-}
-
-// Four values of pack8's layout: a_r = 8 g + 4 h + j (j < 4) of one row, stored at dst (the
-// row's four 16-byte units of group g), the half h of each unit that pack8 would write.
-template <int PREC>
-__device__ __forceinline__ void store_pack4(const float re[4], const float im[4], int h, uint4* dst) {
-  if constexpr (PREC == MPSD_PREC_FP32) {
-    dst[2 * h] = make_uint4(__float_as_uint(re[0]), __float_as_uint(re[1]), __float_as_uint(re[2]),
-                            __float_as_uint(re[3]));
-    dst[2 * h + 1] = make_uint4(__float_as_uint(im[0]), __float_as_uint(im[1]), __float_as_uint(im[2]),
-                                __float_as_uint(im[3]));
-  } else {
-    unsigned rh[2], rl[2], ih[2], il[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const unsigned short h0 = f2bf(re[2 * j]), h1 = f2bf(re[2 * j + 1]);
-      const unsigned short l0 = f2bf(re[2 * j] - bf2f(h0)), l1 = f2bf(re[2 * j + 1] - bf2f(h1));
-      const unsigned short g0 = f2bf(im[2 * j]), g1 = f2bf(im[2 * j + 1]);
-      const unsigned short m0 = f2bf(im[2 * j] - bf2f(g0)), m1 = f2bf(im[2 * j + 1] - bf2f(g1));
-      rh[j] = (unsigned)h0 | ((unsigned)h1 << 16);
-      rl[j] = (unsigned)l0 | ((unsigned)l1 << 16);
-      ih[j] = (unsigned)g0 | ((unsigned)g1 << 16);
-      il[j] = (unsigned)m0 | ((unsigned)m1 << 16);
-    }
-    reinterpret_cast<uint2*>(dst + 0)[h] = make_uint2(rh[0], rh[1]);
-    reinterpret_cast<uint2*>(dst + 1)[h] = make_uint2(rl[0], rl[1]);
-    reinterpret_cast<uint2*>(dst + 2)[h] = make_uint2(ih[0], ih[1]);
-    reinterpret_cast<uint2*>(dst + 3)[h] = make_uint2(il[0], il[1]);
   }
 }
 
@@ -741,9 +742,9 @@ hipError_t launch_lp_pack(const LpPackDesc& p, int prec, hipStream_t s) {
   if (prec == MPSD_PREC_FP64)
     hipLaunchKernelGGL(pk64_pack_kernel, dim3(grid_for(total)), dim3(kT), 0, s, p);
   else if (prec == MPSD_PREC_BF16X3)
-    hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_BF16X3>, dim3(grid_for(total)), dim3(kT), 0, s, p);
+    hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_BF16X3>, dim3(grid_for(2 * total)), dim3(kT), 0, s, p);
   else if (prec == MPSD_PREC_FP32)
-    hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_FP32>, dim3(grid_for(total)), dim3(kT), 0, s, p);
+    hipLaunchKernelGGL(lp_pack_kernel<MPSD_PREC_FP32>, dim3(grid_for(2 * total)), dim3(kT), 0, s, p);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
