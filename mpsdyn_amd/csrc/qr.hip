@@ -31,25 +31,19 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return {a.x * b.
 // conj(a) * b
 __device__ __forceinline__ double2 cjmul(double2 a, double2 b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }
 
-// Workgroup sum of NV complex values per thread, fixed order; out[u] visible to all threads
-// on return.
-template <int NV, int NW>
-__device__ __forceinline__ void block_sum(const double2 (&v)[NV], double2 (*red)[NV], double2* out) {
+// Workgroup sum of one real value per thread (a squared norm), fixed order: out[0].x, with
+// out[0].y = 0, visible to all threads on return.
+template <int NW>
+__device__ __forceinline__ void block_sum_real(double v, double2 (*red)[1], double2* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int u = 0; u < NV; ++u) {
-    const double x = wave_sum_x(v[u].x), y = wave_sum_x(v[u].y);
-    if (lane == 0) red[wave][u] = double2{x, y};
-  }
+  const double x = wave_sum_x(v);
+  if (lane == 0) red[wave][0] = double2{x, 0.0};
   __syncthreads();
-  if (threadIdx.x < NV) {
-    double2 s{0.0, 0.0};
+  if (threadIdx.x == 0) {
+    double sm = 0.0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      s.x += red[w][threadIdx.x].x;
-      s.y += red[w][threadIdx.x].y;
-    }
-    out[threadIdx.x] = s;
+    for (int w = 0; w < NW; ++w) sm += red[w][0].x;
+    out[0] = double2{sm, 0.0};
   }
   __syncthreads();
 }
@@ -136,11 +130,11 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
     if (t < pb) {
       const int c = j0 + t;
       // squared norm of the column below the diagonal
-      double2 s1[1] = {{0.0, 0.0}};
+      double s1 = 0.0;
 #pragma unroll
       for (int i = 0; i < RPT; ++i)
-        if (row[i] > c) s1[0].x += a[i][t].x * a[i][t].x + a[i][t].y * a[i][t].y;
-      block_sum<1, NW>(s1, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
+        if (row[i] > c) s1 += a[i][t].x * a[i][t].x + a[i][t].y * a[i][t].y;
+      block_sum_real<NW>(s1, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
       // A column whose squares underflow or overflow is rescaled by its largest entry first (the
       // role of zlarfg's safmin loop and of dznrm2's scaling); the test reads shared memory, so
       // every thread takes the branch.
@@ -151,16 +145,16 @@ __global__ void __launch_bounds__(NT) qr_panel_kernel(double2* __restrict__ A, i
         for (int i = 0; i < RPT; ++i)
           if (row[i] >= c) mx = fmax(mx, fmax(fabs(a[i][t].x), fabs(a[i][t].y)));
         S = block_max<NW>(mx, &red[0][0].x);
-        double2 s2[1] = {{0.0, 0.0}};
+        double s2 = 0.0;
         if (S > 0.0) {
 #pragma unroll
           for (int i = 0; i < RPT; ++i)
             if (row[i] > c) {
               const double xr = a[i][t].x / S, xi = a[i][t].y / S;
-              s2[0].x += xr * xr + xi * xi;
+              s2 += xr * xr + xi * xi;
             }
         }
-        block_sum<1, NW>(s2, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
+        block_sum_real<NW>(s2, reinterpret_cast<double2(*)[1]>(&red[0][0]), sums);
         xs2 = sums[0].x;
         if (S == 0.0) S = 1.0;
       }
