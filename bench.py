@@ -371,10 +371,10 @@ def batched_form(dev, n):
                      "failed": sum(1 for r in res if r.status != 0)}
     out["mixed_vs_fp64_max_rel_energy_diff"] = float(max(abs(a - b) / abs(a) for a, b in
                                                           zip(energies["fp64"], energies["mixed_fp32"])))
-    # the same FP64 solves with the slots' matvecs merged into grouped persistent launches, and
-    # those launches replayed from a captured hipGraph (slower here; reported beside the plain
-    # streams, with the energies' agreement)
-    for name, kw in (("fp64_grouped", {"batching": True}), ("fp64_grouped_graph", {"batching": True, "graph": True})):
+    # the same FP64 solves with the slots' matvecs merged into grouped persistent launches
+    # (slower here), and with each slot's Lanczos steps replayed from captured hipGraphs;
+    # reported beside the plain streams, with the energies' agreement
+    for name, kw in (("fp64_grouped", {"batching": True}), ("fp64_graph", {"batching": False, "graph": True})):
         cfgs = [dev.gs_config(dev.MODEL_ISING, L, D, p, convtol=1e-10, seed=k + 1, precision=0, tol_schedule=True)
                 for k, (L, D, p) in enumerate(specs)]
         srv = dev.Server(slots=16, **kw)
@@ -389,7 +389,7 @@ def batched_form(dev, n):
                                                               for r, e in zip(res, energies["fp64"])))}
     out["note"] = ("fp64 with the local-tolerance schedule; each request runs on its own stream (16 slots); "
                    "fp64_grouped: matvecs of the slots merged into grouped persistent launches; "
-                   "fp64_grouped_graph: those launches replayed from a captured hipGraph")
+                   "fp64_graph: each slot's Lanczos steps replayed from captured hipGraphs")
     return out
 
 

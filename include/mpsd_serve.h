@@ -29,7 +29,9 @@ typedef struct mpsd_serve_config {
   int64_t hbm_budget; /* bytes the admitted solves may hold together (0: 80% of free HBM) */
   int batching;       /* 1: merge concurrent matvecs into grouped launches */
   int window_us;      /* longest wait of a matvec for others to join its group */
-  int graph;          /* 1: replay the grouped matvec launch sequence as a captured hipGraph */
+  int graph;          /* 1: the workers' eigensolver steps replayed from captured hipGraphs
+                         (MPSD_OPT_GRAPH on every worker context; solves whose matvecs go
+                         through the batcher keep stream launches) */
 } mpsd_serve_config;
 
 typedef struct mpsd_serve_result {
@@ -53,7 +55,7 @@ typedef struct mpsd_serve_stats {
   int64_t max_batch;
   int64_t peak_admitted_bytes;
   int peak_running;
-  int64_t graph_replays;
+  int64_t graph_replays;    /* captured eigensolver-step graphs launched by the workers */
 } mpsd_serve_stats;
 
 /* HBM footprint estimate of one solve (bytes). */

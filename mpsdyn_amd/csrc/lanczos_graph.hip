@@ -347,6 +347,7 @@ int mpsd_lanczos_steps(mpsd_lanczos* lz, int k, double* ab_host, int* j_end) {
   hipError_t e = hipGraphLaunch(ex, lz->ctx->stream);
   if (e != hipSuccess) return lz_err(lz, e, "graph launch");
   ++lz->replays;
+  ++lz->ctx->graph_launches;
   lz->jn += k;
   *j_end = lz->jn;
   return mpsd_copy_to_host(lz->ctx, lz->ab, ab_host, (int64_t)sizeof(double) * (2 + 2 * (*j_end)));

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <atomic>
 #include <map>
 #include <string>
 #include <tuple>
@@ -190,7 +191,7 @@ struct mpsd_ctx {
   // Scratch (grown on demand, never shrunk): 0, 1 = the two intermediates of a contraction;
   // 2 = Householder scalars of a gauge step; 3 = copy/permute buffer of a gauge step;
   // 4, 5 = intermediates (T1 + partials, T2) of a grouped matvec; 6 = Householder QR workspace
-  // (V, Z, Y, W); 7 = second gauge buffer; 8 = Krylov basis of the one-workgroup eigensolver;
+  // (V, T, chunk partials); 7 = second gauge buffer; 8 = Krylov basis of the one-workgroup eigensolver;
   // 9, 10, 11 = packed low-precision L, R and x of a precision-form matvec.
   static constexpr int kScratch = 12;
   void* scratch[kScratch] = {};
@@ -215,6 +216,7 @@ struct mpsd_ctx {
   void* dots = nullptr;
   size_t dots_bytes = 0;
   void* gs_partial = nullptr;  // second-pass block partials of the fused Gram-Schmidt step
+  std::atomic<int64_t> graph_launches{0};  // captured eigensolver-step graphs launched on the stream
   size_t gs_partial_bytes = 0;
   void* coef = nullptr;       // device ring, 8 slots x 1024 complex
   void* coef_host = nullptr;  // pinned host ring, same layout

@@ -11,8 +11,8 @@
 //   it reduces alongside, and writes V and -T.
 // * The trailing columns take C <- (I - V T^H V^H) C = C + V ((-T)^H (V^H C)), and Q is
 //   accumulated backwards, Q <- (I - V T V^H) Q = Q + V ((-T) (V^H Q)).  V^H C (16 rows, long K)
-//   is a chunked product whose ordered chunk sum also applies the 16 x 16 factor; the rank-16
-//   update is a BRGEMM launch.
+//   is a chunked product (one launch of chunk partials); a second launch sums the chunks in
+//   order, applies the 16 x 16 factor and adds the rank-16 product to C, tile by tile.
 //
 // A panel step is a handful of launches where rocSOLVER's blocked QR issues several per
 // column, which is what the small and medium gauge steps of a sweep are bound by.
@@ -622,68 +622,71 @@ __global__ void __launch_bounds__(256) vh_partial_kernel(const double2* __restri
   if (a < pb && jb + j < cols) P[((int64_t)blockIdx.y * cols + jb + j) * kPB + a] = acc;
 }
 
-// W = op(Tn) (V^H C): the chunk partials of V^H C summed in chunk order, then multiplied by the
-// panel's -T (op = conjugate transpose for the trailing columns, none for Q).  One workgroup per
-// 16 columns of W; thread (a, jj) owns W[a][j0 + jj].
-__global__ void __launch_bounds__(256) vh_reduce_kernel(const double2* __restrict__ P, int nch, int pb, int cols,
-                                                        const double2* __restrict__ Tg, int conjT,
-                                                        double2* __restrict__ W) {
+// C += V (op(Tn) (V^H C)) from the chunk partials of V^H C, in one launch after vh_partial_kernel
+// (in place of vh_reduce_kernel and the rank-16 BRGEMM): workgroup (column block, 64-row tile)
+// first forms its 16 x 16 block of W exactly as vh_reduce_kernel does (chunk sums in order, then
+// the factor), then updates its 64 x 16 tile of C with the rank-pb product, summed over the
+// panel's columns in order and added to C.  Every row tile repeats its column block's W (a few
+// KB from L2).
+constexpr int kVR = 64;  // rows of C per workgroup
+__global__ void __launch_bounds__(256) vh_apply_kernel(const double2* __restrict__ V, int ldv, int rows, int pb,
+                                                       double2* __restrict__ C, int ldc, int cols,
+                                                       const double2* __restrict__ P, int nch,
+                                                       const double2* __restrict__ Tg, int conjT) {
   __shared__ double2 Ws[kPB][kVJ + 1];
+  __shared__ double2 Wf[kPB][kVJ + 1];
   __shared__ double2 Ts[kPB][kPB + 1];
-  const int tid = threadIdx.x, a = tid % kPB, jj = tid / kPB, j = blockIdx.x * kVJ + jj;
+  __shared__ double2 Vs[kVR][kPB + 1];
+  const int tid = threadIdx.x, a = tid % kPB, jj = tid / kPB;
+  const int jb = blockIdx.x * kVJ, j = jb + jj, r0 = blockIdx.y * kVR;
   Ts[tid / kPB][tid % kPB] = Tg[tid];
-  double2 s{0.0, 0.0};
+  double2 sm{0.0, 0.0};
   if (a < pb && j < cols)
     for (int ch = 0; ch < nch; ++ch) {
       const double2 v = P[((int64_t)ch * cols + j) * kPB + a];
-      s.x += v.x;
-      s.y += v.y;
+      sm.x += v.x;
+      sm.y += v.y;
     }
-  Ws[a][jj] = s;
-  __syncthreads();
-  if (a >= pb || j >= cols) return;
-  double2 o{0.0, 0.0};
-  for (int b = 0; b < pb; ++b) {
-    const double2 p = conjT ? cjmul(Ts[b][a], Ws[b][jj]) : cmul(Ts[a][b], Ws[b][jj]);
-    o.x += p.x;
-    o.y += p.y;
+  Ws[a][jj] = sm;
+  for (int e = tid; e < kVR * kPB; e += 256) {
+    const int r = e % kVR, c = e / kVR;
+    Vs[r][c] = (r0 + r < rows && c < pb) ? V[(r0 + r) + (int64_t)ldv * c] : double2{0.0, 0.0};
   }
-  W[a + (int64_t)pb * j] = o;
+  __syncthreads();
+  {
+    double2 o{0.0, 0.0};
+    if (a < pb && j < cols)
+      for (int b = 0; b < pb; ++b) {
+        const double2 p = conjT ? cjmul(Ts[b][a], Ws[b][jj]) : cmul(Ts[a][b], Ws[b][jj]);
+        o.x += p.x;
+        o.y += p.y;
+      }
+    Wf[a][jj] = o;
+  }
+  __syncthreads();
+  for (int e = tid; e < kVR * kVJ; e += 256) {
+    const int r = e % kVR, c = e / kVR;
+    if (r0 + r >= rows || jb + c >= cols) continue;
+    double2 t{0.0, 0.0};
+    for (int b = 0; b < pb; ++b) {
+      const double2 p = cmul(Vs[r][b], Wf[b][c]);
+      t.x += p.x;
+      t.y += p.y;
+    }
+    double2* cp = C + (r0 + r) + (int64_t)ldc * (jb + c);
+    const double2 cv = *cp;
+    *cp = double2{cv.x + t.x, cv.y + t.y};
+  }
 }
 
-// W (pb x cols, ld pb) = op(Tn) V^H C, V rows x pb (ld ldv), C rows x cols (ld ldc), Tn the
-// panel's -T (kPB x kPB, row-major); P: partials.
-hipError_t vh_times(const double2* V, int ldv, int rows, int pb, const double2* C, int ldc, int cols,
-                    const double2* Tn, int conjT, double2* W, double2* P, hipStream_t s) {
-  const int nch = (rows + kVC - 1) / kVC;
-  hipLaunchKernelGGL(vh_partial_kernel, dim3((cols + kVJ - 1) / kVJ, nch), dim3(256), 0, s, V, ldv, rows, pb, C, ldc,
-                     cols, P);
-  hipLaunchKernelGGL(vh_reduce_kernel, dim3((cols + kVJ - 1) / kVJ), dim3(256), 0, s, P, nch, pb, cols, Tn, conjT, W);
+// C += V op(Tn) V^H C for a panel: chunk partials, then vh_apply_kernel.  P: partials.
+hipError_t block_reflect(const double2* V, int ldv, int rows, int pb, double2* C, int ldc, int cols,
+                         const double2* Tn, int conjT, double2* P, hipStream_t s) {
+  const int nch = (rows + kVC - 1) / kVC, ncb = (cols + kVJ - 1) / kVJ;
+  hipLaunchKernelGGL(vh_partial_kernel, dim3(ncb, nch), dim3(256), 0, s, V, ldv, rows, pb, C, ldc, cols, P);
+  hipLaunchKernelGGL(vh_apply_kernel, dim3(ncb, (rows + kVR - 1) / kVR), dim3(256), 0, s, V, ldv, rows, pb, C, ldc,
+                     cols, (const double2*)P, nch, Tn, conjT);
   return hipGetLastError();
-}
-
-hipError_t plus_times(const double2* P, int ldp, int rows, int pb, const double2* W, double2* C, int ldc, int cols,
-                      hipStream_t s) {
-  // C += P W (in place: each output element reads its own Dadd value first)
-  ZgemmDesc g;
-  g.A = P;
-  g.B = W;
-  g.C = C;
-  g.Dadd = C;
-  g.beta = 1.0;
-  g.M = rows;
-  g.N = cols;
-  g.K = pb;
-  g.NB = 1;
-  g.sAm = 1;
-  g.sAk = ldp;
-  g.sBk = 1;
-  g.sBn = pb;
-  g.sCm = 1;
-  g.sCn = ldc;
-  g.a_mfast = 1;
-  g.b_kfast = 1;
-  return launch_zbrgemm_f64(g, TileCfg{32, 32}, s);
 }
 
 }  // namespace
@@ -718,13 +721,12 @@ int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
   const int k = std::min(m, n);
   const int64_t mk = (int64_t)m * k;
   const int64_t wn = std::max(n, k), nch = (m + kVC - 1) / kVC, npan = (k + kPB - 1) / kPB;
-  const size_t need = sizeof(double2) * (size_t)(mk + npan * kPB * kPB + kPB * wn * (1 + nch));
+  const size_t need = sizeof(double2) * (size_t)(mk + npan * kPB * kPB + kPB * wn * nch);
   int rc = ensure_scratch(ctx, 6, need);
   if (rc) return rc;
   double2* V = (double2*)ctx->scratch[6];
   double2* T = V + mk;                  // -T of panel p at T + p kPB^2
-  double2* W = T + npan * kPB * kPB;
-  double2* P = W + kPB * wn;            // chunk partials of V^H C
+  double2* P = T + npan * kPB * kPB;    // chunk partials of V^H C
   hipStream_t s = ctx->stream;
   hipError_t e = hipSuccess;
   for (int j0 = 0; j0 < k && e == hipSuccess; j0 += kPB) {
@@ -743,8 +745,7 @@ int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
     const int cols = n - j0 - pb;
     if (e == hipSuccess && cols > 0) {
       double2* C = A + j0 + (int64_t)m * (j0 + pb);
-      e = vh_times(Vp, m, rows, pb, C, m, cols, Tp, 1, W, P, s);
-      if (e == hipSuccess) e = plus_times(Vp, m, rows, pb, W, C, m, cols, s);
+      e = block_reflect(Vp, m, rows, pb, C, m, cols, Tp, 1, P, s);
     }
   }
   if (e == hipSuccess) {
@@ -759,8 +760,7 @@ int house_qr(mpsd_ctx* ctx, double2* A, int m, int n, double2* Q, double2* R) {
     double2* Qs = Q + j0 + (int64_t)m * j0;
     const double2* Vp = V + j0 + (int64_t)m * j0;
     const double2* Tp = T + (int64_t)(j0 / kPB) * kPB * kPB;
-    e = vh_times(Vp, m, rows, pb, Qs, m, cols, Tp, 0, W, P, s);
-    if (e == hipSuccess) e = plus_times(Vp, m, rows, pb, W, Qs, m, cols, s);
+    e = block_reflect(Vp, m, rows, pb, Qs, m, cols, Tp, 0, P, s);
   }
   return hip_err(ctx, e, "house_qr");
 }

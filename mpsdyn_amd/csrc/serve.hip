@@ -36,7 +36,7 @@ typedef std::chrono::steady_clock Clock;
 // ------------------------------------------------------------------------------ batcher
 class Batcher {
  public:
-  Batcher(mpsd_ctx* ctx, int window_us, bool graph) : ctx_(ctx), window_us_(window_us), graph_(graph) {
+  Batcher(mpsd_ctx* ctx, int window_us) : ctx_(ctx), window_us_(window_us) {
     th_ = std::thread([this] { run(); });
   }
   ~Batcher() {
@@ -152,7 +152,6 @@ class Batcher {
   }
   mpsd_ctx* ctx_;
   int window_us_;
-  bool graph_;
   std::thread th_;
   std::mutex mu_, ev_mu_;
   std::condition_variable cv_in_, cv_out_;
@@ -323,6 +322,12 @@ int mpsd_server_create(const mpsd_serve_config* cfg, mpsd_server** out) {
       return rc;
     }
     s->wctx.push_back(c);
+    // eigensolver steps replayed from captured graphs (used where a solve's matvec is its own
+    // launch sequence, i.e. without the batcher)
+    if (cfg->graph && (rc = mpsd_ctx_set_option(c, MPSD_OPT_GRAPH, 1)) != 0) {
+      for (auto* w : s->wctx) mpsd_ctx_destroy(w);
+      return rc;
+    }
   }
   if (cfg->batching) {
     int rc = mpsd_ctx_create(cfg->device, &s->bctx);
@@ -330,7 +335,7 @@ int mpsd_server_create(const mpsd_serve_config* cfg, mpsd_server** out) {
       for (auto* w : s->wctx) mpsd_ctx_destroy(w);
       return rc;
     }
-    s->batcher.reset(new Batcher(s->bctx, s->cfg.window_us, cfg->graph != 0));
+    s->batcher.reset(new Batcher(s->bctx, s->cfg.window_us));
   }
   mpsd_server* raw = s.get();
   for (int i = 0; i < cfg->slots; ++i) raw->workers.emplace_back([raw, i] { raw->worker(i); });
@@ -418,8 +423,9 @@ int mpsd_server_stats(const mpsd_server* s, mpsd_serve_stats* out) {
     out->batches = b.batches;
     out->batched_matvecs = b.batched_matvecs;
     out->max_batch = b.max_batch;
-    out->graph_replays = b.graph_replays;
   }
+  out->graph_replays = 0;
+  for (mpsd_ctx* c : m->wctx) out->graph_replays += c->graph_launches.load();
   return MPSD_OK;
 }
 

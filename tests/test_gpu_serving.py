@@ -113,6 +113,25 @@ def test_large_request_served_equals_alone(ctx, batching):
     assert st.peak_running >= 2  # the solves overlapped on the device
 
 
+def test_served_graph_steps_equal_alone(ctx):
+    """Served with graph = 1 (each worker's Lanczos steps replayed from captured hipGraphs, no
+    batcher), every request gives the energy of the same solve alone on stream launches (1e-12
+    relative) and the workers did launch captured graphs."""
+    from mpsdyn_amd import device as dev
+    cfgs = _cfgs(dev)[:5]
+    alone = [dev.gs_solve(ctx, c).energy for c in cfgs]
+    srv = dev.Server(slots=len(cfgs), batching=False, graph=True)
+    res = srv.run_trace(cfgs, [0.0] * len(cfgs))
+    st = srv.stats()
+    srv.close()
+    print(f"graph_replays={st.graph_replays} peak_running={st.peak_running}", flush=True)
+    for a, r in zip(alone, res):
+        assert r.status == 0
+        print(f"alone={a:.16f} served={r.energy:.16f} bitwise={a == r.energy}", flush=True)
+        assert abs(a - r.energy) <= 1e-12 * abs(a)
+    assert st.graph_replays > 0
+
+
 def test_admission_respects_budget():
     """With a budget that holds two of the requests, never more than two run at once and
     the admitted footprint stays within the budget; later arrivals wait, then finish."""
