@@ -652,6 +652,14 @@ __global__ void __launch_bounds__(256) vh_apply_kernel(const double2* __restrict
     const int r = e % kVR, c = e / kVR;
     Vs[r][c] = (r0 + r < rows && c < pb) ? V[(r0 + r) + (int64_t)ldv * c] : double2{0.0, 0.0};
   }
+  // this thread's elements of C, loaded now so their latency overlaps the factor product
+  constexpr int kPer = kVR * kVJ / 256;
+  double2 cv[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = tid + 256 * q, r = e % kVR, c = e / kVR;
+    cv[q] = (r0 + r < rows && jb + c < cols) ? C[(r0 + r) + (int64_t)ldc * (jb + c)] : double2{0.0, 0.0};
+  }
   __syncthreads();
   {
     double2 o{0.0, 0.0};
@@ -664,8 +672,9 @@ __global__ void __launch_bounds__(256) vh_apply_kernel(const double2* __restrict
     Wf[a][jj] = o;
   }
   __syncthreads();
-  for (int e = tid; e < kVR * kVJ; e += 256) {
-    const int r = e % kVR, c = e / kVR;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = tid + 256 * q, r = e % kVR, c = e / kVR;
     if (r0 + r >= rows || jb + c >= cols) continue;
     double2 t{0.0, 0.0};
     for (int b = 0; b < pb; ++b) {
@@ -673,9 +682,7 @@ __global__ void __launch_bounds__(256) vh_apply_kernel(const double2* __restrict
       t.x += p.x;
       t.y += p.y;
     }
-    double2* cp = C + (r0 + r) + (int64_t)ldc * (jb + c);
-    const double2 cv = *cp;
-    *cp = double2{cv.x + t.x, cv.y + t.y};
+    C[(r0 + r) + (int64_t)ldc * (jb + c)] = double2{cv[q].x + t.x, cv[q].y + t.y};
   }
 }
 
